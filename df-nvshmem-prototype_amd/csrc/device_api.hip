@@ -1,0 +1,179 @@
+/*
+ * device_api.hip -- the device API (reference nvshmem_device.cu:431-584) and its
+ * cooperative extensions, compiled as relocatable device code (-fgpu-rdc) into
+ * libnvshmem.a / libnvshmem_amd.so.
+ *
+ * Differences from the reference that keep its contract but not its mechanism:
+ *  - p/g use one agent-scope atomic store/load instead of a volatile access;
+ *  - put/get copy with the widest access the relative alignment allows
+ *    (16 B when target and source agree mod 16), four loads in flight, instead
+ *    of one volatile 4/8-byte access per element (nvshmem_device.cu:497-500);
+ *  - quiet is an agent-scope fence: all PEs share one agent;
+ *  - waits poll with s_sleep back-off and give up after a wall-clock budget,
+ *    recording NVSHMEMI_ERR_WAIT_TIMEOUT, where the reference spins forever
+ *    reading %globaltimer (nvshmem_device.cu:551-553).
+ */
+#include "nvshmem_device.h"
+#include "nvshmemx_device.h"
+#include "nvshmemi_device.h"
+#include "nvshmemi_trace.h"
+
+/* the single device-state symbol, uploaded by the host runtime at start-up */
+__constant__ nvshmemi_device_state_t nvshmemi_dstate;
+
+using namespace nvshmemi;
+
+/* ------------------------------------------------------------ quiet -- */
+
+extern "C" __device__ void nvshmem_quiet(void) {
+    /* reference nvshmem_device.cu:431-436 */
+    fence_acq_rel();
+}
+
+/* ------------------------------------------------------------- p / g -- */
+
+#define NVSHMEMI_DEF_P(Name, Type, OpType)                                            \
+    extern "C" __device__ void nvshmem_##Name##_p(Type *target, const Type value, int pe) { \
+        NVSHMEMI_TRACE(OpType, (const void *)0, target, 1, pe, 0);                   \
+        store_relaxed((Type *)translate(target, pe), value);                          \
+    }
+NVSHMEMI_DEF_P(int, int, NVSHMEMI_OP_PUT_INT)
+NVSHMEMI_DEF_P(float, float, NVSHMEMI_OP_PUT_FLOAT)
+NVSHMEMI_DEF_P(double, double, NVSHMEMI_OP_PUT_DOUBLE)
+
+#define NVSHMEMI_DEF_G(Name, Type, OpType)                                            \
+    extern "C" __device__ Type nvshmem_##Name##_g(Type *source, int pe) {               \
+        NVSHMEMI_TRACE(OpType, (const void *)0, source, 1, pe, 0);                   \
+        return load_relaxed((const Type *)translate(source, pe));                     \
+    }
+NVSHMEMI_DEF_G(int, int, NVSHMEMI_OP_GET_INT)
+NVSHMEMI_DEF_G(float, float, NVSHMEMI_OP_GET_FLOAT)
+NVSHMEMI_DEF_G(double, double, NVSHMEMI_OP_GET_DOUBLE)
+
+/* --------------------------------------------------- per-thread bulk -- */
+
+#define NVSHMEMI_DEF_PUT(Name, Type, OpType)                                                \
+    extern "C" __device__ void nvshmem_##Name##_put(Type *target, Type *source, int len, int pe) { \
+        NVSHMEMI_TRACE(OpType, source, target, len, pe, 0);                                \
+        if (len <= 0) return;                                                               \
+        copy<16, 4, false>(translate(target, pe), source, (size_t)len * sizeof(Type), 0, 1);   \
+    }
+NVSHMEMI_DEF_PUT(int, int, NVSHMEMI_OP_PUT_INT)
+NVSHMEMI_DEF_PUT(float, float, NVSHMEMI_OP_PUT_FLOAT)
+NVSHMEMI_DEF_PUT(double, double, NVSHMEMI_OP_PUT_DOUBLE)
+
+#define NVSHMEMI_DEF_GET(Name, Type, OpType)                                                \
+    extern "C" __device__ void nvshmem_##Name##_get(Type *target, Type *source, int len, int pe) { \
+        NVSHMEMI_TRACE(OpType, target, source, len, pe, 0);                                \
+        if (len <= 0) return;                                                               \
+        copy<16, 4, false>(target, translate(source, pe), (size_t)len * sizeof(Type), 0, 1);   \
+    }
+NVSHMEMI_DEF_GET(int, int, NVSHMEMI_OP_GET_INT)
+NVSHMEMI_DEF_GET(float, float, NVSHMEMI_OP_GET_FLOAT)
+NVSHMEMI_DEF_GET(double, double, NVSHMEMI_OP_GET_DOUBLE)
+
+/* --------------------------------------------------------------- waits -- */
+
+extern "C" __device__ void nvshmem_int_wait(int *ivar, int cmp_value) {
+    /* reference nvshmem_device.cu:542-562: block while *ivar == cmp_value */
+    NVSHMEMI_TRACE_TIMED_BEGIN();
+    wait_until(ivar, CMP_NE, cmp_value);
+    NVSHMEMI_TRACE_TIMED_END(NVSHMEMI_OP_WAIT_INT, ivar, (const void *)0, 1, my_pe());
+}
+
+extern "C" __device__ void nvshmem_int_wait_until(int *ivar, int cmp, int cmp_value) {
+    /* reference nvshmem_device.cu:564-584: block until *ivar == cmp_value;
+     * the comparison operator `cmp` is ignored, as in the reference */
+    (void)cmp;
+    NVSHMEMI_TRACE_TIMED_BEGIN();
+    wait_until(ivar, CMP_EQ, cmp_value);
+    NVSHMEMI_TRACE_TIMED_END(NVSHMEMI_OP_WAIT_UNTIL_INT, ivar, (const void *)0, 1, my_pe());
+}
+
+extern "C" __device__ int nvshmemx_int_wait_until_cmp(int *ivar, int cmp, int value) {
+    return wait_until(ivar, cmp, value);
+}
+
+/* ---------------------------------------------------------- extensions -- */
+
+extern "C" __device__ int nvshmemx_my_pe_d(void) { return my_pe(); }
+extern "C" __device__ int nvshmemx_n_pes_d(void) { return nvshmemi_dstate.npes; }
+extern "C" __device__ void *nvshmemx_ptr_d(const void *addr, int pe) { return translate(addr, pe); }
+
+namespace {
+/* rank of the calling lane among the wave's active lanes, and their count */
+__device__ __forceinline__ void active_lanes(unsigned &rank, unsigned &count) {
+    const uint64_t exec = __builtin_amdgcn_read_exec();
+    rank = __builtin_amdgcn_mbcnt_hi((unsigned)(exec >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)exec, 0u));
+    count = (unsigned)__builtin_popcountll(exec);
+}
+__device__ __forceinline__ void wave_copy(void *dst, const void *src, size_t nbytes) {
+    unsigned r, n;
+    active_lanes(r, n);
+    copy<16, 4, false>(dst, src, nbytes, r, n);
+}
+__device__ __forceinline__ void block_copy(void *dst, const void *src, size_t nbytes) {
+    copy<16, 4, false>(dst, src, nbytes, flat_tid(), block_threads());
+    fence_release();
+    __syncthreads();
+}
+} // namespace
+
+extern "C" __device__ void nvshmemx_putmem_wave(void *target, const void *source, size_t nbytes, int pe) {
+    wave_copy(translate(target, pe), source, nbytes);
+}
+extern "C" __device__ void nvshmemx_getmem_wave(void *target, const void *source, size_t nbytes, int pe) {
+    wave_copy(target, translate(source, pe), nbytes);
+}
+extern "C" __device__ void nvshmemx_putmem_block(void *target, const void *source, size_t nbytes, int pe) {
+    block_copy(translate(target, pe), source, nbytes);
+}
+extern "C" __device__ void nvshmemx_getmem_block(void *target, const void *source, size_t nbytes, int pe) {
+    block_copy(target, translate(source, pe), nbytes);
+}
+
+#define NVSHMEMI_DEF_COOP(Name, Type)                                                              \
+    extern "C" __device__ void nvshmemx_##Name##_put_wave(Type *t, const Type *s, int len, int pe) { \
+        if (len > 0) wave_copy(translate(t, pe), s, (size_t)len * sizeof(Type));                   \
+    }                                                                                              \
+    extern "C" __device__ void nvshmemx_##Name##_get_wave(Type *t, const Type *s, int len, int pe) { \
+        if (len > 0) wave_copy(t, translate(s, pe), (size_t)len * sizeof(Type));                   \
+    }                                                                                              \
+    extern "C" __device__ void nvshmemx_##Name##_put_block(Type *t, const Type *s, int len, int pe) { \
+        block_copy(translate(t, pe), s, len > 0 ? (size_t)len * sizeof(Type) : 0);                 \
+    }                                                                                              \
+    extern "C" __device__ void nvshmemx_##Name##_get_block(Type *t, const Type *s, int len, int pe) { \
+        block_copy(t, translate(s, pe), len > 0 ? (size_t)len * sizeof(Type) : 0);                 \
+    }
+NVSHMEMI_DEF_COOP(int, int)
+NVSHMEMI_DEF_COOP(float, float)
+NVSHMEMI_DEF_COOP(double, double)
+
+/* ------------------------------------------------ workgroup barrier -- */
+
+/* Counter + generation barrier over every workgroup of the grid.  The
+ * generation word is the barrier's sense: a workgroup reads it before
+ * arriving (it cannot advance before this workgroup arrives), the last
+ * arriver resets the counter and advances the generation with release, and
+ * the others wait for the generation to change.  No per-workgroup state
+ * survives between launches, so grids of different sizes may reuse it. */
+extern "C" __device__ void nvshmemx_barrier_all_wg(void) {
+    fence_release();
+    __syncthreads();
+    if (flat_tid() == 0) {
+        char *internal = nvshmemi_dstate.heap_base[0];
+        int *count = (int *)(internal + NVSHMEMI_OFF_WGBAR_COUNT);
+        int *gen = (int *)(internal + NVSHMEMI_OFF_WGBAR_SENSE);
+        const int nwg = (int)(gridDim.x * gridDim.y * gridDim.z);
+        const int g0 = load_acquire(gen);
+        const int old = __hip_atomic_fetch_add(count, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == nwg - 1) {
+            store_relaxed(count, 0);
+            store_release(gen, g0 + 1);
+        } else {
+            wait_until(gen, CMP_NE, g0);
+        }
+        fence_acquire();
+    }
+    __syncthreads();
+}

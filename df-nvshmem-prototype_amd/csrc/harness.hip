@@ -1,0 +1,409 @@
+/*
+ * harness.hip -- caller-side kernels that exercise the device API the way the
+ * reference's microbenchmarks do (microbench/shmem_latency.cu:48-81,
+ * microbench/shmem_sync_latency.cu:48-78), plus the seeded data generator and
+ * the ordering checks.  They are ordinary callers of the public device API and
+ * are exported through plain C entry points (prefix nvshmemh_) so the test and
+ * benchmark drivers can launch them without a compiler.
+ *
+ * Seeded data: PCG32 (O'Neill 2014, pcg32_random_r / pcg32_srandom_r of the
+ * PCG "minimal C" reference), word i of a buffer = the i-th output after
+ * seeding with (seed, seq).  Each lane jumps ahead with the LCG advance
+ * formula (O(log n) multiply-adds), so a wavefront writes 64 consecutive words
+ * per store.
+ */
+#include <stdint.h>
+
+#include "nvshmem.h"
+#include "nvshmem_device.h"
+#include "nvshmemx.h"
+#include "nvshmemx_device.h"
+#include "nvshmemi_device.h"
+#include "nvshmemi_host.h"
+
+namespace {
+
+constexpr uint64_t kPcgMult = 6364136223846793005ULL;
+
+__host__ __device__ inline uint32_t pcg_output(uint64_t old) {
+    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+
+/* state after `delta` LCG steps from `state` (Brown, "Random number generation
+ * with arbitrary strides", 1994) */
+__host__ __device__ inline uint64_t pcg_advance(uint64_t state, uint64_t delta, uint64_t inc) {
+    uint64_t acc_mult = 1, acc_plus = 0, cur_mult = kPcgMult, cur_plus = inc;
+    while (delta) {
+        if (delta & 1) {
+            acc_mult *= cur_mult;
+            acc_plus = acc_plus * cur_mult + cur_plus;
+        }
+        cur_plus = (cur_mult + 1) * cur_plus;
+        cur_mult *= cur_mult;
+        delta >>= 1;
+    }
+    return acc_mult * state + acc_plus;
+}
+
+__host__ __device__ inline uint64_t pcg_seeded_state(uint64_t seed, uint64_t seq, uint64_t &inc) {
+    inc = (seq << 1u) | 1u;
+    uint64_t s = 0;
+    s = s * kPcgMult + inc;
+    s += seed;
+    s = s * kPcgMult + inc;
+    return s;
+}
+
+constexpr int kFillWordsPerLane = 64;
+
+/* harness loops stop at the first recorded device error (a timed-out wait),
+ * so a broken protocol costs one wait budget, not one per iteration */
+__device__ __forceinline__ bool failed() { return nvshmemi::load_relaxed(nvshmemi_dstate.errors) != 0; }
+
+__global__ void k_fill_pcg32(uint32_t *dst, size_t nwords, uint64_t s0, uint64_t inc, uint64_t m64, uint64_t a64) {
+    const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const unsigned lane = threadIdx.x & 63u;
+    const size_t base = wave * 64 * kFillWordsPerLane;
+    if (base >= nwords) return;
+    uint64_t st = pcg_advance(s0, base + lane, inc);
+    for (int j = 0; j < kFillWordsPerLane; ++j) {
+        const size_t idx = base + (size_t)j * 64 + lane;
+        if (idx < nwords) dst[idx] = pcg_output(st);
+        st = m64 * st + a64;
+    }
+}
+
+/* ---- per-thread API drivers ---- */
+
+template <typename T>
+__device__ void thread_put(T *target, T *source, int len, int pe);
+template <> __device__ void thread_put<int>(int *t, int *s, int l, int pe) { nvshmem_int_put(t, s, l, pe); }
+template <> __device__ void thread_put<float>(float *t, float *s, int l, int pe) { nvshmem_float_put(t, s, l, pe); }
+template <> __device__ void thread_put<double>(double *t, double *s, int l, int pe) { nvshmem_double_put(t, s, l, pe); }
+template <typename T>
+__device__ void thread_get(T *target, T *source, int len, int pe);
+template <> __device__ void thread_get<int>(int *t, int *s, int l, int pe) { nvshmem_int_get(t, s, l, pe); }
+template <> __device__ void thread_get<float>(float *t, float *s, int l, int pe) { nvshmem_float_get(t, s, l, pe); }
+template <> __device__ void thread_get<double>(double *t, double *s, int l, int pe) { nvshmem_double_get(t, s, l, pe); }
+
+/* thread i copies elements [i*chunk, min((i+1)*chunk, len)) with one
+ * per-thread call, so every thread's copy is complete on its own */
+template <typename T>
+__global__ void k_thread_putget(int op, T *target, T *source, int len, int chunk, int pe) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long lo = (long long)i * chunk;
+    if (lo >= len) return;
+    const int n = (int)((lo + chunk <= len) ? chunk : len - lo);
+    if (op == 0) thread_put<T>(target + lo, source + lo, n, pe);
+    else thread_get<T>(target + lo, source + lo, n, pe);
+    nvshmem_quiet();
+}
+
+/* scope 0: every wave copies its slice with the wave-cooperative call;
+ * scope 1: every workgroup copies its slice with the block-cooperative call */
+__global__ void k_coop_putget(int op, int scope, char *target, const char *source, size_t nbytes, size_t slice,
+                              int pe) {
+    size_t unit = scope == 0 ? ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / 64 : (size_t)blockIdx.x;
+    const size_t lo = unit * slice;
+    const size_t n = lo >= nbytes ? 0 : (lo + slice <= nbytes ? slice : nbytes - lo);
+    if (scope == 0) {
+        if (n == 0) return;
+        if (op == 0) nvshmemx_putmem_wave(target + lo, source + lo, n, pe);
+        else nvshmemx_getmem_wave(target + lo, source + lo, n, pe);
+    } else {
+        if (op == 0) nvshmemx_putmem_block(target + lo, source + lo, n, pe);
+        else nvshmemx_getmem_block(target + lo, source + lo, n, pe);
+    }
+}
+
+/* thread i: p(target + i, values[i], pe) or out[i] = g(target + i, pe) */
+template <typename T>
+__global__ void k_pg(int op, T *target, const T *values, T *out, int n, int pe);
+template <>
+__global__ void k_pg<int>(int op, int *t, const int *v, int *o, int n, int pe) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (op == 0) nvshmem_int_p(t + i, v[i], pe);
+    else o[i] = nvshmem_int_g(t + i, pe);
+}
+template <>
+__global__ void k_pg<float>(int op, float *t, const float *v, float *o, int n, int pe) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (op == 0) nvshmem_float_p(t + i, v[i], pe);
+    else o[i] = nvshmem_float_g(t + i, pe);
+}
+template <>
+__global__ void k_pg<double>(int op, double *t, const double *v, double *o, int n, int pe) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (op == 0) nvshmem_double_p(t + i, v[i], pe);
+    else o[i] = nvshmem_double_g(t + i, pe);
+}
+
+/* ---- ping-pong (p + wait round trips, optional payload put) ----
+ * Initiator (PE `a`): for i = 1..iters: [put payload; quiet;] p(flag, i, b);
+ * wait(flag, i-1) until b's reply arrives.  Responder: wait(flag, i-1);
+ * [put payload back; quiet;] p(flag, i, a).  Waiting with "while equal to the
+ * previous value" cannot miss an update the way the reference's
+ * wait_until(flag, i+1) equality can (microbench/shmem_sync_latency.cu:60-68).
+ * samples[i] = wall_clock64 ticks of round trip i (initiator only). */
+__device__ void pingpong_body(int *flag_sym, int me, int peer, int initiator, int iters, int *payload_sym,
+                              int payload_len, long long *samples) {
+    int *myflag = (int *)nvshmemx_ptr_d(flag_sym, me);
+    int *mypayload = payload_len > 0 ? (int *)nvshmemx_ptr_d(payload_sym, me) : nullptr;
+    for (int i = 1; i <= iters; ++i) {
+        if (initiator) {
+            const long long t0 = wall_clock64();
+            if (payload_len > 0) {
+                nvshmem_int_put(payload_sym, mypayload, payload_len, peer);
+                nvshmem_quiet();
+            }
+            nvshmem_int_p(flag_sym, i, peer);
+            nvshmem_quiet();
+            nvshmem_int_wait(myflag, i - 1);
+            const long long t1 = wall_clock64();
+            if (samples) samples[i - 1] = t1 - t0;
+            if (failed()) break;
+        } else {
+            nvshmem_int_wait(myflag, i - 1);
+            if (failed()) break;
+            if (payload_len > 0) {
+                nvshmem_int_put(payload_sym, mypayload, payload_len, peer);
+                nvshmem_quiet();
+            }
+            nvshmem_int_p(flag_sym, i, peer);
+            nvshmem_quiet();
+        }
+    }
+}
+
+__global__ void k_pingpong_process(int *flag_sym, int me, int peer, int initiator, int iters, int *payload_sym,
+                                   int payload_len, long long *samples) {
+    if (threadIdx.x != 0) return;
+    pingpong_body(flag_sym, me, peer, initiator, iters, payload_sym, payload_len, samples);
+}
+
+/* local mode: workgroup 0 is PE a (initiator), workgroup 1 is PE b */
+__global__ void k_pingpong_local(int *flag_sym, int pe_a, int pe_b, int iters, int *payload_sym, int payload_len,
+                                 long long *samples) {
+    if (threadIdx.x != 0) return;
+    const int me = blockIdx.x == 0 ? pe_a : pe_b;
+    const int peer = blockIdx.x == 0 ? pe_b : pe_a;
+    pingpong_body(flag_sym, me, peer, blockIdx.x == 0, iters, payload_sym, payload_len, blockIdx.x == 0 ? samples : nullptr);
+}
+
+/* ---- ordering check: payload published by quiet + flag ----
+ * Producer workgroup (PE a): round k: fill its local staging buffer with k,
+ * block-put it into PE b's `data`, then one thread does quiet + p(flag, k, b);
+ * then it waits for b's ack == k before reusing the buffer.
+ * Consumer workgroup (PE b): waits for flag != k-1, reads every element of its
+ * own `data`, counts elements != k, acks.  violations[0] = total mismatches,
+ * violations[1] = flag values observed out of order (flag < k). */
+__global__ void k_order_local(int *data_sym, int *stage_sym, int *flag_sym, int *ack_sym, int n, int rounds,
+                              int pe_a, int pe_b, unsigned long long *violations) {
+    const bool producer = blockIdx.x == 0;
+    const int me = producer ? pe_a : pe_b;
+    const int peer = producer ? pe_b : pe_a;
+    int *mydata = (int *)nvshmemx_ptr_d(data_sym, me);
+    int *mystage = (int *)nvshmemx_ptr_d(stage_sym, me);
+    int *myflag = (int *)nvshmemx_ptr_d(flag_sym, me);
+    int *myack = (int *)nvshmemx_ptr_d(ack_sym, me);
+    __shared__ int seen;
+    __shared__ int stop;
+    for (int k = 1; k <= rounds; ++k) {
+        if (threadIdx.x == 0) stop = failed();
+        __syncthreads();
+        if (stop) break;
+        if (producer) {
+            for (int i = threadIdx.x; i < n; i += blockDim.x) mystage[i] = k;
+            nvshmemi::fence_release();
+            __syncthreads();
+            nvshmemx_int_put_block(data_sym, mystage, n, peer);
+            if (threadIdx.x == 0) {
+                nvshmem_quiet();
+                nvshmem_int_p(flag_sym, k, peer);
+                nvshmem_quiet();
+                nvshmem_int_wait(myack, k - 1);
+            }
+            __syncthreads();
+        } else {
+            if (threadIdx.x == 0) seen = nvshmemx_int_wait_until_cmp(myflag, NVSHMEMX_CMP_NE, k - 1);
+            __syncthreads();
+            if (threadIdx.x == 0 && seen < k) atomicAdd(&violations[1], 1ull);
+            nvshmemi::fence_acquire();
+            unsigned long long bad = 0;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) bad += (mydata[i] != k);
+            if (bad) atomicAdd(&violations[0], bad);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                nvshmem_int_p(ack_sym, k, peer);
+                nvshmem_quiet();
+            }
+        }
+    }
+}
+
+/* ---- workgroup-PE barrier check (persistent local-mode kernel) ----
+ * Each PE (its first workgroup) writes round r into slot[me] of every PE with
+ * p, then all workgroups meet in nvshmemx_barrier_all_wg; afterwards every
+ * slot of the PE's own copy must read >= r.  violations counts slots < r. */
+__global__ void k_wg_barrier_check(int *slots_sym, int rounds, unsigned long long *violations) {
+    const int me = nvshmemx_my_pe_d();
+    const int npes = nvshmemx_n_pes_d();
+    const bool leader = (blockIdx.x & ((1u << nvshmemi_dstate.wg_shift) - 1u)) == 0;
+    int *mine = (int *)nvshmemx_ptr_d(slots_sym, me);
+    __shared__ int stop;
+    for (int r = 1; r <= rounds; ++r) {
+        if (threadIdx.x == 0) stop = failed();
+        __syncthreads();
+        if (stop) break;
+        if (leader) {
+            for (int p = threadIdx.x; p < npes; p += blockDim.x) nvshmem_int_p(slots_sym + me, r, p);
+            nvshmem_quiet();
+        }
+        nvshmemx_barrier_all_wg();
+        if (leader)
+            for (int p = threadIdx.x; p < npes; p += blockDim.x)
+                if (nvshmemi::load_relaxed(mine + p) < r) atomicAdd(violations, 1ull);
+        nvshmemx_barrier_all_wg();
+    }
+}
+
+/* ---- offload-barrier check across processes (one kernel per round step) ---- */
+__global__ void k_publish_round(int *slots_sym, int me, int npes, int r) {
+    for (int p = threadIdx.x; p < npes; p += blockDim.x) nvshmem_int_p(slots_sym + me, r, p);
+    nvshmem_quiet();
+}
+__global__ void k_check_round(const int *slots_local, int npes, int r, unsigned long long *violations) {
+    for (int p = threadIdx.x; p < npes; p += blockDim.x)
+        if (nvshmemi::load_relaxed(slots_local + p) < r) atomicAdd(violations, 1ull);
+}
+
+/* one thread waits once: until == 0 uses nvshmem_int_wait (while equal),
+ * until == 1 uses nvshmem_int_wait_until (until equal); out[0] = value seen */
+__global__ void k_wait_once(int *ivar, int cmp_value, int until, int *out) {
+    if (threadIdx.x != 0) return;
+    if (until) nvshmem_int_wait_until(ivar, 0, cmp_value);
+    else nvshmem_int_wait(ivar, cmp_value);
+    out[0] = nvshmemi::load_relaxed(ivar);
+}
+
+} // namespace
+
+/* --------------------------------------------------------- C entry points -- */
+
+extern "C" {
+
+void nvshmemh_fill_pcg32(void *dst, size_t nwords, unsigned long long seed, unsigned long long seq, hipStream_t s) {
+    if (nwords == 0) return;
+    uint64_t inc = 0;
+    const uint64_t s0 = pcg_seeded_state(seed, seq, inc);
+    /* one 64-step jump as an affine map: state' = m64 * state + a64 */
+    const uint64_t m64 = pcg_advance(1, 64, 0) /* multiplier only */;
+    const uint64_t a64 = pcg_advance(0, 64, inc);
+    const size_t words_per_wave = 64 * (size_t)kFillWordsPerLane;
+    const size_t waves = (nwords + words_per_wave - 1) / words_per_wave;
+    const size_t blocks = (waves * 64 + 255) / 256;
+    hipLaunchKernelGGL(k_fill_pcg32, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t *)dst, nwords, s0, inc, m64, a64);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+/* host-side generator with the same definition (used by tests to pin the
+ * device generator against the oracle) */
+void nvshmemh_pcg32_host(uint32_t *out, size_t nwords, unsigned long long seed, unsigned long long seq) {
+    uint64_t inc = 0;
+    uint64_t st = pcg_seeded_state(seed, seq, inc);
+    for (size_t i = 0; i < nwords; ++i) {
+        out[i] = pcg_output(st);
+        st = st * kPcgMult + inc;
+    }
+}
+
+/* dtype: 0 int, 1 float, 2 double; op: 0 put, 1 get */
+void nvshmemh_thread_putget(int op, int dtype, void *target, void *source, int len, int chunk, int pe, hipStream_t s) {
+    if (len <= 0) return;
+    if (chunk <= 0) chunk = len;
+    const int threads = (len + chunk - 1) / chunk;
+    const int block = threads < 256 ? threads : 256;
+    const int grid = (threads + block - 1) / block;
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_thread_putget<int>, dim3(grid), dim3(block), 0, s, op, (int *)target, (int *)source, len, chunk, pe);
+    else if (dtype == 1)
+        hipLaunchKernelGGL(k_thread_putget<float>, dim3(grid), dim3(block), 0, s, op, (float *)target, (float *)source, len, chunk, pe);
+    else
+        hipLaunchKernelGGL(k_thread_putget<double>, dim3(grid), dim3(block), 0, s, op, (double *)target, (double *)source, len, chunk, pe);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+/* scope: 0 wave, 1 block; each wave / workgroup copies `slice` bytes */
+void nvshmemh_coop_putget(int op, int scope, void *target, const void *source, size_t nbytes, size_t slice, int pe,
+                          int block, hipStream_t s) {
+    if (nbytes == 0) return;
+    if (slice == 0) slice = nbytes;
+    const size_t units = (nbytes + slice - 1) / slice;
+    const size_t per_block = scope == 0 ? (size_t)block / 64 : 1;
+    const size_t grid = (units + per_block - 1) / per_block;
+    hipLaunchKernelGGL(k_coop_putget, dim3((unsigned)grid), dim3(block), 0, s, op, scope, (char *)target,
+                       (const char *)source, nbytes, slice, pe);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+void nvshmemh_pg(int op, int dtype, void *target, const void *values, void *out, int n, int pe, hipStream_t s) {
+    if (n <= 0) return;
+    const int block = 256, grid = (n + block - 1) / block;
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_pg<int>, dim3(grid), dim3(block), 0, s, op, (int *)target, (const int *)values, (int *)out, n, pe);
+    else if (dtype == 1)
+        hipLaunchKernelGGL(k_pg<float>, dim3(grid), dim3(block), 0, s, op, (float *)target, (const float *)values, (float *)out, n, pe);
+    else
+        hipLaunchKernelGGL(k_pg<double>, dim3(grid), dim3(block), 0, s, op, (double *)target, (const double *)values, (double *)out, n, pe);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+void nvshmemh_pingpong_process(int *flag_sym, int peer, int initiator, int iters, int *payload_sym, int payload_len,
+                               long long *samples, hipStream_t s) {
+    hipLaunchKernelGGL(k_pingpong_process, dim3(1), dim3(64), 0, s, flag_sym, nvshmem_my_pe(), peer, initiator, iters,
+                       payload_sym, payload_len, samples);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+void nvshmemh_pingpong_local(int *flag_sym, int pe_a, int pe_b, int iters, int *payload_sym, int payload_len,
+                             long long *samples, hipStream_t s) {
+    hipLaunchKernelGGL(k_pingpong_local, dim3(2), dim3(64), 0, s, flag_sym, pe_a, pe_b, iters, payload_sym,
+                       payload_len, samples);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+void nvshmemh_order_local(int *data_sym, int *stage_sym, int *flag_sym, int *ack_sym, int n, int rounds, int pe_a,
+                          int pe_b, unsigned long long *violations, hipStream_t s) {
+    hipLaunchKernelGGL(k_order_local, dim3(2), dim3(256), 0, s, data_sym, stage_sym, flag_sym, ack_sym, n, rounds,
+                       pe_a, pe_b, violations);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+void nvshmemh_wg_barrier_check(int *slots_sym, int rounds, unsigned long long *violations, int block, hipStream_t s) {
+    const int npes = nvshmem_n_pes();
+    const int grid = npes * nvshmemx_wgs_per_pe();
+    void *args[] = {&slots_sym, &rounds, &violations};
+    nvshmemx_collective_launch((const void *)k_wg_barrier_check, dim3(grid), dim3(block), args, 0, s);
+}
+
+void nvshmemh_wait_once(int *ivar, int cmp_value, int until, int *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_wait_once, dim3(1), dim3(64), 0, s, ivar, cmp_value, until, out);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+void nvshmemh_publish_round(int *slots_sym, int r, hipStream_t s) {
+    hipLaunchKernelGGL(k_publish_round, dim3(1), dim3(128), 0, s, slots_sym, nvshmem_my_pe(), nvshmem_n_pes(), r);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+void nvshmemh_check_round(const int *slots_local, int r, unsigned long long *violations, hipStream_t s) {
+    hipLaunchKernelGGL(k_check_round, dim3(1), dim3(128), 0, s, slots_local, nvshmem_n_pes(), r, violations);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+} /* extern "C" */

@@ -1,0 +1,107 @@
+/*
+ * kernels.hip -- host-launched kernels of the library: the grid-wide copy
+ * behind nvshmemx_{put,get}mem_on_stream and the stream-ordered offload
+ * barrier behind nvshmem_barrier_all_offload.
+ *
+ * Copy: bulk put/get is HBM-bound (2 bytes of traffic per byte moved, no
+ * arithmetic), so the kernel is a streaming copy: 16-byte accesses, UNROLL
+ * independent loads in flight per lane, lanes interleaved so each wavefront
+ * access covers 64 * 16 = 1 KiB of contiguous memory, and a grid-stride loop
+ * over the whole range so every CU streams from the start.
+ *
+ * Barrier (reference nvshmem_kernels.cu:56-102 gathers arrivals on PE 0 with
+ * one spinning thread per peer, epoch values mod 65536, double-buffered by
+ * parity, and fails open after 10^6 cycles): here one lane per PE does an
+ * agent-scope fetch-add on a counter in PE 0's internal heap region and waits
+ * for the generation word (the barrier's sense) to advance; the last arriver
+ * resets the counter and advances the generation with release.  A wait that
+ * outlives the wall-clock budget records NVSHMEMI_ERR_BARRIER_TIMEOUT.
+ */
+#include "nvshmemi_device.h"
+#include "nvshmemi_host.h"
+
+namespace nvshmemi {
+
+template <int W, int U, bool NT>
+__global__ void __launch_bounds__(1024) k_copy(char *dst, const char *src, size_t nbytes) {
+    const size_t lane = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)gridDim.x * blockDim.x;
+    copy<W, U, NT>(dst, src, nbytes, lane, n);
+}
+
+typedef void (*copy_kernel_t)(char *, const char *, size_t);
+
+template <int W, bool NT> static copy_kernel_t pick_unroll(int u) {
+    switch (u) {
+    case 1: return k_copy<W, 1, NT>;
+    case 2: return k_copy<W, 2, NT>;
+    case 8: return k_copy<W, 8, NT>;
+    default: return k_copy<W, 4, NT>;
+    }
+}
+
+static copy_kernel_t pick_copy_kernel(const CopyConfig &c) {
+    if (c.nontemporal) {
+        if (c.vec_bytes >= 16) return pick_unroll<16, true>(c.unroll);
+        if (c.vec_bytes >= 8) return pick_unroll<8, true>(c.unroll);
+        return pick_unroll<4, true>(c.unroll);
+    }
+    if (c.vec_bytes >= 16) return pick_unroll<16, false>(c.unroll);
+    if (c.vec_bytes >= 8) return pick_unroll<8, false>(c.unroll);
+    return pick_unroll<4, false>(c.unroll);
+}
+
+CopyConfig default_copy_config(size_t nbytes) {
+    CopyConfig c;
+    c.block = 256;
+    c.vec_bytes = 16;
+    c.unroll = 4;
+    c.nontemporal = 0;
+    const size_t per_wg = (size_t)c.block * c.vec_bytes * c.unroll;
+    size_t wgs = (nbytes + per_wg - 1) / per_wg;
+    if (wgs < 1) wgs = 1;
+    if (wgs > 2048) wgs = 2048; /* 8 workgroups per CU on 256 CUs */
+    c.workgroups = (int)wgs;
+    return c;
+}
+
+void launch_copy(void *dst, const void *src, size_t nbytes, const CopyConfig &cfg, hipStream_t s) {
+    if (cfg.block < 64 || cfg.block > 1024 || cfg.block % 64 != 0)
+        NVSHMEMI_FATAL("copy block size %d must be a multiple of 64 in [64, 1024]", cfg.block);
+    if (cfg.workgroups < 1) NVSHMEMI_FATAL("copy grid size %d must be positive", cfg.workgroups);
+    copy_kernel_t k = pick_copy_kernel(cfg);
+    hipLaunchKernelGGL(k, dim3(cfg.workgroups), dim3(cfg.block), 0, s, (char *)dst, (const char *)src, nbytes);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+__global__ void k_offload_barrier(int *count, int *gen, int npes) {
+    if (threadIdx.x != 0) return;
+    const int g0 = load_acquire(gen);
+    const int old = __hip_atomic_fetch_add(count, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == npes - 1) {
+        store_relaxed(count, 0);
+        store_release(gen, g0 + 1);
+        return;
+    }
+    const uint64_t budget = nvshmemi_dstate.wait_timeout_ticks;
+    const uint64_t t0 = wall_clock64();
+    int backoff = 1;
+    while (load_relaxed(gen) == g0) {
+        for (int i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(1);
+        if (backoff < 8) backoff <<= 1;
+        if ((uint64_t)(wall_clock64() - t0) > budget) {
+            set_error(NVSHMEMI_ERR_BARRIER_TIMEOUT);
+            break;
+        }
+    }
+    fence_acquire();
+}
+
+void launch_offload_barrier(hipStream_t s) {
+    char *root = g.heap_base[0];
+    hipLaunchKernelGGL(k_offload_barrier, dim3(1), dim3(64), 0, s, (int *)(root + NVSHMEMI_OFF_BAR_COUNT),
+                       (int *)(root + NVSHMEMI_OFF_BAR_SENSE), g.npes);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+} // namespace nvshmemi

@@ -1,0 +1,204 @@
+/*
+ * nvshmemi_device.h -- inline device primitives shared by the device API, the
+ * host-launched copy kernels, the batched-put engine and the harness kernels.
+ *
+ * Memory model (gfx950, LLVM AMDGPU memory model for GFX942/GFX950):
+ *  - every PE lives on the same agent (one MI355X), so agent scope is the
+ *    widest scope any ordering here needs; the reference's system-scope fence
+ *    (__threadfence_system, nvshmem_device.cu:434) would also write back and
+ *    invalidate for the host, which no PE is;
+ *  - the eight XCDs each have an L2 that is not coherent with the others for
+ *    ordinary (coarse-grained) device memory.  Agent-scope release writes the
+ *    local L2 back (buffer_wbl2 sc1) and agent-scope acquire invalidates it
+ *    (buffer_inv sc1); flag words are accessed with agent-scope atomic
+ *    loads/stores (sc1), which bypass the non-coherent L2 state;
+ *  - bulk payload is moved with plain (or non-temporal) 16-byte vector
+ *    accesses and published by one release, never by per-element volatile
+ *    accesses as in the reference (nvshmem_device.cu:497-500).
+ */
+#ifndef NVSHMEMI_DEVICE_H
+#define NVSHMEMI_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "nvshmemi_state.h"
+
+extern __constant__ nvshmemi_device_state_t nvshmemi_dstate;
+
+namespace nvshmemi {
+
+/* ---------------------------------------------------------------- state -- */
+
+__device__ __forceinline__ char *translate(const void *addr, int pe) {
+    const nvshmemi_device_state_t &s = nvshmemi_dstate;
+    uint64_t off = ((uint64_t)((const char *)addr - s.anchor)) & s.offset_mask;
+    return s.heap_base[pe] + off;
+}
+
+/* PE of the calling workgroup: the process PE, or blockIdx.x >> wg_shift in
+ * local (workgroup-partitioned) mode */
+__device__ __forceinline__ int my_pe() {
+    return nvshmemi_dstate.mype + (int)(blockIdx.x >> nvshmemi_dstate.wg_shift);
+}
+
+__device__ __forceinline__ void set_error(int bits) {
+    __hip_atomic_fetch_or(nvshmemi_dstate.errors, bits, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+/* --------------------------------------------------------- memory model -- */
+
+__device__ __forceinline__ void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+__device__ __forceinline__ void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void fence_acq_rel() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent"); }
+
+template <typename T> struct bits_of;
+template <> struct bits_of<int> { typedef int type; };
+template <> struct bits_of<float> { typedef int type; };
+template <> struct bits_of<double> { typedef long long type; };
+template <> struct bits_of<long long> { typedef long long type; };
+template <> struct bits_of<unsigned int> { typedef unsigned int type; };
+
+template <typename T> __device__ __forceinline__ void store_relaxed(T *p, T v) {
+    typedef typename bits_of<T>::type B;
+    __hip_atomic_store((B *)p, __builtin_bit_cast(B, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ __forceinline__ T load_relaxed(const T *p) {
+    typedef typename bits_of<T>::type B;
+    B b = __hip_atomic_load((B *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_bit_cast(T, b);
+}
+template <typename T> __device__ __forceinline__ void store_release(T *p, T v) {
+    typedef typename bits_of<T>::type B;
+    __hip_atomic_store((B *)p, __builtin_bit_cast(B, v), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ __forceinline__ T load_acquire(const T *p) {
+    typedef typename bits_of<T>::type B;
+    B b = __hip_atomic_load((B *)p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_bit_cast(T, b);
+}
+
+/* --------------------------------------------------------------- waits -- */
+
+enum cmp_op { CMP_EQ = 0, CMP_NE = 1, CMP_GT = 2, CMP_GE = 3, CMP_LT = 4, CMP_LE = 5 };
+
+__device__ __forceinline__ bool cmp_holds(int v, int op, int ref) {
+    switch (op) {
+    case CMP_EQ: return v == ref;
+    case CMP_NE: return v != ref;
+    case CMP_GT: return v > ref;
+    case CMP_GE: return v >= ref;
+    case CMP_LT: return v < ref;
+    default: return v <= ref;
+    }
+}
+
+/* Poll *ivar with agent-scope loads until `v op ref` holds.  Back-off is a
+ * short s_sleep ladder (64 clocks per step, at most 8 steps) so a waiting
+ * wave yields issue slots without adding more than ~250 ns of latency.  A wait
+ * that outlives the configured wall-clock budget records
+ * NVSHMEMI_ERR_WAIT_TIMEOUT and returns instead of hanging the device.
+ * Returns the last value read.  Ends with an agent-scope acquire so payload
+ * published before the flag is visible to the caller. */
+__device__ __forceinline__ int wait_until(int *ivar, int op, int ref) {
+    int v = load_relaxed(ivar);
+    if (!cmp_holds(v, op, ref)) {
+        const uint64_t budget = nvshmemi_dstate.wait_timeout_ticks;
+        const uint64_t t0 = wall_clock64();
+        int backoff = 1;
+        for (;;) {
+            for (int i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(1);
+            if (backoff < 8) backoff <<= 1;
+            v = load_relaxed(ivar);
+            if (cmp_holds(v, op, ref)) break;
+            if ((uint64_t)(wall_clock64() - t0) > budget) {
+                set_error(NVSHMEMI_ERR_WAIT_TIMEOUT);
+                break;
+            }
+        }
+    }
+    fence_acquire();
+    return v;
+}
+
+/* ---------------------------------------------------------- bulk copies -- */
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+template <int W> struct vec;
+template <> struct vec<16> { typedef u32x4 t; };
+template <> struct vec<8> { typedef u32x2 t; };
+template <> struct vec<4> { typedef unsigned int t; };
+template <> struct vec<2> { typedef unsigned short t; };
+template <> struct vec<1> { typedef unsigned char t; };
+
+template <bool NT, typename V> __device__ __forceinline__ V vload(const V *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename V> __device__ __forceinline__ void vstore(V *p, V v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+/* Copy with `n` cooperating lanes (lane index `lane`) using W-byte accesses.
+ * Requires (dst ^ src) % W == 0.  The unaligned head (< W bytes) and tail are
+ * copied byte-wise by the first lanes; the body is streamed with UNROLL
+ * independent loads in flight per lane, lanes interleaved so each wavefront
+ * touches 64*W contiguous bytes per access. */
+template <int W, int UNROLL, bool NT>
+__device__ __forceinline__ void copy_w(char *dst, const char *src, size_t nbytes, size_t lane, size_t n) {
+    typedef typename vec<W>::t V;
+    size_t head = (size_t)((W - ((uintptr_t)dst & (W - 1))) & (W - 1));
+    if (head > nbytes) head = nbytes;
+    const size_t nvec = (nbytes - head) / W;
+    const size_t tail_start = head + nvec * W;
+    const size_t tail = nbytes - tail_start;
+    if (W > 1) {
+        for (size_t i = lane; i < head; i += n) dst[i] = src[i];
+        for (size_t i = lane; i < tail; i += n) dst[tail_start + i] = src[tail_start + i];
+    }
+    const V *s = (const V *)(src + head);
+    V *d = (V *)(dst + head);
+    size_t i = lane;
+    if (UNROLL > 1) {
+        const size_t step = n * UNROLL;
+        for (; i + (UNROLL - 1) * n < nvec; i += step) {
+            V r[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) r[u] = vload<NT>(s + i + u * n);
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) vstore<NT>(d + i + u * n, r[u]);
+        }
+    }
+    for (; i < nvec; i += n) vstore<NT>(d + i, vload<NT>(s + i));
+}
+
+/* Pick the widest access width <= MAXW that the relative alignment of dst and
+ * src allows, then copy.  Every lane of the cooperating group must call this
+ * with identical arguments. */
+template <int MAXW, int UNROLL, bool NT>
+__device__ __forceinline__ void copy(void *dstv, const void *srcv, size_t nbytes, size_t lane, size_t n) {
+    char *dst = (char *)dstv;
+    const char *src = (const char *)srcv;
+    const uintptr_t rel = (uintptr_t)dst ^ (uintptr_t)src;
+    if (MAXW >= 16 && (rel & 15) == 0) copy_w<(MAXW >= 16 ? 16 : 1), UNROLL, NT>(dst, src, nbytes, lane, n);
+    else if (MAXW >= 8 && (rel & 7) == 0) copy_w<(MAXW >= 8 ? 8 : 1), UNROLL, NT>(dst, src, nbytes, lane, n);
+    else if (MAXW >= 4 && (rel & 3) == 0) copy_w<(MAXW >= 4 ? 4 : 1), UNROLL, NT>(dst, src, nbytes, lane, n);
+    else if ((rel & 1) == 0) copy_w<2, UNROLL, NT>(dst, src, nbytes, lane, n);
+    else copy_w<1, UNROLL, NT>(dst, src, nbytes, lane, n);
+}
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+__device__ __forceinline__ unsigned flat_tid() {
+    return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+}
+__device__ __forceinline__ unsigned flat_block() {
+    return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+}
+__device__ __forceinline__ unsigned block_threads() { return blockDim.x * blockDim.y * blockDim.z; }
+
+} // namespace nvshmemi
+
+#endif /* NVSHMEMI_DEVICE_H */

@@ -1,0 +1,107 @@
+/*
+ * nvshmemi_host.h -- host-side runtime state and internal entry points.
+ *
+ * The reference keeps its runtime in file-scope globals of nvshmem.cu
+ * (nvshmem.cu:119-191); here one struct holds the same facts plus what the
+ * MI355X design adds (PE layout mode, heap granularity, wait budget, stream).
+ * Like the reference, the host runtime is single-threaded and not thread-safe.
+ */
+#ifndef NVSHMEMI_HOST_H
+#define NVSHMEMI_HOST_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string>
+#include <vector>
+
+#include "nvshmemi_state.h"
+#include "nvshmemi_trace.h"
+
+namespace nvshmemi {
+
+/* Process-group services used only at start-up and in host barriers
+ * (reference: MPI_Allgather/MPI_Barrier/MPI_Bcast, nvshmem.cu:509,551,863,372). */
+struct Bootstrap {
+    virtual ~Bootstrap() {}
+    virtual const char *name() const = 0;
+    virtual int rank() const = 0;
+    virtual int size() const = 0;
+    virtual void allgather(const void *in, void *out, size_t bytes) = 0;
+    virtual void barrier() = 0;
+    virtual void bcast(void *buf, size_t bytes, int root) = 0;
+    virtual void finalize() {}
+};
+
+/* MPI when the program initialised it, else the shared-memory bootstrap when
+ * NVSHMEM_BOOTSTRAP=shm, else a single process. */
+Bootstrap *bootstrap_create();
+bool mpi_available();
+
+struct Region {
+    size_t offset;
+    size_t size;
+};
+
+struct HostState {
+    bool initialized = false;
+    int mode = NVSHMEMI_MODE_PROCESS;
+    int mype = 0;
+    int npes = 1;
+    int wgs_per_pe = 1;
+    int device = 0;
+    bool fine_grained = false;
+    size_t heap_size = 0;   /* user bytes per PE */
+    size_t heap_bytes = 0;  /* bytes per PE including the internal region */
+    size_t heap_stride = 0; /* local mode: distance between PE heaps (power of two) */
+    char *heap_alloc = nullptr; /* this process's allocation */
+    char *heap_base[NVSHMEMI_MAX_PES] = {};
+    size_t tail = 0; /* next free user offset */
+    std::vector<Region> regions;
+    hipStream_t stream = nullptr;
+    int *d_errors = nullptr;
+    Bootstrap *boot = nullptr;
+    int wallclock_khz = 100000;
+    uint64_t wait_timeout_ms = 10000;
+    nvshmemi_device_state_t dstate = {};
+    nvshmemi_trace_dev_t *trace_dev = nullptr; /* device copy of the ring descriptor */
+};
+
+extern HostState g;
+
+[[noreturn]] void fatal(const char *file, int line, const char *fmt, ...);
+void warn(const char *fmt, ...);
+
+#define NVSHMEMI_FATAL(...) ::nvshmemi::fatal(__FILE__, __LINE__, __VA_ARGS__)
+#define NVSHMEMI_HIP(stmt)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (stmt);                                                            \
+        if (e_ != hipSuccess) NVSHMEMI_FATAL("hip failed with %s (%s)", hipGetErrorString(e_), #stmt); \
+    } while (0)
+
+void require_init(const char *fn);
+void upload_device_state();
+size_t parse_size(const char *s, size_t dflt);
+
+/* kernels.hip */
+void launch_offload_barrier(hipStream_t s);
+struct CopyConfig {
+    int workgroups; /* 0: derive from size */
+    int block;      /* threads per workgroup */
+    int vec_bytes;  /* 4, 8 or 16 */
+    int unroll;     /* 1, 2, 4 or 8 */
+    int nontemporal;
+};
+CopyConfig default_copy_config(size_t nbytes);
+void launch_copy(void *dst, const void *src, size_t nbytes, const CopyConfig &cfg, hipStream_t s);
+
+/* trace_host.hip (no-ops unless built with NVSHMEM_TRACING); trace_init
+ * returns the device-side ring descriptor, or nullptr when tracing is off */
+nvshmemi_trace_dev_t *trace_init();
+void trace_flush();
+void trace_finalize();
+
+} // namespace nvshmemi
+
+#endif /* NVSHMEMI_HOST_H */

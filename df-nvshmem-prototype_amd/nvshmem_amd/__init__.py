@@ -1,0 +1,257 @@
+"""ctypes loader for the MI355X OpenSHMEM-subset library.
+
+The product is the C-ABI shared library ``build/libnvshmem_amd.so`` (and the
+static ``build/libnvshmem.a`` for compiled callers).  This module only loads it,
+declares argument types, and offers HIP runtime plumbing (device memory,
+copies, streams, events) for test and benchmark drivers.  Nothing here computes
+a result: if the library is missing the import of :func:`load` fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent.parent          # df-nvshmem-prototype_amd/
+REPO_DIR = PKG_DIR.parent
+BUILD_DIR = PKG_DIR / "build"
+INCLUDE_DIR = REPO_DIR / "include"
+
+DTYPES = {"int": 0, "float": 1, "double": 2}
+DTYPE_BYTES = {"int": 4, "float": 4, "double": 8}
+
+ERR_WAIT_TIMEOUT = 0x1
+ERR_BARRIER_TIMEOUT = 0x2
+ERR_BAD_PE = 0x4
+ERR_TRACE_OVERFLOW = 0x8
+ERR_TRACE_CONFLICT = 0x10
+
+
+def lib_path(trace: bool = False) -> Path:
+    return (BUILD_DIR / "trace" if trace else BUILD_DIR) / "libnvshmem_amd.so"
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+_P = C.c_void_p
+_SZ = C.c_size_t
+_I = C.c_int
+_U64 = C.c_ulonglong
+
+# name -> (restype, argtypes); every symbol the loaders use
+SIGNATURES = {
+    # host API (include/nvshmem.h)
+    "nvstart_pes": (None, []),
+    "nvstop_pes": (None, []),
+    "nvshmem_my_pe": (_I, []),
+    "nvshmem_n_pes": (_I, []),
+    "nvshmalloc": (_P, [_SZ]),
+    "nvshmcleanup": (None, []),
+    "nvshmem_barrier_all": (None, []),
+    "nvshmem_barrier_all_offload": (None, []),
+    "nvshmem_ptr": (_P, [_P, _I]),
+    # extensions (include/nvshmemx.h)
+    "nvshmemx_start_local_pes": (_I, [_I, _I]),
+    "nvshmemx_initialized": (_I, []),
+    "nvshmemx_mode": (_I, []),
+    "nvshmemx_wgs_per_pe": (_I, []),
+    "nvshmemx_heap_size": (_SZ, []),
+    "nvshmemx_bootstrap_name": (C.c_char_p, []),
+    "nvshmemx_wallclock_khz": (_I, []),
+    "nvshmemx_get_last_error": (_I, []),
+    "nvshmemx_clear_errors": (None, []),
+    "nvshmemx_set_wait_timeout_ms": (None, [_U64]),
+    "nvshmemx_set_stream": (None, [_P]),
+    "nvshmemx_get_stream": (_P, []),
+    "nvshmemx_putmem_on_stream": (None, [_P, _P, _SZ, _I, _P]),
+    "nvshmemx_getmem_on_stream": (None, [_P, _P, _SZ, _I, _P]),
+    "nvshmemx_int_put_on_stream": (None, [_P, _P, _SZ, _I, _P]),
+    "nvshmemx_float_put_on_stream": (None, [_P, _P, _SZ, _I, _P]),
+    "nvshmemx_double_put_on_stream": (None, [_P, _P, _SZ, _I, _P]),
+    "nvshmemx_int_get_on_stream": (None, [_P, _P, _SZ, _I, _P]),
+    "nvshmemx_float_get_on_stream": (None, [_P, _P, _SZ, _I, _P]),
+    "nvshmemx_double_get_on_stream": (None, [_P, _P, _SZ, _I, _P]),
+    "nvshmemx_putmem_config_on_stream": (None, [_P, _P, _SZ, _I, _P, _P]),
+    "nvshmemx_barrier_all_on_stream": (None, [_P]),
+    "nvshmemx_trace_flush": (None, []),
+    "nvshmemx_trace_enabled": (_I, []),
+    # caller harness (csrc/harness.hip)
+    "nvshmemh_fill_pcg32": (None, [_P, _SZ, _U64, _U64, _P]),
+    "nvshmemh_pcg32_host": (None, [_P, _SZ, _U64, _U64]),
+    "nvshmemh_thread_putget": (None, [_I, _I, _P, _P, _I, _I, _I, _P]),
+    "nvshmemh_coop_putget": (None, [_I, _I, _P, _P, _SZ, _SZ, _I, _I, _P]),
+    "nvshmemh_pg": (None, [_I, _I, _P, _P, _P, _I, _I, _P]),
+    "nvshmemh_pingpong_process": (None, [_P, _I, _I, _I, _P, _I, _P, _P]),
+    "nvshmemh_pingpong_local": (None, [_P, _I, _I, _I, _P, _I, _P, _P]),
+    "nvshmemh_order_local": (None, [_P, _P, _P, _P, _I, _I, _I, _I, _P, _P]),
+    "nvshmemh_wg_barrier_check": (None, [_P, _I, _P, _I, _P]),
+    "nvshmemh_publish_round": (None, [_P, _I, _P]),
+    "nvshmemh_wait_once": (None, [_P, _I, _I, _P, _P]),
+    "nvshmemh_check_round": (None, [_P, _I, _P, _P]),
+}
+
+
+class CopyConfig(C.Structure):
+    """nvshmemx_copy_config_t (include/nvshmemx.h)."""
+    _fields_ = [("workgroups", _I), ("block", _I), ("vec_bytes", _I), ("unroll", _I), ("nontemporal", _I)]
+
+
+def load(trace: bool = False) -> C.CDLL:
+    path = lib_path(trace)
+    if not path.exists():
+        raise LibraryMissing(f"{path} is not built; run __graft_entry__.build() (make -C {PKG_DIR})")
+    lib = C.CDLL(str(path), mode=C.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def declared_symbols() -> set[str]:
+    """Every function name declared with C linkage in include/*.h."""
+    names: set[str] = set()
+    pat = re.compile(r"^\s*(?:__device__\s+)?[A-Za-z_][\w\s\*]*?\b((?:nv|nvshmem)\w*)\s*\(", re.M)
+    for h in sorted(INCLUDE_DIR.glob("*.h")):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in pat.finditer(text):
+            names.add(m.group(1))
+    return names
+
+
+def declared_host_symbols() -> set[str]:
+    """Host (non-__device__) functions declared in include/*.h."""
+    names: set[str] = set()
+    pat = re.compile(r"^\s*(?!__device__)(?:const\s+)?[A-Za-z_]\w*[\s\*]+((?:nv|nvshmem)\w*)\s*\(", re.M)
+    for h in sorted(INCLUDE_DIR.glob("*.h")):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        text = re.sub(r"#define[^\n]*(\\\n[^\n]*)*", "", text)
+        for m in pat.finditer(text):
+            names.add(m.group(1))
+    return names
+
+
+# ---------------------------------------------------------------- HIP plumbing
+
+class HipError(RuntimeError):
+    pass
+
+
+class Hip:
+    """Minimal HIP runtime access for drivers: memory, copies, streams, events."""
+
+    H2D, D2H, D2D = 1, 2, 3
+
+    def __init__(self):
+        self.lib = C.CDLL("libamdhip64.so.7", mode=C.RTLD_GLOBAL)
+        L = self.lib
+        for name, res, args in [
+            ("hipMalloc", _I, [C.POINTER(_P), _SZ]),
+            ("hipFree", _I, [_P]),
+            ("hipMemcpy", _I, [_P, _P, _SZ, _I]),
+            ("hipMemcpyAsync", _I, [_P, _P, _SZ, _I, _P]),
+            ("hipMemset", _I, [_P, _I, _SZ]),
+            ("hipMemsetAsync", _I, [_P, _I, _SZ, _P]),
+            ("hipDeviceSynchronize", _I, []),
+            ("hipStreamCreate", _I, [C.POINTER(_P)]),
+            ("hipStreamDestroy", _I, [_P]),
+            ("hipStreamSynchronize", _I, [_P]),
+            ("hipEventCreate", _I, [C.POINTER(_P)]),
+            ("hipEventRecord", _I, [_P, _P]),
+            ("hipEventSynchronize", _I, [_P]),
+            ("hipEventElapsedTime", _I, [C.POINTER(C.c_float), _P, _P]),
+            ("hipEventDestroy", _I, [_P]),
+            ("hipGetErrorString", C.c_char_p, [_I]),
+            ("hipGetDeviceCount", _I, [C.POINTER(_I)]),
+            ("hipSetDevice", _I, [_I]),
+        ]:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+
+    def check(self, rc, what=""):
+        if rc != 0:
+            raise HipError(f"{what}: {self.lib.hipGetErrorString(rc).decode()} ({rc})")
+
+    def malloc(self, n):
+        p = _P()
+        self.check(self.lib.hipMalloc(C.byref(p), max(int(n), 1)), "hipMalloc")
+        return p.value
+
+    def free(self, p):
+        if p:
+            self.check(self.lib.hipFree(p), "hipFree")
+
+    def htod(self, dst, arr):
+        import numpy as np
+        a = np.ascontiguousarray(arr)
+        self.check(self.lib.hipMemcpy(dst, a.ctypes.data, a.nbytes, self.H2D), "hipMemcpy H2D")
+
+    def dtoh(self, src, nbytes, dtype="u1"):
+        import numpy as np
+        out = np.empty(int(nbytes) // np.dtype(dtype).itemsize, dtype=dtype)
+        if out.nbytes:
+            self.check(self.lib.hipMemcpy(out.ctypes.data, src, out.nbytes, self.D2H), "hipMemcpy D2H")
+        return out
+
+    def dtod(self, dst, src, nbytes):
+        self.check(self.lib.hipMemcpy(dst, src, nbytes, self.D2D), "hipMemcpy D2D")
+
+    def memset(self, p, val, n):
+        self.check(self.lib.hipMemset(p, val, n), "hipMemset")
+
+    def sync(self):
+        self.check(self.lib.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    def stream(self):
+        s = _P()
+        self.check(self.lib.hipStreamCreate(C.byref(s)), "hipStreamCreate")
+        return s.value
+
+    def stream_sync(self, s):
+        self.check(self.lib.hipStreamSynchronize(s), "hipStreamSynchronize")
+
+    def stream_destroy(self, s):
+        self.check(self.lib.hipStreamDestroy(s), "hipStreamDestroy")
+
+    def event(self):
+        e = _P()
+        self.check(self.lib.hipEventCreate(C.byref(e)), "hipEventCreate")
+        return e.value
+
+    def record(self, e, s=None):
+        self.check(self.lib.hipEventRecord(e, s), "hipEventRecord")
+
+    def elapsed_ms(self, a, b):
+        self.check(self.lib.hipEventSynchronize(b), "hipEventSynchronize")
+        ms = C.c_float()
+        self.check(self.lib.hipEventElapsedTime(C.byref(ms), a, b), "hipEventElapsedTime")
+        return float(ms.value)
+
+    def event_destroy(self, e):
+        self.check(self.lib.hipEventDestroy(e), "hipEventDestroy")
+
+    def device_count(self):
+        n = _I()
+        rc = self.lib.hipGetDeviceCount(C.byref(n))
+        return int(n.value) if rc == 0 else 0
+
+
+def shm_job_env(rank: int, nranks: int, name: str, extra: dict | None = None) -> dict:
+    """Environment for one PE process of a shared-memory-bootstrapped job."""
+    env = dict(os.environ)
+    env.update({
+        "NVSHMEM_BOOTSTRAP": "shm",
+        "NVSHMEM_BOOTSTRAP_RANK": str(rank),
+        "NVSHMEM_BOOTSTRAP_NRANKS": str(nranks),
+        "NVSHMEM_BOOTSTRAP_NAME": name,
+    })
+    if extra:
+        env.update({k: str(v) for k, v in extra.items()})
+    return env

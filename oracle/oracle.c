@@ -1,0 +1,183 @@
+/*
+ * oracle.c -- CPU restatement of the reference semantics (see oracle.h).
+ * TEST INFRASTRUCTURE ONLY; never linked into the product.
+ */
+#include "oracle.h"
+
+#include <stdio.h>
+#include <string.h>
+
+/* ------------------------------------------------------------- PCG32 -- */
+/* O'Neill, "PCG: A Family of Simple Fast Space-Efficient Statistically Good
+ * Algorithms for Random Number Generation" (2014); minimal C reference
+ * pcg32_srandom_r / pcg32_random_r.  Not part of the reference repository:
+ * it generates the seeded workloads its configurations describe. */
+#define PCG_MULT 6364136223846793005ULL
+
+static uint32_t pcg_out(uint64_t old) {
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+
+void oracle_pcg32_fill(uint32_t *out, size_t n, uint64_t seed, uint64_t seq) {
+    uint64_t inc = (seq << 1u) | 1u, st = 0;
+    st = st * PCG_MULT + inc;
+    st += seed;
+    st = st * PCG_MULT + inc;
+    for (size_t i = 0; i < n; ++i) {
+        out[i] = pcg_out(st);
+        st = st * PCG_MULT + inc;
+    }
+}
+
+void oracle_pcg32_uniform_pm1(float *out, size_t n, uint64_t seed, uint64_t seq) {
+    uint32_t *w = (uint32_t *)out;
+    oracle_pcg32_fill(w, n, seed, seq);
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t u = w[i] >> 8; /* 24 bits */
+        out[i] = (float)u * (1.0f / 8388608.0f) - 1.0f; /* u * 2^-23 - 1: exact */
+    }
+}
+
+void oracle_pcg32_uniform01_f64(double *out, size_t n, uint64_t seed, uint64_t seq) {
+    uint32_t *w = (uint32_t *)out; /* 2n words fit exactly in n doubles */
+    oracle_pcg32_fill(w, 2 * n, seed, seq);
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t hi = w[2 * i], lo = w[2 * i + 1];
+        uint64_t u = ((hi << 32) | lo) >> 11; /* 53 bits */
+        out[i] = (double)u * (1.0 / 9007199254740992.0);
+    }
+}
+
+/* -------------------------------------------------------------- heap -- */
+/* reference nvshmem.cu:775-802 (nvshmalloc), 805-819 (nvshmcleanup) */
+
+void oracle_heap_init(oracle_heap_t *h, size_t heap_size, size_t align) {
+    h->heap_size = heap_size;
+    h->tail = 0;
+    h->align = align ? align : 1;
+    h->nregions = 0;
+}
+
+long long oracle_nvshmalloc(oracle_heap_t *h, size_t size) {
+    size_t off = (h->tail + h->align - 1) / h->align * h->align;
+    if (off > h->heap_size || size > h->heap_size - off) return -1; /* nvshmem.cu:780-783 */
+    h->tail = off + size;                                          /* nvshmem.cu:785-786 */
+    h->nregions++;                                                 /* nvshmem.cu:788-799 */
+    return (long long)off;
+}
+
+void oracle_nvshmcleanup(oracle_heap_t *h) {
+    h->tail = 0; /* nvshmem.cu:818 */
+    h->nregions = 0;
+}
+
+/* ------------------------------------------------------- translation -- */
+/* reference nvshmem.cu:882: heap_pointers[pe] + (local - my_heap).  With
+ * mask = all ones and anchor = my heap base this is that formula; the local
+ * (single-process) layout uses anchor = PE 0's base and mask = stride - 1. */
+uint64_t oracle_translate(const uint64_t *heap_base, uint64_t anchor, uint64_t mask, uint64_t addr, int pe) {
+    return heap_base[pe] + ((addr - anchor) & mask);
+}
+
+/* ----------------------------------------------------- data movement -- */
+
+void oracle_put(uint8_t **heaps, size_t heap_bytes, size_t target_off, const void *source, size_t nbytes, int pe) {
+    /* reference nvshmem_device.cu:494-500: remote[i] = source[i], i < len */
+    if (target_off > heap_bytes || nbytes > heap_bytes - target_off) return;
+    memcpy(heaps[pe] + target_off, source, nbytes);
+}
+
+void oracle_get(uint8_t **heaps, size_t heap_bytes, void *target, size_t source_off, size_t nbytes, int pe) {
+    /* reference nvshmem_device.cu:521-527: target[i] = remote[i], i < len */
+    if (source_off > heap_bytes || nbytes > heap_bytes - source_off) return;
+    memcpy(target, heaps[pe] + source_off, nbytes);
+}
+
+void oracle_p32(uint8_t **heaps, size_t target_off, uint32_t bits, int pe) {
+    memcpy(heaps[pe] + target_off, &bits, 4); /* nvshmem_device.cu:445-448 */
+}
+void oracle_p64(uint8_t **heaps, size_t target_off, uint64_t bits, int pe) {
+    memcpy(heaps[pe] + target_off, &bits, 8);
+}
+uint32_t oracle_g32(uint8_t *const *heaps, size_t source_off, int pe) {
+    uint32_t v;
+    memcpy(&v, heaps[pe] + source_off, 4); /* nvshmem_device.cu:469-472 */
+    return v;
+}
+uint64_t oracle_g64(uint8_t *const *heaps, size_t source_off, int pe) {
+    uint64_t v;
+    memcpy(&v, heaps[pe] + source_off, 8);
+    return v;
+}
+
+/* ------------------------------------------------------------- waits -- */
+
+int oracle_int_wait_done(int observed, int cmp_value) {
+    return observed != cmp_value; /* nvshmem_device.cu:551: spin while == */
+}
+
+int oracle_int_wait_until_done(int observed, int cmp, int cmp_value) {
+    (void)cmp;                     /* README:170: "ignored currently" */
+    return observed == cmp_value;  /* nvshmem_device.cu:573: spin while != */
+}
+
+int oracle_cmp_holds(int v, int op, int ref) {
+    switch (op) {
+    case 0: return v == ref;
+    case 1: return v != ref;
+    case 2: return v > ref;
+    case 3: return v >= ref;
+    case 4: return v < ref;
+    case 5: return v <= ref;
+    default: return 0;
+    }
+}
+
+/* ----------------------------------------------------------- barrier -- */
+
+int oracle_ref_barrier_value(long long k) {
+    /* nvshmem.cu:139 starts at 1; :877 advances mod 64*1024 */
+    return (int)((1 + k) % 65536);
+}
+int oracle_ref_barrier_half(long long k) {
+    return oracle_ref_barrier_value(k) % 2; /* nvshmem.cu:873 */
+}
+
+void oracle_gen_barrier_init(oracle_gen_barrier_t *b, int npes) {
+    b->count = 0;
+    b->gen = 0;
+    b->npes = npes;
+}
+int oracle_gen_barrier_read_gen(const oracle_gen_barrier_t *b) { return b->gen; }
+int oracle_gen_barrier_arrive(oracle_gen_barrier_t *b) {
+    int old = b->count++;
+    if (old == b->npes - 1) {
+        b->count = 0;
+        b->gen++;
+        return 1;
+    }
+    return 0;
+}
+int oracle_gen_barrier_released(const oracle_gen_barrier_t *b, int gen_seen) { return b->gen != gen_seen; }
+
+/* ------------------------------------------------------------- trace -- */
+
+static const char *const kOps[8] = {"put_int", "put_float", "put_double", "get_int",
+                                    "get_float", "get_double", "wait_int", "wait_until_int"}; /* nvshmem.cu:117 */
+
+const char *oracle_trace_header(void) {
+    /* nvshmem.cu:390 */
+    return "# timestamp, gridid, blockid, tid, type, local_address, remote_address, size, remote_pe, timespent\n";
+}
+const char *oracle_trace_op_name(int type) { return (type >= 0 && type < 8) ? kOps[type] : "unknown"; }
+
+int oracle_trace_row(char *buf, size_t cap, long long tstamp_ns, unsigned long long gridid, int blockid, int tid,
+                     int type, uint64_t local_addr, uint64_t remote_addr, int size, int remote_pe,
+                     long long tspent_ns) {
+    /* nvshmem.cu:411-421 (block-level format) */
+    return snprintf(buf, cap, "%lld, %llu, %d, %d, %s, %p, %p, %d, %d, %lld\n", tstamp_ns, gridid, blockid, tid,
+                    oracle_trace_op_name(type), (void *)(uintptr_t)local_addr, (void *)(uintptr_t)remote_addr, size,
+                    remote_pe, tspent_ns);
+}

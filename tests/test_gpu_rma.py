@@ -1,0 +1,285 @@
+"""GPU parity of the data-movement path against the CPU oracle.
+
+All PEs live in this process (local mode): PE p's heap is one stride of one
+allocation and kernels translate symmetric addresses exactly as they would for
+IPC-mapped peers.  Sources are seeded PCG32 words from the oracle; the expected
+destination is the oracle's put/get applied to a host image of the same heap
+region (canary bytes around the target must survive untouched).  Bit-exact.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import nvshmem_amd as na
+
+pytestmark = pytest.mark.gpu
+
+CANARY = 0xA5
+GUARD = 256
+
+
+def _seed(*parts):
+    h = 0x9E3779B97F4A7C15
+    for p in parts:
+        h = (h * 0x100000001B3 ^ (hash(p) & 0xFFFFFFFF)) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def _src_bytes(oracle, nbytes, seed):
+    words = oracle.pcg32((nbytes + 3) // 4, seed)
+    return words.view(np.uint8)[:nbytes].copy()
+
+
+def test_device_pcg_matches_oracle(local_pes, hip, oracle):
+    lib = local_pes.start(2)
+    for n in [1, 63, 64, 4095, 4096, 4097, (1 << 20) + 5]:
+        d = hip.malloc(n * 4)
+        try:
+            lib.nvshmemh_fill_pcg32(d, n, 42, 54, None)
+            hip.sync()
+            got = hip.dtoh(d, n * 4, np.uint32)
+        finally:
+            hip.free(d)
+        np.testing.assert_array_equal(got, oracle.pcg32(n, 42, 54))
+
+
+def _check_region(hip, oracle, lib, dst_dev_region, image, nbytes_region):
+    got = hip.dtoh(dst_dev_region, nbytes_region)
+    if not np.array_equal(got, image):
+        bad = np.flatnonzero(got != image)
+        raise AssertionError(f"{bad.size} bytes differ, first at {bad[0]} (got {got[bad[0]]}, want {image[bad[0]]})")
+
+
+def _run_put(lib, hip, oracle, *, path, dtype, nelem, toff, soff, pe_src=0, pe_dst=1, seed=1, chunk=0):
+    """put `nelem` elements of `dtype` from a local buffer (PE pe_src's heap) into
+    PE pe_dst's symmetric buffer at byte offset toff; compare with the oracle."""
+    w = na.DTYPE_BYTES[dtype]
+    nbytes = nelem * w
+    region = nbytes + toff + 2 * GUARD
+    sym = lib.nvshmalloc(region)
+    src_sym = lib.nvshmalloc(nbytes + soff + 64)
+    dst_on_pe = lib.nvshmem_ptr(sym, pe_dst)
+    src_local = lib.nvshmem_ptr(src_sym, pe_src) + soff
+    data = _src_bytes(oracle, nbytes, seed)
+    hip.memset(dst_on_pe, CANARY, region)
+    if nbytes:
+        hip.htod(src_local, data)
+    target = sym + GUARD + toff  # symmetric address
+    if path == "grid":
+        getattr(lib, f"nvshmemx_{dtype}_put_on_stream")(target, src_local, nelem, pe_dst, None)
+    elif path == "thread":
+        lib.nvshmemh_thread_putget(0, na.DTYPES[dtype], target, src_local, nelem, chunk, pe_dst, None)
+    elif path in ("wave", "block"):
+        lib.nvshmemh_coop_putget(0, 0 if path == "wave" else 1, target, src_local, nbytes, chunk * w, pe_dst, 256,
+                                 None)
+    else:
+        raise ValueError(path)
+    hip.sync()
+    assert lib.nvshmemx_get_last_error() == 0
+    # oracle: a put into PE pe_dst's image of the region
+    images = [np.full(region, CANARY, np.uint8) for _ in range(2)]
+    oracle.put(images, GUARD + toff, data, 1)
+    _check_region(hip, oracle, lib, dst_on_pe, images[1], region)
+
+
+def _run_get(lib, hip, oracle, *, path, dtype, nelem, toff, soff, pe_me=0, pe_src=1, seed=2, chunk=0):
+    """get `nelem` elements from PE pe_src's symmetric buffer (byte offset soff)
+    into a local buffer at byte offset toff; compare with the oracle."""
+    w = na.DTYPE_BYTES[dtype]
+    nbytes = nelem * w
+    region = nbytes + toff + 2 * GUARD
+    src_sym = lib.nvshmalloc(nbytes + soff + 64)
+    dst_sym = lib.nvshmalloc(region)
+    data = _src_bytes(oracle, nbytes, seed)
+    if nbytes:
+        hip.htod(lib.nvshmem_ptr(src_sym, pe_src) + soff, data)
+    dst_local = lib.nvshmem_ptr(dst_sym, pe_me)
+    hip.memset(dst_local, CANARY, region)
+    target = dst_local + GUARD + toff
+    source = src_sym + soff
+    if path == "grid":
+        getattr(lib, f"nvshmemx_{dtype}_get_on_stream")(target, source, nelem, pe_src, None)
+    elif path == "thread":
+        lib.nvshmemh_thread_putget(1, na.DTYPES[dtype], target, source, nelem, chunk, pe_src, None)
+    elif path in ("wave", "block"):
+        lib.nvshmemh_coop_putget(1, 0 if path == "wave" else 1, target, source, nbytes, chunk * w, pe_src, 256, None)
+    hip.sync()
+    assert lib.nvshmemx_get_last_error() == 0
+    heap_img = [np.zeros(nbytes + soff + 64, np.uint8) for _ in range(2)]
+    heap_img[1][soff:soff + nbytes] = data
+    expect = np.full(region, CANARY, np.uint8)
+    view = expect[GUARD + toff:GUARD + toff + nbytes]
+    oracle.get(heap_img, view, soff, 1)
+    _check_region(hip, oracle, lib, dst_local, expect, region)
+
+
+@pytest.mark.parametrize("dtype", ["int", "float", "double"])
+@pytest.mark.parametrize("op", ["put", "get"])
+def test_grid_putget_sizes(local_pes, hip, oracle, dtype, op):
+    """host-launched grid copies, 8 B .. 64 MiB by powers of two"""
+    lib = local_pes.start(2, heap="512M")
+    w = na.DTYPE_BYTES[dtype]
+    nb = 8
+    while nb <= (64 << 20):
+        lib.nvshmcleanup()
+        fn = _run_put if op == "put" else _run_get
+        fn(lib, hip, oracle, path="grid", dtype=dtype, nelem=nb // w, toff=0, soff=0, seed=_seed(dtype, op, nb))
+        nb *= 2
+
+
+@pytest.mark.parametrize("path", ["grid", "thread", "wave", "block"])
+@pytest.mark.parametrize("op", ["put", "get"])
+def test_ragged_and_misaligned(local_pes, hip, oracle, path, op):
+    """empty, odd-length and relatively misaligned copies on every path"""
+    lib = local_pes.start(2, heap="256M")
+    fn = _run_put if op == "put" else _run_get
+    for dtype in ["int", "double"]:
+        w = na.DTYPE_BYTES[dtype]
+        for nelem in [0, 1, 3, 17, 1023, 4099]:
+            for toff, soff in [(0, 0), (w, 0), (0, w), (3 * w, w), (8, 12) if w == 4 else (8, 0)]:
+                lib.nvshmcleanup()
+                chunk = 0 if path == "grid" else max(1, nelem // 3)
+                fn(lib, hip, oracle, path=path, dtype=dtype, nelem=nelem, toff=toff, soff=soff, chunk=chunk,
+                   seed=_seed(path, op, dtype, nelem, toff, soff))
+
+
+@pytest.mark.parametrize("path,chunk", [("thread", 0), ("thread", 4096), ("wave", 65536), ("block", 1 << 20)])
+def test_device_api_bulk(local_pes, hip, oracle, path, chunk):
+    """per-thread (reference semantics), wave- and block-cooperative device
+    calls at 4 MiB, every dtype, both directions"""
+    lib = local_pes.start(2, heap="256M")
+    for dtype in ["int", "float", "double"]:
+        w = na.DTYPE_BYTES[dtype]
+        n = (4 << 20) // w if chunk else (256 << 10) // w
+        c = chunk // w if chunk else 0
+        lib.nvshmcleanup()
+        _run_put(lib, hip, oracle, path=path, dtype=dtype, nelem=n, toff=0, soff=0, chunk=c, seed=_seed(path, dtype))
+        lib.nvshmcleanup()
+        _run_get(lib, hip, oracle, path=path, dtype=dtype, nelem=n, toff=0, soff=0, chunk=c, seed=_seed(dtype, path))
+
+
+@pytest.mark.parametrize("dtype", ["int", "float", "double"])
+def test_p_and_g(local_pes, hip, oracle, dtype):
+    """single-element p and g with every bit pattern class (NaN payloads too)"""
+    lib = local_pes.start(4)
+    w = na.DTYPE_BYTES[dtype]
+    n = 3000
+    bits = oracle.pcg32(n * w // 4, _seed("pg", dtype)).view(np.uint8)
+    vals_dev = hip.malloc(n * w)
+    out_dev = hip.malloc(n * w)
+    try:
+        hip.htod(vals_dev, bits)
+        sym = lib.nvshmalloc(n * w)
+        for pe in range(4):
+            hip.memset(lib.nvshmem_ptr(sym, pe), 0, n * w)
+        lib.nvshmemh_pg(0, na.DTYPES[dtype], sym, vals_dev, None, n, 2, None)  # p into PE 2
+        hip.sync()
+        images = [np.zeros(n * w, np.uint8) for _ in range(4)]
+        for i in range(n):
+            if w == 4:
+                oracle.lib.oracle_p32((C.c_void_p * 4)(*[im.ctypes.data for im in images]), i * 4,
+                                      int(bits[i * 4:i * 4 + 4].view(np.uint32)[0]), 2)
+            else:
+                oracle.lib.oracle_p64((C.c_void_p * 4)(*[im.ctypes.data for im in images]), i * 8,
+                                      int(bits[i * 8:i * 8 + 8].view(np.uint64)[0]), 2)
+        for pe in range(4):
+            np.testing.assert_array_equal(hip.dtoh(lib.nvshmem_ptr(sym, pe), n * w), images[pe])
+        lib.nvshmemh_pg(1, na.DTYPES[dtype], sym, None, out_dev, n, 2, None)  # g from PE 2
+        hip.sync()
+        np.testing.assert_array_equal(hip.dtoh(out_dev, n * w), images[2])
+        assert lib.nvshmemx_get_last_error() == 0
+    finally:
+        hip.free(vals_dev)
+        hip.free(out_dev)
+
+
+def test_pingpong_local(local_pes, hip):
+    """p + wait round trips between two workgroup PEs: every round trip completes"""
+    lib = local_pes.start(2)
+    iters = 10000
+    flag = lib.nvshmalloc(64)
+    for pe in range(2):
+        hip.memset(lib.nvshmem_ptr(flag, pe), 0, 64)
+    samples = hip.malloc(iters * 8)
+    try:
+        lib.nvshmemh_pingpong_local(flag, 0, 1, iters, None, 0, samples, None)
+        hip.sync()
+        s = hip.dtoh(samples, iters * 8, np.int64)
+    finally:
+        hip.free(samples)
+    assert lib.nvshmemx_get_last_error() == 0
+    assert (s > 0).all()
+    # both flags end at the last round number
+    for pe in range(2):
+        assert hip.dtoh(lib.nvshmem_ptr(flag, pe), 4, np.int32)[0] == iters
+    khz = lib.nvshmemx_wallclock_khz()
+    p50 = float(np.percentile(s, 50)) * 1e6 / khz
+    print(f"local p+wait round trip p50 {p50:.0f} ns p99 {float(np.percentile(s, 99)) * 1e6 / khz:.0f} ns")
+
+
+def test_quiet_flag_ordering(local_pes, hip):
+    """payload published by block put + quiet + p is complete when the flag
+    is seen (sequence counter k never observed with stale payload)"""
+    lib = local_pes.start(2)
+    n, rounds = 16384, 300
+    data = lib.nvshmalloc(n * 4)
+    stage = lib.nvshmalloc(n * 4)
+    flag = lib.nvshmalloc(64)
+    ack = lib.nvshmalloc(64)
+    for pe in range(2):
+        for p in (data, stage, flag, ack):
+            hip.memset(lib.nvshmem_ptr(p, pe), 0, 64 if p in (flag, ack) else n * 4)
+    viol = hip.malloc(16)
+    try:
+        hip.memset(viol, 0, 16)
+        lib.nvshmemh_order_local(data, stage, flag, ack, n, rounds, 0, 1, viol, None)
+        hip.sync()
+        v = hip.dtoh(viol, 16, np.uint64)
+    finally:
+        hip.free(viol)
+    assert lib.nvshmemx_get_last_error() == 0
+    assert v[0] == 0 and v[1] == 0, v
+    assert hip.dtoh(lib.nvshmem_ptr(data, 1), n * 4, np.int32).tolist() == [rounds] * n
+
+
+@pytest.mark.parametrize("npes,wgs", [(8, 1), (8, 4), (64, 1)])
+def test_workgroup_pe_barrier(local_pes, hip, npes, wgs):
+    """device barrier across workgroup PEs of one persistent kernel: no PE
+    observes a slot older than the round it is in"""
+    lib = local_pes.start(npes, wgs_per_pe=wgs, heap="16M")
+    slots = lib.nvshmalloc(npes * 4)
+    for pe in range(npes):
+        hip.memset(lib.nvshmem_ptr(slots, pe), 0, npes * 4)
+    viol = hip.malloc(8)
+    try:
+        hip.memset(viol, 0, 8)
+        lib.nvshmemh_wg_barrier_check(slots, 50, viol, 128, None)
+        hip.sync()
+        v = int(hip.dtoh(viol, 8, np.uint64)[0])
+    finally:
+        hip.free(viol)
+    assert lib.nvshmemx_get_last_error() == 0
+    assert v == 0
+
+
+@pytest.mark.parametrize("until", [0, 1])
+def test_wait_timeout_is_reported(local_pes, hip, until):
+    """a wait nobody satisfies ends at the configured budget with the sticky
+    error bit set (the reference spins forever)"""
+    lib = local_pes.start(2, timeout_ms=5000)
+    lib.nvshmemx_set_wait_timeout_ms(50)
+    try:
+        flag = lib.nvshmalloc(64)
+        p = lib.nvshmem_ptr(flag, 0)
+        hip.memset(p, 0, 64)
+        out = hip.malloc(4)
+        # while-equal wait on 0 (stays 0), or until-equal wait on 7 (never 7)
+        lib.nvshmemh_wait_once(p, 0 if until == 0 else 7, until, out, None)
+        hip.sync()
+        hip.free(out)
+        assert lib.nvshmemx_get_last_error() & na.ERR_WAIT_TIMEOUT
+        lib.nvshmemx_clear_errors()
+        assert lib.nvshmemx_get_last_error() == 0
+    finally:
+        lib.nvshmemx_set_wait_timeout_ms(5000)
