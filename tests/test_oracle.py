@@ -1,0 +1,247 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+The reference ships no tests and no golden vectors, so each oracle function is
+pinned by a published known-answer vector, a closed form, an invariant, or a
+brute-force enumeration, chosen so that a plausible slip (off-by-one bound,
+wrong sign, transposed argument) fails.  No GPU needed.
+"""
+import itertools
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+# ---------------------------------------------------------------- PCG32
+
+def test_pcg32_published_known_answer(oracle):
+    """pcg32-demo from the PCG minimal C reference: srandom(42, 54), first six
+    outputs of round 1 (published with the library)."""
+    kat = json.loads((GOLDEN / "pcg32_kat.json").read_text())
+    got = oracle.pcg32(len(kat["outputs"]), kat["seed"], kat["seq"])
+    assert [int(x) for x in got] == [int(x, 16) for x in kat["outputs"]]
+
+
+def test_pcg32_fixture_hashes(oracle):
+    """first words and a digest of longer streams, committed as fixtures"""
+    import hashlib
+    fx = json.loads((GOLDEN / "pcg32_streams.json").read_text())
+    for case in fx["cases"]:
+        w = oracle.pcg32(case["n"], case["seed"], case["seq"])
+        assert [int(x) for x in w[:8]] == case["first8"]
+        assert hashlib.sha256(w.tobytes()).hexdigest() == case["sha256"]
+
+
+def test_uniform_pm1_is_exact_and_in_range(oracle):
+    n = 1 << 16
+    f = oracle.uniform_pm1(n, 42)
+    w = oracle.pcg32(n, 42)
+    assert f.dtype == np.float32 and (f >= -1).all() and (f < 1).all()
+    # closed form: (w >> 8) * 2^-23 - 1, every value exactly representable
+    np.testing.assert_array_equal(f.astype(np.float64), (w >> 8).astype(np.float64) * 2.0 ** -23 - 1.0)
+
+
+def test_uniform01_f64_is_exact_and_in_range(oracle):
+    n = 1 << 12
+    d = oracle.uniform01_f64(n, 9)
+    w = oracle.pcg32(2 * n, 9).astype(np.uint64)
+    u = ((w[0::2] << np.uint64(32)) | w[1::2]) >> np.uint64(11)
+    np.testing.assert_array_equal(d, u.astype(np.float64) * 2.0 ** -53)
+    assert (d >= 0).all() and (d < 1).all()
+
+
+# ---------------------------------------------------------------- heap
+
+def test_heap_reference_arithmetic(oracle):
+    """align 1 restates nvshmalloc exactly: offsets are prefix sums; a request
+    fails iff tail + size > heap (an exact fit succeeds)"""
+    h = oracle.heap(100, 1)
+    assert [oracle.nvshmalloc(h, s) for s in (10, 0, 30, 60)] == [0, 10, 10, 40]
+    assert oracle.nvshmalloc(h, 1) == -1          # 100 + 1 > 100
+    oracle.lib.oracle_nvshmcleanup(h)
+    assert oracle.nvshmalloc(h, 100) == 0          # exact fit after cleanup
+    assert oracle.nvshmalloc(h, 0) == 100          # zero-size at the end still fits
+
+
+def test_heap_aligned_and_symmetric(oracle):
+    rng = np.random.default_rng(5)
+    sizes = [int(x) for x in rng.integers(0, 5000, 200)]
+    a, b = oracle.heap(1 << 20, 256), oracle.heap(1 << 20, 256)
+    prev_end = 0
+    for s in sizes:
+        oa, ob = oracle.nvshmalloc(a, s), oracle.nvshmalloc(b, s)
+        assert oa == ob                       # same call sequence -> same offsets on every PE
+        if oa < 0:
+            break
+        assert oa % 256 == 0 and oa >= prev_end and oa - prev_end < 256
+        prev_end = oa + s
+
+
+# ---------------------------------------------------------------- translation
+
+def test_translate_process_mode_is_reference_formula(oracle):
+    bases = [0x7F0000000000, 0x7E1234560000, 0x7D0000001000]
+    mine = bases[1]
+    for off in [0, 1, 255, 4096, (1 << 30) - 8]:
+        for pe in range(3):
+            assert oracle.translate(bases, mine, (1 << 64) - 1, mine + off, pe) == bases[pe] + off
+
+
+def test_translate_local_mode_any_pe_coordinates(oracle):
+    stride = 1 << 28
+    base0 = 0x7F0000000000 + 12345 * 256     # allocation base need not be stride-aligned
+    bases = [base0 + i * stride for i in range(8)]
+    for off in [0, 8, stride - 4]:
+        for src_pe in range(8):
+            for pe in range(8):
+                got = oracle.translate(bases, base0, stride - 1, bases[src_pe] + off, pe)
+                assert got == bases[pe] + off
+                # composition: translating twice lands where translating once does
+                assert oracle.translate(bases, base0, stride - 1, got, (pe + 3) % 8) == bases[(pe + 3) % 8] + off
+
+
+# ---------------------------------------------------------------- data movement
+
+def test_put_get_closed_form(oracle):
+    rng = np.random.default_rng(11)
+    heaps = [np.zeros(4096, np.uint8) for _ in range(3)]
+    before = [h.copy() for h in heaps]
+    src = rng.integers(0, 256, 777, dtype=np.uint8)
+    oracle.put(heaps, 123, src, 2)
+    assert np.array_equal(heaps[2][123:900], src)
+    assert np.array_equal(heaps[2][:123], before[2][:123]) and np.array_equal(heaps[2][900:], before[2][900:])
+    assert np.array_equal(heaps[0], before[0]) and np.array_equal(heaps[1], before[1])
+    back = np.zeros(777, np.uint8)
+    oracle.get(heaps, back, 123, 2)
+    assert np.array_equal(back, src)
+
+
+# ---------------------------------------------------------------- waits
+
+def test_wait_predicates_follow_readme(oracle):
+    L = oracle.lib
+    for obs, cmp_value in itertools.product([-2, -1, 0, 1, 7], repeat=2):
+        # README:157-164: nvshmem_int_wait blocks while *ivar == cmp_value
+        assert L.oracle_int_wait_done(obs, cmp_value) == (obs != cmp_value)
+        # README:166-174: wait_until blocks until *ivar == cmp_value; cmp ignored
+        for cmp in range(-1, 7):
+            assert L.oracle_int_wait_until_done(obs, cmp, cmp_value) == (obs == cmp_value)
+        ops = [obs == cmp_value, obs != cmp_value, obs > cmp_value, obs >= cmp_value, obs < cmp_value,
+               obs <= cmp_value]
+        assert [L.oracle_cmp_holds(obs, op, cmp_value) for op in range(6)] == [int(x) for x in ops]
+
+
+# ---------------------------------------------------------------- barrier
+
+def test_reference_barrier_value_sequence(oracle):
+    L = oracle.lib
+    vals = [L.oracle_ref_barrier_value(k) for k in range(65540)]
+    assert vals[:3] == [1, 2, 3] and vals[65534] == 65535 and vals[65535] == 0 and vals[65536] == 1
+    assert all(L.oracle_ref_barrier_half(k) == vals[k] % 2 for k in range(0, 65540, 997))
+
+
+def _explore_barrier(npes, episodes, arrive, read_gen, released, init):
+    """Enumerate every interleaving of the per-PE steps [read gen, arrive,
+    wait-until-released] for `episodes` barrier episodes and check that no PE
+    leaves episode e before all PEs arrived in e.  Returns #states visited."""
+    seen = set()
+    stack = [((0,) * npes, (None,) * npes, (0,) * npes, init())]
+    visited = 0
+    while stack:
+        phase, gen_seen, ep, state = stack.pop()
+        key = (phase, gen_seen, ep, tuple(state))
+        if key in seen:
+            continue
+        seen.add(key)
+        visited += 1
+        for p in range(npes):
+            if ep[p] >= episodes:
+                continue
+            st = list(state)
+            ph, gs, e = list(phase), list(gen_seen), list(ep)
+            if ph[p] == 0:
+                gs[p] = read_gen(st)
+                ph[p] = 1
+            elif ph[p] == 1:
+                arrive(st)
+                ph[p] = 2
+            else:
+                if not released(st, gs[p]):
+                    continue
+                # leaving episode e: every PE must have arrived in e
+                arrived = [(ep[q] > e[p]) or (ep[q] == e[p] and phase[q] == 2) for q in range(npes)]
+                assert all(arrived), f"PE {p} released early in episode {e[p]}"
+                ph[p], e[p] = 0, e[p] + 1
+            stack.append((tuple(ph), tuple(gs), tuple(e), tuple(st)))
+    return visited
+
+
+def _gen_barrier_model(npes, release_at):
+    def init():
+        return [0, 0]  # count, gen
+
+    def read_gen(st):
+        return st[1]
+
+    def arrive(st):
+        old = st[0]
+        st[0] += 1
+        if old == release_at:
+            st[0] = 0
+            st[1] += 1
+
+    def released(st, g):
+        return st[1] != g
+    return init, read_gen, arrive, released
+
+
+@pytest.mark.parametrize("npes", [1, 2, 3, 4])
+def test_generation_barrier_exhaustive(oracle, npes):
+    """the counter/generation barrier (oracle state machine) never releases a
+    PE early, over every interleaving of 3 episodes"""
+    b = oracle.gen_barrier(npes)
+    L = oracle.lib
+
+    def init():
+        L.oracle_gen_barrier_init(b, npes)
+        return [0, 0]
+
+    def read_gen(st):
+        b[0], b[1] = st
+        return L.oracle_gen_barrier_read_gen(b)
+
+    def arrive(st):
+        b[0], b[1] = st
+        L.oracle_gen_barrier_arrive(b)
+        st[0], st[1] = b[0], b[1]
+
+    def released(st, g):
+        b[0], b[1] = st
+        return bool(L.oracle_gen_barrier_released(b, g))
+    assert _explore_barrier(npes, 3, arrive, read_gen, released, init) > 0
+
+
+def test_barrier_checker_catches_planted_mistake():
+    """the exhaustive checker fails a barrier that releases one arrival early
+    (mistake planted in a CPU model, never on the GPU)"""
+    init, read_gen, arrive, released = _gen_barrier_model(3, release_at=1)
+    with pytest.raises(AssertionError, match="released early"):
+        _explore_barrier(3, 2, arrive, read_gen, released, init)
+
+
+# ---------------------------------------------------------------- trace CSV
+
+def test_trace_csv_known_answer(oracle):
+    """header text of nvshmem.cu:390 and a row of the nvshmem.cu:411 format"""
+    assert oracle.lib.oracle_trace_header().decode() == (
+        "# timestamp, gridid, blockid, tid, type, local_address, remote_address, size, remote_pe, timespent\n")
+    row = oracle.trace_row(123, 456, 7, 8, 0, 0x1000, 0x2000, 4, 1, 9)
+    assert row == "123, 456, 7, 8, put_int, 0x1000, 0x2000, 4, 1, 9\n"
+    row = oracle.trace_row(5, 6, 0, 0, 7, 0x10, 0, 1, 0, 250)
+    assert row == "5, 6, 0, 0, wait_until_int, 0x10, (nil), 1, 0, 250\n"
+    names = [oracle.lib.oracle_trace_op_name(i).decode() for i in range(8)]
+    assert names == ["put_int", "put_float", "put_double", "get_int", "get_float", "get_double", "wait_int",
+                     "wait_until_int"]
