@@ -56,7 +56,7 @@ NVSHMEMI_DEF_G(double, double, NVSHMEMI_OP_GET_DOUBLE)
     extern "C" __device__ void nvshmem_##Name##_put(Type *target, Type *source, int len, int pe) { \
         NVSHMEMI_TRACE(OpType, source, target, len, pe, 0);                                \
         if (len <= 0) return;                                                               \
-        copy<16, 4, false>(translate(target, pe), source, (size_t)len * sizeof(Type), 0, 1);   \
+        copy<16, 4>(translate(target, pe), source, (size_t)len * sizeof(Type), 0, 1);   \
     }
 NVSHMEMI_DEF_PUT(int, int, NVSHMEMI_OP_PUT_INT)
 NVSHMEMI_DEF_PUT(float, float, NVSHMEMI_OP_PUT_FLOAT)
@@ -66,7 +66,7 @@ NVSHMEMI_DEF_PUT(double, double, NVSHMEMI_OP_PUT_DOUBLE)
     extern "C" __device__ void nvshmem_##Name##_get(Type *target, Type *source, int len, int pe) { \
         NVSHMEMI_TRACE(OpType, target, source, len, pe, 0);                                \
         if (len <= 0) return;                                                               \
-        copy<16, 4, false>(target, translate(source, pe), (size_t)len * sizeof(Type), 0, 1);   \
+        copy<16, 4>(target, translate(source, pe), (size_t)len * sizeof(Type), 0, 1);   \
     }
 NVSHMEMI_DEF_GET(int, int, NVSHMEMI_OP_GET_INT)
 NVSHMEMI_DEF_GET(float, float, NVSHMEMI_OP_GET_FLOAT)
@@ -110,10 +110,10 @@ __device__ __forceinline__ void active_lanes(unsigned &rank, unsigned &count) {
 __device__ __forceinline__ void wave_copy(void *dst, const void *src, size_t nbytes) {
     unsigned r, n;
     active_lanes(r, n);
-    copy<16, 4, false>(dst, src, nbytes, r, n);
+    copy<16, 4>(dst, src, nbytes, r, n);
 }
 __device__ __forceinline__ void block_copy(void *dst, const void *src, size_t nbytes) {
-    copy<16, 4, false>(dst, src, nbytes, flat_tid(), block_threads());
+    copy<16, 4>(dst, src, nbytes, flat_tid(), block_threads());
     fence_release();
     __syncthreads();
 }

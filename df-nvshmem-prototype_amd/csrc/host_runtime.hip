@@ -386,6 +386,7 @@ extern "C" void nvshmemx_putmem_config_on_stream(void *target, const void *sourc
         if (cfg->vec_bytes > 0) c.vec_bytes = cfg->vec_bytes;
         if (cfg->unroll > 0) c.unroll = cfg->unroll;
         c.nontemporal = cfg->nontemporal;
+        c.mode = cfg->mode;
     }
     launch_copy(nvshmem_ptr(target, pe), source, nbytes, c, s);
 }

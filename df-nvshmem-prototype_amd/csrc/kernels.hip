@@ -22,45 +22,73 @@
 
 namespace nvshmemi {
 
-template <int W, int U, bool NT>
+/* MODE 0: all lanes of the grid interleave over the whole range (grid-stride);
+ * MODE 1: each workgroup streams one contiguous slice. */
+template <int W, int U, int NT, int MODE>
 __global__ void __launch_bounds__(1024) k_copy(char *dst, const char *src, size_t nbytes) {
-    const size_t lane = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t n = (size_t)gridDim.x * blockDim.x;
-    copy<W, U, NT>(dst, src, nbytes, lane, n);
+    if constexpr (MODE == 0) {
+        const size_t lane = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const size_t n = (size_t)gridDim.x * blockDim.x;
+        copy<W, U, NT>(dst, src, nbytes, lane, n);
+    } else {
+        copy_chunked<W, U, NT>(dst, src, nbytes, blockIdx.x, gridDim.x, threadIdx.x, blockDim.x);
+    }
 }
 
 typedef void (*copy_kernel_t)(char *, const char *, size_t);
 
-template <int W, bool NT> static copy_kernel_t pick_unroll(int u) {
+template <int W, int NT, int MODE> static copy_kernel_t pick_unroll(int u) {
     switch (u) {
-    case 1: return k_copy<W, 1, NT>;
-    case 2: return k_copy<W, 2, NT>;
-    case 8: return k_copy<W, 8, NT>;
-    default: return k_copy<W, 4, NT>;
+    case 1: return k_copy<W, 1, NT, MODE>;
+    case 2: return k_copy<W, 2, NT, MODE>;
+    case 8: return k_copy<W, 8, NT, MODE>;
+    default: return k_copy<W, 4, NT, MODE>;
     }
+}
+
+template <int W, int MODE> static copy_kernel_t pick_nt(int nt, int u) {
+    switch (nt & 3) {
+    case NT_LOAD: return pick_unroll<W, NT_LOAD, MODE>(u);
+    case NT_STORE: return pick_unroll<W, NT_STORE, MODE>(u);
+    case NT_BOTH: return pick_unroll<W, NT_BOTH, MODE>(u);
+    default: return pick_unroll<W, NT_NONE, MODE>(u);
+    }
+}
+
+template <int MODE> static copy_kernel_t pick_width(const CopyConfig &c) {
+    if (c.vec_bytes >= 16) return pick_nt<16, MODE>(c.nontemporal, c.unroll);
+    if (c.vec_bytes >= 8) return pick_nt<8, MODE>(c.nontemporal, c.unroll);
+    return pick_nt<4, MODE>(c.nontemporal, c.unroll);
 }
 
 static copy_kernel_t pick_copy_kernel(const CopyConfig &c) {
-    if (c.nontemporal) {
-        if (c.vec_bytes >= 16) return pick_unroll<16, true>(c.unroll);
-        if (c.vec_bytes >= 8) return pick_unroll<8, true>(c.unroll);
-        return pick_unroll<4, true>(c.unroll);
-    }
-    if (c.vec_bytes >= 16) return pick_unroll<16, false>(c.unroll);
-    if (c.vec_bytes >= 8) return pick_unroll<8, false>(c.unroll);
-    return pick_unroll<4, false>(c.unroll);
+    return c.mode == 1 ? pick_width<1>(c) : pick_width<0>(c);
 }
 
+/* Launch shape by size, from a sweep of {interleaved, chunked} x block
+ * {256,512,1024} x grid {2K..32K} x unroll {2,4,8} x NT {none,load,store,both}
+ * on MI355X (copies rotated over a 1 GiB footprint so the 256 MB MALL cannot
+ * serve them).  Non-temporal loads and stores won at every size >= 64 MiB. */
 CopyConfig default_copy_config(size_t nbytes) {
     CopyConfig c;
-    c.block = 256;
     c.vec_bytes = 16;
-    c.unroll = 4;
-    c.nontemporal = 0;
+    c.nontemporal = NT_BOTH;
+    if (nbytes >= ((size_t)512 << 20)) {
+        c.mode = 1, c.block = 1024, c.workgroups = 8192, c.unroll = 8;
+        return c;
+    }
+    if (nbytes >= ((size_t)128 << 20)) {
+        c.mode = 0, c.block = 512, c.workgroups = 16384, c.unroll = 8;
+        return c;
+    }
+    c.mode = 0;
+    c.block = nbytes >= ((size_t)16 << 20) ? 512 : 256;
+    c.unroll = nbytes >= ((size_t)16 << 20) ? 8 : 4;
+    if (nbytes < ((size_t)8 << 20)) c.nontemporal = NT_NONE; /* small puts are read again soon */
     const size_t per_wg = (size_t)c.block * c.vec_bytes * c.unroll;
     size_t wgs = (nbytes + per_wg - 1) / per_wg;
     if (wgs < 1) wgs = 1;
-    if (wgs > 2048) wgs = 2048; /* 8 workgroups per CU on 256 CUs */
+    if (wgs > 4096) wgs = 4096;
     c.workgroups = (int)wgs;
     return c;
 }

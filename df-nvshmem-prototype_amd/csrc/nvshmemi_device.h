@@ -142,43 +142,76 @@ template <bool NT, typename V> __device__ __forceinline__ void vstore(V *p, V v)
     else *p = v;
 }
 
-/* Copy with `n` cooperating lanes (lane index `lane`) using W-byte accesses.
- * Requires (dst ^ src) % W == 0.  The unaligned head (< W bytes) and tail are
- * copied byte-wise by the first lanes; the body is streamed with UNROLL
- * independent loads in flight per lane, lanes interleaved so each wavefront
- * touches 64*W contiguous bytes per access. */
-template <int W, int UNROLL, bool NT>
-__device__ __forceinline__ void copy_w(char *dst, const char *src, size_t nbytes, size_t lane, size_t n) {
-    typedef typename vec<W>::t V;
-    size_t head = (size_t)((W - ((uintptr_t)dst & (W - 1))) & (W - 1));
-    if (head > nbytes) head = nbytes;
-    const size_t nvec = (nbytes - head) / W;
-    const size_t tail_start = head + nvec * W;
-    const size_t tail = nbytes - tail_start;
-    if (W > 1) {
-        for (size_t i = lane; i < head; i += n) dst[i] = src[i];
-        for (size_t i = lane; i < tail; i += n) dst[tail_start + i] = src[tail_start + i];
-    }
-    const V *s = (const V *)(src + head);
-    V *d = (V *)(dst + head);
-    size_t i = lane;
+/* Non-temporal policy bits: NT_LOAD streams loads past the caches' normal
+ * retention, NT_STORE does the same for stores (global_load/store ... nt). */
+enum { NT_NONE = 0, NT_LOAD = 1, NT_STORE = 2, NT_BOTH = 3 };
+
+/* Stream vectors [lo, hi) with `n` cooperating lanes (lane index `lane`),
+ * UNROLL independent loads in flight per lane; lanes are interleaved so each
+ * wavefront access covers 64 * sizeof(V) contiguous bytes. */
+template <int UNROLL, int NT, typename V>
+__device__ __forceinline__ void stream_vectors(V *d, const V *s, size_t lo, size_t hi, size_t lane, size_t n) {
+    size_t i = lo + lane;
     if (UNROLL > 1) {
         const size_t step = n * UNROLL;
-        for (; i + (UNROLL - 1) * n < nvec; i += step) {
+        for (; i + (UNROLL - 1) * n < hi; i += step) {
             V r[UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) r[u] = vload<NT>(s + i + u * n);
+            for (int u = 0; u < UNROLL; ++u) r[u] = vload<(NT & NT_LOAD) != 0>(s + i + u * n);
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) vstore<NT>(d + i + u * n, r[u]);
+            for (int u = 0; u < UNROLL; ++u) vstore<(NT & NT_STORE) != 0>(d + i + u * n, r[u]);
         }
     }
-    for (; i < nvec; i += n) vstore<NT>(d + i, vload<NT>(s + i));
+    for (; i < hi; i += n) vstore<(NT & NT_STORE) != 0>(d + i, vload<(NT & NT_LOAD) != 0>(s + i));
+}
+
+/* Split of a W-aligned copy: `head` unaligned leading bytes, `nvec` W-byte
+ * vectors, then the trailing bytes from `tail_start`.  Requires
+ * (dst ^ src) % W == 0 so that one head aligns both sides. */
+template <int W> struct split {
+    size_t head, nvec, tail_start, tail;
+    __device__ __forceinline__ split(const char *dst, size_t nbytes) {
+        head = (size_t)((W - ((uintptr_t)dst & (W - 1))) & (W - 1));
+        if (head > nbytes) head = nbytes;
+        nvec = (nbytes - head) / W;
+        tail_start = head + nvec * W;
+        tail = nbytes - tail_start;
+    }
+};
+
+/* Interleaved copy: `n` lanes with lane index `lane` share the whole range. */
+template <int W, int UNROLL, int NT>
+__device__ __forceinline__ void copy_w(char *dst, const char *src, size_t nbytes, size_t lane, size_t n) {
+    typedef typename vec<W>::t V;
+    const split<W> sp(dst, nbytes);
+    if (W > 1) {
+        for (size_t i = lane; i < sp.head; i += n) dst[i] = src[i];
+        for (size_t i = lane; i < sp.tail; i += n) dst[sp.tail_start + i] = src[sp.tail_start + i];
+    }
+    stream_vectors<UNROLL, NT>((V *)(dst + sp.head), (const V *)(src + sp.head), 0, sp.nvec, lane, n);
+}
+
+/* Chunked copy: workgroup `part` of `parts` streams one contiguous slice of
+ * the vectors with its `n` threads; part 0 also copies head and tail bytes. */
+template <int W, int UNROLL, int NT>
+__device__ __forceinline__ void copy_w_chunked(char *dst, const char *src, size_t nbytes, size_t part, size_t parts,
+                                               size_t lane, size_t n) {
+    typedef typename vec<W>::t V;
+    const split<W> sp(dst, nbytes);
+    if (W > 1 && part == 0) {
+        for (size_t i = lane; i < sp.head; i += n) dst[i] = src[i];
+        for (size_t i = lane; i < sp.tail; i += n) dst[sp.tail_start + i] = src[sp.tail_start + i];
+    }
+    const size_t per = (sp.nvec + parts - 1) / parts;
+    const size_t lo = part * per;
+    const size_t hi = lo + per < sp.nvec ? lo + per : sp.nvec;
+    if (lo < hi) stream_vectors<UNROLL, NT>((V *)(dst + sp.head), (const V *)(src + sp.head), lo, hi, lane, n);
 }
 
 /* Pick the widest access width <= MAXW that the relative alignment of dst and
  * src allows, then copy.  Every lane of the cooperating group must call this
  * with identical arguments. */
-template <int MAXW, int UNROLL, bool NT>
+template <int MAXW, int UNROLL, int NT = NT_NONE>
 __device__ __forceinline__ void copy(void *dstv, const void *srcv, size_t nbytes, size_t lane, size_t n) {
     char *dst = (char *)dstv;
     const char *src = (const char *)srcv;
@@ -188,6 +221,22 @@ __device__ __forceinline__ void copy(void *dstv, const void *srcv, size_t nbytes
     else if (MAXW >= 4 && (rel & 3) == 0) copy_w<(MAXW >= 4 ? 4 : 1), UNROLL, NT>(dst, src, nbytes, lane, n);
     else if ((rel & 1) == 0) copy_w<2, UNROLL, NT>(dst, src, nbytes, lane, n);
     else copy_w<1, UNROLL, NT>(dst, src, nbytes, lane, n);
+}
+
+template <int MAXW, int UNROLL, int NT = NT_NONE>
+__device__ __forceinline__ void copy_chunked(void *dstv, const void *srcv, size_t nbytes, size_t part, size_t parts,
+                                             size_t lane, size_t n) {
+    char *dst = (char *)dstv;
+    const char *src = (const char *)srcv;
+    const uintptr_t rel = (uintptr_t)dst ^ (uintptr_t)src;
+    if (MAXW >= 16 && (rel & 15) == 0)
+        copy_w_chunked<(MAXW >= 16 ? 16 : 1), UNROLL, NT>(dst, src, nbytes, part, parts, lane, n);
+    else if (MAXW >= 8 && (rel & 7) == 0)
+        copy_w_chunked<(MAXW >= 8 ? 8 : 1), UNROLL, NT>(dst, src, nbytes, part, parts, lane, n);
+    else if (MAXW >= 4 && (rel & 3) == 0)
+        copy_w_chunked<(MAXW >= 4 ? 4 : 1), UNROLL, NT>(dst, src, nbytes, part, parts, lane, n);
+    else if ((rel & 1) == 0) copy_w_chunked<2, UNROLL, NT>(dst, src, nbytes, part, parts, lane, n);
+    else copy_w_chunked<1, UNROLL, NT>(dst, src, nbytes, part, parts, lane, n);
 }
 
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }

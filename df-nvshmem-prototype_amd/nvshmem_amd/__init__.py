@@ -96,7 +96,8 @@ SIGNATURES = {
 
 class CopyConfig(C.Structure):
     """nvshmemx_copy_config_t (include/nvshmemx.h)."""
-    _fields_ = [("workgroups", _I), ("block", _I), ("vec_bytes", _I), ("unroll", _I), ("nontemporal", _I)]
+    _fields_ = [("workgroups", _I), ("block", _I), ("vec_bytes", _I), ("unroll", _I), ("nontemporal", _I),
+                ("mode", _I)]
 
 
 def load(trace: bool = False) -> C.CDLL:

@@ -76,7 +76,8 @@ typedef struct nvshmemx_copy_config {
     int block;       /* threads per workgroup (64..1024, multiple of 64) */
     int vec_bytes;   /* widest access: 4, 8 or 16 */
     int unroll;      /* independent accesses in flight per lane: 1, 2, 4 or 8 */
-    int nontemporal; /* 1: non-temporal loads and stores */
+    int nontemporal; /* bit 0: non-temporal loads, bit 1: non-temporal stores */
+    int mode;        /* 0: lanes of the whole grid interleave; 1: one contiguous slice per workgroup */
 } nvshmemx_copy_config_t;
 
 void nvshmemx_putmem_config_on_stream(void *target, const void *source, size_t nbytes, int pe,

@@ -1,0 +1,101 @@
+"""One PE process of a multi-process test job (shared-memory bootstrap, heaps
+IPC-mapped between processes on the same GPU).  Launched by
+test_gpu_multiprocess.py; prints one JSON line with what it observed.
+
+Scenarios (every PE runs the same collective sequence):
+  putget   ring exchange: PE p puts seeded words into PE p+1, gets from PE p+1;
+           checked bit-exact against the oracle's generator
+  pingpong p + wait round trips between PE 0 and PE 1 (process kernels)
+  barrier  rounds of: publish round r to every PE with p, offload barrier,
+           check every slot >= r (sequence-counter ordering)
+"""
+import json
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT / "df-nvshmem-prototype_amd"))
+sys.path.insert(0, str(ROOT / "tests"))
+
+import numpy as np  # noqa: E402
+
+import nvshmem_amd as na  # noqa: E402
+
+
+def main():
+    scenario = sys.argv[1]
+    lib, hip = na.load(), na.Hip()
+    lib.nvstart_pes()
+    me, n = lib.nvshmem_my_pe(), lib.nvshmem_n_pes()
+    out = {"pe": me, "npes": n, "mode": lib.nvshmemx_mode(), "bootstrap": lib.nvshmemx_bootstrap_name().decode()}
+    if scenario == "putget":
+        import oracle_py
+        orc = oracle_py.Oracle()
+        sizes = [4, 12, 4096, 65540, 1 << 20, (8 << 20) + 12]
+        bufs = [lib.nvshmalloc(s) for s in sizes]
+        gots = [lib.nvshmalloc(s) for s in sizes]
+        srcs = []
+        for i, s in enumerate(sizes):
+            data = orc.pcg32(s // 4, 1000 * me + i).view(np.uint8)
+            src = hip.malloc(s)
+            hip.htod(src, data)
+            srcs.append(src)
+            hip.memset(bufs[i], 0, s)
+        hip.sync()
+        lib.nvshmem_barrier_all()
+        right = (me + 1) % n
+        for i, s in enumerate(sizes):
+            lib.nvshmemx_putmem_on_stream(bufs[i], srcs[i], s, right, None)
+        lib.nvshmem_barrier_all()
+        left = (me - 1) % n
+        ok_put = all(np.array_equal(hip.dtoh(bufs[i], s), orc.pcg32(s // 4, 1000 * left + i).view(np.uint8))
+                     for i, s in enumerate(sizes))
+        # get back what PE `right` received (it came from me)
+        for i, s in enumerate(sizes):
+            lib.nvshmemx_getmem_on_stream(gots[i], bufs[i], s, right, None)
+        hip.sync()
+        ok_get = all(np.array_equal(hip.dtoh(gots[i], s), orc.pcg32(s // 4, 1000 * me + i).view(np.uint8))
+                     for i, s in enumerate(sizes))
+        # device-side per-thread put into the right neighbour's second-to-last buffer
+        lib.nvshmem_barrier_all()
+        out.update(put_ok=ok_put, get_ok=ok_get)
+    elif scenario == "pingpong":
+        iters = int(sys.argv[2])
+        flag = lib.nvshmalloc(256)
+        hip.memset(flag, 0, 256)
+        hip.sync()
+        lib.nvshmem_barrier_all()
+        samples = hip.malloc(iters * 8)
+        if me in (0, 1):
+            lib.nvshmemh_pingpong_process(flag, 1 - me, 1 if me == 0 else 0, iters, None, 0, samples, None)
+        hip.sync()
+        if me == 0:
+            s = hip.dtoh(samples, iters * 8, np.int64).astype(np.float64) * 1e6 / lib.nvshmemx_wallclock_khz()
+            out.update(p50_ns=float(np.percentile(s, 50)), p99_ns=float(np.percentile(s, 99)))
+        out["final_flag"] = int(hip.dtoh(flag, 4, np.int32)[0])
+        lib.nvshmem_barrier_all()
+    elif scenario == "barrier":
+        rounds = int(sys.argv[2])
+        slots = lib.nvshmalloc(n * 4)
+        hip.memset(slots, 0, n * 4)
+        viol = hip.malloc(8)
+        hip.memset(viol, 0, 8)
+        hip.sync()
+        lib.nvshmem_barrier_all()
+        for r in range(1, rounds + 1):
+            lib.nvshmemh_publish_round(slots, r, None)
+            lib.nvshmem_barrier_all_offload()
+            lib.nvshmemh_check_round(slots, r, viol, None)
+            if r % 10 == 0 and lib.nvshmemx_get_last_error():
+                break  # a timed-out barrier: stop after one budget, not one per round
+        hip.sync()
+        out["violations"] = int(hip.dtoh(viol, 8, np.uint64)[0])
+        out["slots"] = hip.dtoh(slots, n * 4, np.int32).tolist()
+        lib.nvshmem_barrier_all()
+    out["errors"] = lib.nvshmemx_get_last_error()
+    lib.nvstop_pes()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
