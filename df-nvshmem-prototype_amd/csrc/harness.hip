@@ -291,6 +291,32 @@ __global__ void k_wait_once(int *ivar, int cmp_value, int until, int *out) {
     out[0] = nvshmemi::load_relaxed(ivar);
 }
 
+/* traced sequence (one workgroup, PE `me` talks to `pe`): one put outside the
+ * begin/end bracket (must not be recorded), then `reps` x {int put(16),
+ * float put(8), double get(4), int p, int g, int wait, int wait_until}, then
+ * one more put after trace_end (not recorded either) */
+__global__ void k_traced_ops(int *ints_sym, float *floats_sym, double *doubles_sym, int *flag_sym, int me, int pe,
+                             int reps) {
+    int *mine = (int *)nvshmemx_ptr_d(ints_sym, me);
+    int *myflag = (int *)nvshmemx_ptr_d(flag_sym, me);
+    if (threadIdx.x == 0) nvshmem_int_put(ints_sym, mine, 4, pe);
+    nvshmem_trace_begin();
+    if (threadIdx.x == 0) {
+        for (int r = 0; r < reps; ++r) {
+            nvshmem_int_put(ints_sym, mine, 16, pe);
+            nvshmem_float_put(floats_sym, (float *)mine, 8, pe);
+            nvshmem_double_get((double *)nvshmemx_ptr_d(doubles_sym, me), doubles_sym, 4, pe);
+            nvshmem_int_p(flag_sym + 1, r, pe);
+            (void)nvshmem_int_g(flag_sym + 1, pe);
+            nvshmem_quiet();
+            nvshmem_int_wait(myflag, 12345);            /* *myflag == 7: returns at once */
+            nvshmem_int_wait_until(myflag, 0, 7);
+        }
+    }
+    nvshmem_trace_end();
+    if (threadIdx.x == 0) nvshmem_int_put(ints_sym, mine, 4, pe);
+}
+
 } // namespace
 
 /* --------------------------------------------------------- C entry points -- */
@@ -393,6 +419,14 @@ void nvshmemh_wg_barrier_check(int *slots_sym, int rounds, unsigned long long *v
 
 void nvshmemh_wait_once(int *ivar, int cmp_value, int until, int *out, hipStream_t s) {
     hipLaunchKernelGGL(k_wait_once, dim3(1), dim3(64), 0, s, ivar, cmp_value, until, out);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+void nvshmemh_traced_ops(int *ints_sym, float *floats_sym, double *doubles_sym, int *flag_sym, int me, int pe,
+                         int reps, int block, hipStream_t s) {
+    /* one workgroup: in local mode it is PE 0, so callers pass me = 0 there */
+    hipLaunchKernelGGL(k_traced_ops, dim3(1), dim3(block), 0, s, ints_sym, floats_sym, doubles_sym, flag_sym, me, pe,
+                       reps);
     NVSHMEMI_HIP(hipGetLastError());
 }
 

@@ -1,0 +1,45 @@
+"""The reference's caller pattern as drop-in MPI programs (examples/*.hip,
+linked against libnvshmem.a with relocatable device code), run as 2 MPI
+processes on the one GPU."""
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parent.parent
+EX = ROOT / "df-nvshmem-prototype_amd" / "build" / "examples"
+MPIRUN = Path("/opt/conda/bin/mpirun")
+
+
+def mpirun(exe, n=2, timeout=120):
+    if not MPIRUN.exists():
+        pytest.skip("MPICH mpirun not present")
+    assert (EX / exe).exists(), f"{exe} not built"
+    p = subprocess.run([str(MPIRUN), "-n", str(n), str(EX / exe)], capture_output=True, text=True, timeout=timeout,
+                       env={"PATH": "/usr/bin:/bin", "HSA_ENABLE_IPC_MODE_LEGACY": "0",
+                            "NVSHMEM_HEAP_SIZE": "64M", "NVSHMEM_WAIT_TIMEOUT_MS": "5000"})
+    return p
+
+
+def test_dropin_put_flag_latency():
+    p = mpirun("shmem_latency")
+    assert p.returncode == 0, p.stdout + p.stderr[-2000:]
+    rows = [l.split() for l in p.stdout.splitlines() if re.match(r"^\s*\d+\s", l)]
+    assert [int(r[0]) for r in rows] == [4 << k for k in range(15)]       # 4 B .. 64 KiB
+    assert all(r[3] == "ok" for r in rows), p.stdout
+    print(p.stdout)
+
+
+def test_dropin_sync_latency():
+    p = mpirun("shmem_sync_latency")
+    assert p.returncode == 0, p.stdout + p.stderr[-2000:]
+    m = re.search(r"sync latency: ([\d.]+) ns (\d+) cycles flags (\d+) (\d+)", p.stdout)
+    assert m and int(m.group(3)) == int(m.group(4)) == 10100, p.stdout
+    print(p.stdout)
+
+
+def test_dropin_wrong_pe_count_is_reported():
+    p = mpirun("shmem_sync_latency", n=1)
+    assert p.returncode != 0 and "exactly two processes" in p.stderr

@@ -1,0 +1,64 @@
+"""Tracing build on the GPU: records only between trace_begin/trace_end, in
+issue order with non-decreasing timestamps, written as the reference's CSV
+(header nvshmem.cu:390, row format nvshmem.cu:411, file name nvshmem.cu:377);
+a ring too small for the run reports overflow instead of overwriting."""
+import json
+import os
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = Path(__file__).resolve().parent
+ROW = re.compile(r"^(-?\d+), (\d+), (\d+), (\d+), (\w+), (0x[0-9a-f]+|\(nil\)), (0x[0-9a-f]+|\(nil\)), (-?\d+), "
+                 r"(-?\d+), (-?\d+)$")
+SEQ = [("put_int", 16), ("put_float", 8), ("get_double", 4), ("put_int", 1), ("get_int", 1), ("wait_int", 1),
+       ("wait_until_int", 1)]
+
+
+def run(tmp_path, reps, extra=None):
+    env = dict(os.environ, NVSHMEM_TRACE_DIR=str(tmp_path), NVSHMEM_HEAP_SIZE="16M")
+    env.update(extra or {})
+    p = subprocess.run([sys.executable, str(HERE / "trace_worker.py"), str(reps)], env=env, capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 0, p.stderr[-3000:]
+    files = sorted(tmp_path.glob("nvshmem_processed_tracefile_*_0"))
+    assert len(files) == 1, (list(tmp_path.iterdir()), p.stdout)
+    assert not list(tmp_path.glob("nvshmem_tracefile_*")), "spill file not removed"
+    return p.stdout, json.loads([l for l in p.stdout.splitlines() if l.startswith("{")][-1]), files[0]
+
+
+def test_trace_records_and_csv(tmp_path, oracle):
+    reps = 5
+    stdout, info, f = run(tmp_path, reps)
+    assert info["enabled"] == 1 and info["errors"] == 0
+    lines = f.read_text().splitlines(keepends=True)
+    assert lines[0] == oracle.lib.oracle_trace_header().decode()
+    rows = [ROW.match(l.rstrip("\n")) for l in lines[1:]]
+    assert all(rows), lines[1:4]
+    got = [(m.group(5), int(m.group(8))) for m in rows]
+    assert got == SEQ * reps                     # nothing outside begin/end, issue order kept
+    ts = [int(m.group(1)) for m in rows]
+    assert ts == sorted(ts) and ts[0] > 0
+    assert {int(m.group(9)) for m in rows if not m.group(5).startswith("wait")} == {1}   # remote PE
+    assert {int(m.group(3)) for m in rows} == {0} and {int(m.group(4)) for m in rows} == {0}
+    # each row is exactly the oracle's formatting of its own fields
+    for m, line in zip(rows, lines[1:]):
+        loc = 0 if m.group(6) == "(nil)" else int(m.group(6), 16)
+        rem = 0 if m.group(7) == "(nil)" else int(m.group(7), 16)
+        op = [oracle.lib.oracle_trace_op_name(i).decode() for i in range(8)].index(m.group(5))
+        assert oracle.trace_row(int(m.group(1)), int(m.group(2)), int(m.group(3)), int(m.group(4)), op, loc, rem,
+                                int(m.group(8)), int(m.group(9)), int(m.group(10))) == line
+    assert "overflow: 0 conflict: 0" in stdout
+
+
+def test_trace_overflow_is_reported(tmp_path):
+    # ring of 8 records (640 B x 1 buffer rounds down to a power of two)
+    stdout, info, f = run(tmp_path, 10, {"NVSHMEM_LOG_BUFFER_SIZE": "640", "NVSHMEM_LOG_BUFFER_COUNT": "1"})
+    assert "overflow: 1" in stdout
+    rows = f.read_text().splitlines()[1:]
+    assert len(rows) == 8
+    assert [r.split(", ")[4] for r in rows] == [s[0] for s in SEQ] + [SEQ[0][0]]
