@@ -69,6 +69,167 @@ def worker(name):
     print(json.dumps({"pe1_put_bitexact": ok, "pe1_errors": err}), flush=True)
 
 
+BATCH_NPES, BATCH_WGS, BATCH_SRC, BATCH_DST, BATCH_REQS, BATCH_SEED = 64, 4, 32 << 20, 64 << 20, 4096, 7
+
+
+def batch_worker():
+    """Uneven-put workload on 64 workgroup-partitioned PEs in one process:
+    4096 puts per step, log-uniform 64 B..4 MiB, seed 7.  Times one fused
+    launch, the same under a captured hipGraph, the owner discipline (each PE's
+    workgroups run the puts it initiated) and the per-call path (one launch per
+    put).  Prints one JSON line."""
+    import ctypes as C
+    import nvshmem_amd as na
+    os.environ["NVSHMEM_HEAP_SIZE"] = "96M"
+    lib, hip = na.load(), na.Hip()
+    lib.nvshmemx_start_local_pes(BATCH_NPES, BATCH_WGS)
+    base = lib.nvshmalloc(BATCH_SRC + BATCH_DST)
+    for pe in range(BATCH_NPES):
+        p = lib.nvshmem_ptr(base, pe)
+        lib.nvshmemh_fill_pcg32(p, BATCH_SRC // 4, 100 + pe, 54, None)
+        hip.memset(p + BATCH_SRC, 0, BATCH_DST)
+    hip.sync()
+    reqs = na.gen_requests(lib, BATCH_SEED, BATCH_REQS, BATCH_NPES, BATCH_SRC, BATCH_DST)
+    arr = na.make_requests(lib, reqs, base, 0, BATCH_SRC)
+    n = len(arr)
+    payload = int(reqs["nbytes"].sum())
+    out = {"requests": n, "payload_bytes": payload, "pes": BATCH_NPES}
+
+    def plan_of(launch=None, choices=None):
+        plan = C.c_void_p()
+        lib.nvshmemx_batch_plan_create(arr, n, choices, C.byref(launch) if launch is not None else None,
+                                       C.byref(plan))
+        return plan
+
+    plan = plan_of()
+    out["fused_ms"] = timed(hip, lambda: lib.nvshmemx_batch_plan_launch(plan, None), 20, 3)
+    out["pieces"] = int(lib.nvshmemx_batch_plan_pieces(plan))
+    out["bitexact"] = int(lib.nvshmemh_verify_requests(arr, n)) == 0
+    lib.nvshmemx_batch_plan_capture(plan, 10)
+    out["graph_ms"] = timed(hip, lambda: lib.nvshmemx_batch_plan_graph_launch(plan, None), 3, 1) / 10
+    lib.nvshmemx_batch_plan_destroy(plan)
+    owner = plan_of(na.BatchLaunch(0, 256, 3, na.BATCH_OWNER))
+    out["owner_ms"] = timed(hip, lambda: lib.nvshmemx_batch_plan_launch(owner, None), 10, 2)
+    lib.nvshmemx_batch_plan_destroy(owner)
+    out["per_call_ms"] = timed(hip, lambda: lib.nvshmemx_put_requests_on_stream(arr, n, None), 3, 1)
+    out["bitexact"] = out["bitexact"] and int(lib.nvshmemh_verify_requests(arr, n)) == 0
+    out["errors"] = lib.nvshmemx_get_last_error()
+    lib.nvstop_pes()
+    print(json.dumps(out), flush=True)
+
+
+def learned_worker(n_batches, train_steps):
+    """Learned config choice on the uneven-put workload (64 workgroup PEs):
+    1. exhaustive autotuner: every one of the 54 candidates measured at every
+       point of a 990-point feature grid (53,460 samples);
+    2. the MLP cost model trained on those samples on the GPU;
+    3. `n_batches` held-out batches (seeds 1000..) of 4096 puts each, planned
+       with the MLP's choices, the table's choices (exhaustive autotuner) and
+       the library's untuned rule, each timed as one fused launch; the per-call
+       path is timed on the first 10.  Prints one JSON line."""
+    import ctypes as C
+    import numpy as np
+    import nvshmem_amd as na
+    os.environ["NVSHMEM_HEAP_SIZE"] = "96M"
+    lib, hip = na.load(), na.Hip()
+    lib.nvshmemx_start_local_pes(BATCH_NPES, BATCH_WGS)
+    base = lib.nvshmalloc(BATCH_SRC + BATCH_DST)
+    for pe in range(BATCH_NPES):
+        lib.nvshmemh_fill_pcg32(lib.nvshmem_ptr(base, pe), BATCH_SRC // 4, 100 + pe, 54, None)
+    hip.sync()
+    out = {"batches": n_batches, "requests_per_batch": BATCH_REQS}
+    t0 = time.time()
+    grid = na.tune_grid()
+    times = na.tune_measure(lib, grid, reps=3)
+    feats = na.tune_features(lib, grid)
+    out["tune_points"], out["tune_samples"] = int(len(times)), int(times.size)
+    out["tune_seconds"] = round(time.time() - t0, 1)
+    t0 = time.time()
+    model = na.CostModel(lib, seed=1)
+    T = np.log(np.maximum(times, 1e-3)).astype(np.float32)
+    model.train(feats, T, train_steps, 3e-3)
+    model.train(feats, T, train_steps // 2, 1e-3)
+    pred = model.predict(feats)
+    out["train_seconds"] = round(time.time() - t0, 1)
+    out["train_mse_log"] = round(float(np.mean((pred - T) ** 2)), 5)
+    best_t = times.min(axis=1)
+    regret = times[np.arange(len(times)), pred.argmin(axis=1)] / best_t
+    out["grid_top1_agreement"] = round(float(np.mean(pred.argmin(axis=1) == times.argmin(axis=1))), 3)
+    out["grid_mean_regret"] = round(float(np.mean(regret)), 4)
+
+    def timed_plan(arr, n, choices):
+        plan = C.c_void_p()
+        lib.nvshmemx_batch_plan_create(arr, n, choices, None, C.byref(plan))
+        ms = timed(hip, lambda: lib.nvshmemx_batch_plan_launch(plan, None), 2, 1)
+        lib.nvshmemx_batch_plan_destroy(plan)
+        return ms
+
+    rec = {"mlp": [], "table": [], "rule": [], "per_call": []}
+    bad = 0
+    t0 = time.time()
+    for b in range(n_batches):
+        reqs = na.gen_requests(lib, 1000 + b, BATCH_REQS, BATCH_NPES, BATCH_SRC, BATCH_DST)
+        arr = na.make_requests(lib, reqs, base, 0, BATCH_SRC)
+        n = len(arr)
+        ch_mlp = model.choose(arr, n, n)
+        ch_tab = (na.BatchChoice * n)()
+        lib.nvshmemx_tune_table_choose(feats.ctypes.data, times.ctypes.data, len(times), arr, n, n, ch_tab)
+        rec["mlp"].append(timed_plan(arr, n, ch_mlp))
+        rec["table"].append(timed_plan(arr, n, ch_tab))
+        rec["rule"].append(timed_plan(arr, n, None))
+        if b < 10:
+            rec["per_call"].append(timed(hip, lambda: lib.nvshmemx_put_requests_on_stream(arr, n, None), 1, 0))
+        if b < 3:
+            bad += int(lib.nvshmemh_verify_requests(arr, n))
+    out["eval_seconds"] = round(time.time() - t0, 1)
+    m, t, r = (np.array(rec[k]) for k in ("mlp", "table", "rule"))
+    pc = np.array(rec["per_call"])
+    out.update({
+        "mean_ms": {k: round(float(np.mean(v)), 4) for k, v in rec.items()},
+        "mlp_over_table": round(float(np.mean(m) / np.mean(t)), 4),
+        "mlp_within_5pct_of_table_frac": round(float(np.mean(m <= 1.05 * t)), 3),
+        "mlp_gain_over_rule": round(float(np.mean(r) / np.mean(m)), 4),
+        "mlp_speedup_vs_per_call": round(float(np.mean(pc) / np.mean(m[:len(pc)])), 2),
+        "bitexact": bad == 0,
+        "errors": lib.nvshmemx_get_last_error(),
+    })
+    model.close()
+    lib.nvstop_pes()
+    print(json.dumps(out), flush=True)
+
+
+def run_learned_leg(n_batches, train_steps):
+    env = {k: v for k, v in os.environ.items() if not k.startswith("NVSHMEM_")}
+    try:
+        p = subprocess.run([sys.executable, __file__, "--learned-worker", "--learned-batches", str(n_batches),
+                            "--train-steps", str(train_steps)], capture_output=True, text=True, timeout=900, env=env)
+        lines = [l for l in p.stdout.splitlines() if l.startswith("{")]
+        if p.returncode != 0 or not lines:
+            return {"error": f"learned worker exited {p.returncode}: {p.stderr[-800:]}"}
+        return json.loads(lines[-1])
+    except Exception as exc:
+        return {"error": str(exc)[-500:]}
+
+
+def run_batch_leg():
+    try:
+        env = {k: v for k, v in os.environ.items() if not k.startswith("NVSHMEM_")}
+        p = subprocess.run([sys.executable, __file__, "--batch-worker"], capture_output=True, text=True, timeout=600,
+                           env=env)
+        lines = [l for l in p.stdout.splitlines() if l.startswith("{")]
+        if p.returncode != 0 or not lines:
+            return {"error": f"batch worker exited {p.returncode}: {p.stderr[-800:]}"}
+        rec = json.loads(lines[-1])
+    except Exception as exc:
+        return {"error": str(exc)[-500:]}
+    best = min(rec["fused_ms"], rec["graph_ms"])
+    rec["fused_gbps"] = round(2 * rec["payload_bytes"] / (best * 1e-3) / 1e9, 1)
+    rec["speedup_vs_per_call"] = round(rec["per_call_ms"] / best, 2)
+    for k in ("fused_ms", "graph_ms", "owner_ms", "per_call_ms"):
+        rec[k] = round(rec[k], 4)
+    return rec
+
+
 def timed(hip, fn, steps, warmup):
     e0, e1 = hip.event(), hip.event()
     for _ in range(warmup):
@@ -111,10 +272,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--layout", choices=["process", "local"], default="process")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--pe-worker", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--batch-worker", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--learned-worker", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--learned-batches", type=int, default=1000)
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
     if args.pe_worker:
         worker(args.pe_worker)
+        return
+    if args.batch_worker:
+        batch_worker()
+        return
+    if args.learned_worker:
+        learned_worker(args.learned_batches, args.train_steps)
         return
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: all PEs share one GPU by design; --gpus {args.gpus} is not supported",
@@ -198,6 +371,8 @@ def main():
         if shm.exists():
             shm.unlink()
 
+    batched = None if args.no_batch else run_batch_leg()
+    learned = None if args.no_learned else run_learned_leg(args.learned_batches, args.train_steps)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     r = result
     line = {
@@ -227,6 +402,8 @@ def main():
                           "2 workgroup PEs"},
         "bitexact": {"put": r["put_ok"], "get": r["get_ok"]},
         "device_errors": r["errors"],
+        "batched": batched,
+        "learned": learned,
     }
     print(json.dumps(line), flush=True)
     if not (r["put_ok"] and r["get_ok"]) or r["errors"]:

@@ -12,7 +12,10 @@
  * formula (O(log n) multiply-adds), so a wavefront writes 64 consecutive words
  * per store.
  */
+#include <algorithm>
+#include <math.h>
 #include <stdint.h>
+#include <vector>
 
 #include "nvshmem.h"
 #include "nvshmem_device.h"
@@ -317,9 +320,47 @@ __global__ void k_traced_ops(int *ints_sym, float *floats_sym, double *doubles_s
     if (threadIdx.x == 0) nvshmem_int_put(ints_sym, mine, 4, pe);
 }
 
+/* bench self-check: workgroup i compares request i's target bytes with its
+ * source bytes (the copy is bitwise), counting mismatching bytes */
+struct verify_item {
+    const unsigned char *src;
+    const unsigned char *dst;
+    unsigned long long nbytes;
+};
+__global__ void k_verify(const verify_item *items, int n, unsigned long long *bad) {
+    for (int r = blockIdx.x; r < n; r += gridDim.x) {
+        const verify_item it = items[r];
+        unsigned long long local = 0;
+        for (unsigned long long b = threadIdx.x; b < it.nbytes; b += blockDim.x) local += it.src[b] != it.dst[b];
+        if (local) atomicAdd(bad, local);
+    }
+}
+
 } // namespace
 
 /* --------------------------------------------------------- C entry points -- */
+
+/* Counts bytes whose target differs from the source over all requests
+ * (synchronous). */
+extern "C" unsigned long long nvshmemh_verify_requests(const nvshmemx_put_request_t *reqs, int n) {
+    std::vector<verify_item> items((size_t)n);
+    for (int i = 0; i < n; ++i)
+        items[(size_t)i] = verify_item{(const unsigned char *)reqs[i].source,
+                                       (const unsigned char *)nvshmem_ptr(reqs[i].target, reqs[i].pe),
+                                       (unsigned long long)reqs[i].nbytes};
+    verify_item *d_items = nullptr;
+    unsigned long long *d_bad = nullptr, bad = 0;
+    NVSHMEMI_HIP(hipMalloc((void **)&d_items, std::max<size_t>(1, (size_t)n) * sizeof(verify_item)));
+    NVSHMEMI_HIP(hipMalloc((void **)&d_bad, sizeof(bad)));
+    NVSHMEMI_HIP(hipMemset(d_bad, 0, sizeof(bad)));
+    if (n) NVSHMEMI_HIP(hipMemcpy(d_items, items.data(), (size_t)n * sizeof(verify_item), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_verify, dim3(2048), dim3(256), 0, 0, d_items, n, d_bad);
+    NVSHMEMI_HIP(hipGetLastError());
+    NVSHMEMI_HIP(hipMemcpy(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost));
+    NVSHMEMI_HIP(hipFree(d_items));
+    NVSHMEMI_HIP(hipFree(d_bad));
+    return bad;
+}
 
 extern "C" {
 
@@ -346,6 +387,45 @@ void nvshmemh_pcg32_host(uint32_t *out, size_t nwords, unsigned long long seed, 
         out[i] = pcg_output(st);
         st = st * kPcgMult + inc;
     }
+}
+
+/* The uneven-put workload of the batch benchmark, drawn from one PCG32
+ * stream (seed, stream 54), five draws per request in this order: size
+ * (log-uniform on [2^min_log2, 2^max_log2], rounded down to 4 bytes), element
+ * type (int32 / f32), source PE, destination PE (uniform among the others),
+ * 256-byte aligned source offset within `src_region`.  Destinations are packed
+ * in request order per destination PE, 256-byte aligned, so they never
+ * overlap.  Returns 0, or -1 when a destination region would overflow. */
+int nvshmemh_gen_requests(unsigned long long seed, int nreq, int npes, unsigned long long src_region,
+                          unsigned long long dst_region, int min_log2, int max_log2, int *src_pe, int *dst_pe,
+                          unsigned long long *src_off, unsigned long long *dst_off, unsigned long long *nbytes,
+                          int *dtype) {
+    if (npes < 2 || npes > 1024) return -2;
+    uint64_t inc = 0;
+    uint64_t st = pcg_seeded_state(seed, 54, inc);
+    std::vector<unsigned long long> cursor((size_t)npes, 0ull);
+    for (int i = 0; i < nreq; ++i) {
+        uint32_t draw[5];
+        for (uint32_t &d : draw) {
+            d = pcg_output(st);
+            st = st * kPcgMult + inc;
+        }
+        const double e = (double)min_log2 + (double)(max_log2 - min_log2) * ((double)draw[0] / 4294967296.0);
+        unsigned long long size = ((unsigned long long)exp2(e)) & ~3ull;
+        if (size < 4) size = 4;
+        if (size > src_region) return -1;
+        const int s = (int)(draw[2] % (uint32_t)npes);
+        const int d = (s + 1 + (int)(draw[3] % (uint32_t)(npes - 1))) % npes;
+        src_pe[i] = s;
+        dst_pe[i] = d;
+        dtype[i] = (int)(draw[1] & 1u);
+        nbytes[i] = size;
+        src_off[i] = (unsigned long long)(draw[4] % ((src_region - size) / 256 + 1)) * 256ull;
+        dst_off[i] = cursor[(size_t)d];
+        cursor[(size_t)d] = (cursor[(size_t)d] + size + 255) / 256 * 256;
+        if (cursor[(size_t)d] > dst_region) return -1;
+    }
+    return 0;
 }
 
 /* dtype: 0 int, 1 float, 2 double; op: 0 put, 1 get */

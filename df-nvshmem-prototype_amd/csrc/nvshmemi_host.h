@@ -18,6 +18,7 @@
 
 #include "nvshmemi_state.h"
 #include "nvshmemi_trace.h"
+#include "nvshmemx.h"
 
 namespace nvshmemi {
 
@@ -96,6 +97,20 @@ struct CopyConfig {
 };
 CopyConfig default_copy_config(size_t nbytes);
 void launch_copy(void *dst, const void *src, size_t nbytes, const CopyConfig &cfg, hipStream_t s);
+
+/* batch.hip: plans over requests whose targets are already resolved to this
+ * process's address of the target heap (also used by the autotuner, which
+ * rebuilds one plan per candidate over scratch buffers) */
+struct resolved_req {
+    const char *src;
+    char *dst;
+    size_t nbytes;
+    int set;
+};
+void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, const nvshmemx_batch_choice_t *choices,
+               const nvshmemx_batch_choice_t *one_choice, int nsets, int wgs_per_set, int block, int nontemporal);
+void plan_launch(nvshmemx_batch_plan *p, hipStream_t s);
+nvshmemx_batch_plan *plan_new();
 
 /* trace_host.hip (no-ops unless built with NVSHMEM_TRACING); trace_init
  * returns the device-side ring descriptor, or nullptr when tracing is off */

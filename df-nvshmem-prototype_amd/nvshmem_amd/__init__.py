@@ -95,6 +95,179 @@ SIGNATURES = {
 }
 
 
+class PutRequest(C.Structure):
+    """nvshmemx_put_request_t"""
+    _fields_ = [("target", _P), ("source", _P), ("nbytes", _SZ), ("pe", _I), ("elem_bytes", _I)]
+
+
+class BatchChoice(C.Structure):
+    """nvshmemx_batch_choice_t"""
+    _fields_ = [("pieces", _I), ("vec_bytes", _I), ("mode", _I)]
+
+
+class BatchLaunch(C.Structure):
+    """nvshmemx_batch_launch_t"""
+    _fields_ = [("workgroups", _I), ("block", _I), ("nontemporal", _I), ("discipline", _I)]
+
+
+BATCH_SHARED, BATCH_OWNER = 0, 1
+
+SIGNATURES.update({
+    "nvshmemx_batch_plan_create": (_I, [_P, _I, _P, _P, C.POINTER(_P)]),
+    "nvshmemx_batch_plan_launch": (None, [_P, _P]),
+    "nvshmemx_batch_plan_capture": (_I, [_P, _I]),
+    "nvshmemx_batch_plan_graph_launch": (None, [_P, _P]),
+    "nvshmemx_batch_plan_pieces": (_SZ, [_P]),
+    "nvshmemx_batch_plan_bytes": (_SZ, [_P]),
+    "nvshmemx_batch_plan_destroy": (None, [_P]),
+    "nvshmemx_batch_default_choice": (None, [_SZ, _I, _P]),
+    "nvshmemx_put_requests_on_stream": (None, [_P, _I, _P]),
+    "nvshmemh_gen_requests": (_I, [_U64, _I, _I, _U64, _U64, _I, _I, _P, _P, _P, _P, _P, _P]),
+    "nvshmemh_verify_requests": (_U64, [_P, _I]),
+    "nvshmemx_tune_candidate_count": (_I, []),
+    "nvshmemx_tune_candidate": (None, [_I, _P]),
+    "nvshmemx_align_class": (_I, [_P, _P]),
+    "nvshmemx_tune_features": (None, [C.c_double, _I, _I, _I, _P]),
+    "nvshmemx_tune_measure": (_I, [_I, _P, _P, _P, _P, _I, _I, _I, _P]),
+    "nvshmemx_tune_table_choose": (_I, [_P, _P, _I, _P, _I, _I, _P]),
+    "nvshmemx_costmodel_create": (_I, [_I, _U64, C.POINTER(_P)]),
+    "nvshmemx_costmodel_num_params": (_I, [_P]),
+    "nvshmemx_costmodel_get_params": (None, [_P, _P]),
+    "nvshmemx_costmodel_set_params": (None, [_P, _P]),
+    "nvshmemx_costmodel_gradient": (None, [_P, _P, _P, _I, _P]),
+    "nvshmemx_costmodel_train": (None, [_P, _P, _P, _I, _I, C.c_float]),
+    "nvshmemx_costmodel_predict": (None, [_P, _P, _I, _P]),
+    "nvshmemx_costmodel_choose": (_I, [_P, _P, _I, _I, _P]),
+    "nvshmemx_costmodel_save": (_I, [_P, C.c_char_p]),
+    "nvshmemx_costmodel_load": (_I, [_P, C.c_char_p]),
+    "nvshmemx_costmodel_destroy": (None, [_P]),
+})
+
+NCAND = 54
+
+
+def candidates(lib):
+    """list of (pieces, vec_bytes, mode) for every autotuner candidate"""
+    out = []
+    for i in range(lib.nvshmemx_tune_candidate_count()):
+        c = BatchChoice()
+        lib.nvshmemx_tune_candidate(i, C.byref(c))
+        out.append((c.pieces, c.vec_bytes, c.mode))
+    return out
+
+
+def tune_grid(half_octaves=True, widths=(4, 8), concurrency=(1, 8, 64, 512, 4096), aligns=(0, 1, 2),
+              min_log2=6, max_log2=22):
+    """feature points of the autotuner's exhaustive table"""
+    import numpy as np
+    step = 0.5 if half_octaves else 1.0
+    sizes = np.arange(min_log2, max_log2 + 1e-9, step)
+    pts = [(float(s), w, k, a) for s in sizes for w in widths for k in concurrency for a in aligns]
+    return {"log2_bytes": np.array([p[0] for p in pts], np.float64), "elem_bytes": np.array([p[1] for p in pts],
+                                                                                           np.int32),
+            "concurrency": np.array([p[2] for p in pts], np.int32), "align": np.array([p[3] for p in pts], np.int32)}
+
+
+def tune_measure(lib, grid, reps=3, workgroups=1024, block=256):
+    """times_us[npts, 54] from on-device measurement of every candidate"""
+    import numpy as np
+    n = len(grid["log2_bytes"])
+    out = np.zeros((n, NCAND), np.float32)
+    lib.nvshmemx_tune_measure(n, grid["log2_bytes"].ctypes.data, grid["elem_bytes"].ctypes.data,
+                              grid["concurrency"].ctypes.data, grid["align"].ctypes.data, reps, workgroups, block,
+                              out.ctypes.data)
+    return out
+
+
+def tune_features(lib, grid):
+    import numpy as np
+    n = len(grid["log2_bytes"])
+    f = np.zeros((n, 4), np.float32)
+    for i in range(n):
+        lib.nvshmemx_tune_features(float(grid["log2_bytes"][i]), int(grid["elem_bytes"][i]),
+                                   int(grid["concurrency"][i]), int(grid["align"][i]), f[i].ctypes.data)
+    return f
+
+
+class CostModel:
+    """The library's MLP cost model (all compute in the library's HIP kernels)."""
+
+    def __init__(self, lib, seed=1, ncand=NCAND):
+        self.lib = lib
+        self.h = C.c_void_p()
+        lib.nvshmemx_costmodel_create(ncand, seed, C.byref(self.h))
+        self.ncand = ncand
+        self.nparams = lib.nvshmemx_costmodel_num_params(self.h)
+
+    def params(self):
+        import numpy as np
+        p = np.zeros(self.nparams, np.float32)
+        self.lib.nvshmemx_costmodel_get_params(self.h, p.ctypes.data)
+        return p
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, np.float32)
+        self.lib.nvshmemx_costmodel_set_params(self.h, p.ctypes.data)
+
+    def gradient(self, X, T):
+        import numpy as np
+        X, T = np.ascontiguousarray(X, np.float32), np.ascontiguousarray(T, np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        self.lib.nvshmemx_costmodel_gradient(self.h, X.ctypes.data, T.ctypes.data, len(X), g.ctypes.data)
+        return g
+
+    def train(self, X, T, steps, lr=3e-3):
+        import numpy as np
+        X, T = np.ascontiguousarray(X, np.float32), np.ascontiguousarray(T, np.float32)
+        self.lib.nvshmemx_costmodel_train(self.h, X.ctypes.data, T.ctypes.data, len(X), steps, lr)
+
+    def predict(self, X):
+        import numpy as np
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((len(X), self.ncand), np.float32)
+        self.lib.nvshmemx_costmodel_predict(self.h, X.ctypes.data, len(X), Y.ctypes.data)
+        return Y
+
+    def choose(self, reqs_arr, n, concurrency):
+        out = (BatchChoice * n)()
+        self.lib.nvshmemx_costmodel_choose(self.h, reqs_arr, n, concurrency, out)
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.nvshmemx_costmodel_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def gen_requests(lib, seed, nreq, npes, src_region, dst_region, min_log2=6, max_log2=22):
+    """The uneven-put workload (harness generator); returns a dict of numpy arrays."""
+    import numpy as np
+    out = {k: np.zeros(nreq, dt) for k, dt in [("src_pe", np.int32), ("dst_pe", np.int32), ("src_off", np.uint64),
+                                               ("dst_off", np.uint64), ("nbytes", np.uint64), ("dtype", np.int32)]}
+    rc = lib.nvshmemh_gen_requests(seed, nreq, npes, src_region, dst_region, min_log2, max_log2,
+                                   *[out[k].ctypes.data for k in ("src_pe", "dst_pe", "src_off", "dst_off", "nbytes",
+                                                                  "dtype")])
+    if rc != 0:
+        raise ValueError(f"request generation failed ({rc}): destination region too small")
+    return out
+
+
+def make_requests(lib, reqs, sym_base, src_base, dst_base):
+    """PutRequest array: sources at PE src_pe's heap + src_base + src_off (local
+    addresses), targets at symmetric sym_base + dst_base + dst_off on dst_pe."""
+    n = len(reqs["nbytes"])
+    arr = (PutRequest * n)()
+    for i in range(n):
+        s = int(reqs["src_pe"][i])
+        arr[i].target = sym_base + dst_base + int(reqs["dst_off"][i])
+        arr[i].source = lib.nvshmem_ptr(sym_base + src_base + int(reqs["src_off"][i]), s)
+        arr[i].nbytes = int(reqs["nbytes"][i])
+        arr[i].pe = int(reqs["dst_pe"][i])
+        arr[i].elem_bytes = 4
+    return arr
+
+
 class CopyConfig(C.Structure):
     """nvshmemx_copy_config_t (include/nvshmemx.h)."""
     _fields_ = [("workgroups", _I), ("block", _I), ("vec_bytes", _I), ("unroll", _I), ("nontemporal", _I),

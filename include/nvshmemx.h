@@ -86,6 +86,93 @@ void nvshmemx_putmem_config_on_stream(void *target, const void *source, size_t n
 /* Stream-ordered device barrier on an explicit stream. */
 void nvshmemx_barrier_all_on_stream(hipStream_t stream);
 
+/* ---- batched puts: many puts of uneven size in one fused launch ---- */
+
+typedef struct nvshmemx_put_request {
+    void *target;       /* symmetric address, resolved on `pe` */
+    const void *source; /* local device address */
+    size_t nbytes;
+    int pe;
+    int elem_bytes;     /* element width of the payload (4 or 8) */
+} nvshmemx_put_request_t;
+
+/* How one request runs inside the fused launch. */
+typedef struct nvshmemx_batch_choice {
+    int pieces;    /* 1..256 parts; a part is one workgroup's work (mode 1) or four wavefronts' (mode 0) */
+    int vec_bytes; /* widest access: 4, 8 or 16 */
+    int mode;      /* 0: wavefront pieces, 1: workgroup pieces */
+} nvshmemx_batch_choice_t;
+
+#define NVSHMEMX_BATCH_SHARED 0 /* every workgroup of the launch serves every request */
+#define NVSHMEMX_BATCH_OWNER 1  /* local PEs: PE p's workgroups serve the requests whose source is in PE p's heap */
+
+typedef struct nvshmemx_batch_launch {
+    int workgroups;  /* fused grid size (0: library default) */
+    int block;       /* threads per workgroup, multiple of 64 (0: 256) */
+    int nontemporal; /* bit 0 loads, bit 1 stores */
+    int discipline;  /* NVSHMEMX_BATCH_SHARED or NVSHMEMX_BATCH_OWNER */
+} nvshmemx_batch_launch_t;
+
+typedef struct nvshmemx_batch_plan *nvshmemx_batch_plan_t;
+
+/* Build a plan (device-resident piece queues) for `nreq` requests.  `choices`
+ * may be NULL (library rule, nvshmemx_batch_default_choice), `launch` may be
+ * NULL (defaults).  Returns 0. */
+int nvshmemx_batch_plan_create(const nvshmemx_put_request_t *reqs, int nreq, const nvshmemx_batch_choice_t *choices,
+                               const nvshmemx_batch_launch_t *launch, nvshmemx_batch_plan_t *plan);
+/* One fused launch executing every request of the plan. */
+void nvshmemx_batch_plan_launch(nvshmemx_batch_plan_t plan, hipStream_t stream);
+/* Capture `steps` back-to-back launches of the plan into a hipGraph; replay
+ * it with nvshmemx_batch_plan_graph_launch.  The graph holds kernel nodes
+ * only.  Returns 0. */
+int nvshmemx_batch_plan_capture(nvshmemx_batch_plan_t plan, int steps);
+void nvshmemx_batch_plan_graph_launch(nvshmemx_batch_plan_t plan, hipStream_t stream);
+size_t nvshmemx_batch_plan_pieces(nvshmemx_batch_plan_t plan);
+size_t nvshmemx_batch_plan_bytes(nvshmemx_batch_plan_t plan);
+void nvshmemx_batch_plan_destroy(nvshmemx_batch_plan_t plan);
+/* The library's untuned rule for one request. */
+void nvshmemx_batch_default_choice(size_t nbytes, int elem_bytes, nvshmemx_batch_choice_t *out);
+/* The per-call path: one grid put launch per request. */
+void nvshmemx_put_requests_on_stream(const nvshmemx_put_request_t *reqs, int nreq, hipStream_t stream);
+
+/* ---- exhaustive autotuner of batch choices ---- */
+
+/* The 54 candidates: pieces {1,2,..,256} x access {4,8,16} B x {wave, workgroup}. */
+int nvshmemx_tune_candidate_count(void);
+void nvshmemx_tune_candidate(int index, nvshmemx_batch_choice_t *out);
+/* 0: source and target agree modulo 16; 1: modulo 8; 2: otherwise */
+int nvshmemx_align_class(const void *source, const void *target);
+/* the four normalised model inputs of one request */
+void nvshmemx_tune_features(double log2_bytes, int elem_bytes, int concurrency, int align_class, float *features4);
+/* Measure every candidate at every point (fused launches over private
+ * scratch buffers, HIP-event timed); times_us[p * 54 + c].  Returns 0. */
+int nvshmemx_tune_measure(int npts, const double *log2_bytes, const int *elem_bytes, const int *concurrency,
+                          const int *align_class, int reps, int workgroups, int block, float *times_us);
+/* Per request: the measured-best candidate of the nearest measured point. */
+int nvshmemx_tune_table_choose(const float *features, const float *times_us, int npts,
+                               const nvshmemx_put_request_t *reqs, int nreq, int concurrency,
+                               nvshmemx_batch_choice_t *out);
+
+/* ---- learned cost model: fp32 MLP 4 -> 32 -> 32 -> C, trained on the GPU ---- */
+
+typedef struct nvshmemx_costmodel *nvshmemx_costmodel_t;
+int nvshmemx_costmodel_create(int ncand, unsigned long long seed, nvshmemx_costmodel_t *model);
+int nvshmemx_costmodel_num_params(nvshmemx_costmodel_t model);
+void nvshmemx_costmodel_get_params(nvshmemx_costmodel_t model, float *params);
+void nvshmemx_costmodel_set_params(nvshmemx_costmodel_t model, const float *params);
+/* gradient of the mean squared error of log-times over (X: N x 4, T: N x C) */
+void nvshmemx_costmodel_gradient(nvshmemx_costmodel_t model, const float *X, const float *T, int N, float *grad);
+/* `steps` full-batch Adam steps (graph-captured) */
+void nvshmemx_costmodel_train(nvshmemx_costmodel_t model, const float *X, const float *T, int N, int steps, float lr);
+/* predicted log(time in microseconds) per candidate: Y is N x C */
+void nvshmemx_costmodel_predict(nvshmemx_costmodel_t model, const float *X, int N, float *Y);
+/* per request: the candidate with the smallest predicted time */
+int nvshmemx_costmodel_choose(nvshmemx_costmodel_t model, const nvshmemx_put_request_t *reqs, int nreq,
+                              int concurrency, nvshmemx_batch_choice_t *out);
+int nvshmemx_costmodel_save(nvshmemx_costmodel_t model, const char *path);
+int nvshmemx_costmodel_load(nvshmemx_costmodel_t model, const char *path);
+void nvshmemx_costmodel_destroy(nvshmemx_costmodel_t model);
+
 /* Tracing (library built with NVSHMEM_TRACING): drain the ring now; report
  * whether records are being kept.  Both are no-ops / 0 otherwise. */
 void nvshmemx_trace_flush(void);

@@ -4,6 +4,7 @@
  */
 #include "oracle.h"
 
+#include <math.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -180,4 +181,54 @@ int oracle_trace_row(char *buf, size_t cap, long long tstamp_ns, unsigned long l
     return snprintf(buf, cap, "%lld, %llu, %d, %d, %s, %p, %p, %d, %d, %lld\n", tstamp_ns, gridid, blockid, tid,
                     oracle_trace_op_name(type), (void *)(uintptr_t)local_addr, (void *)(uintptr_t)remote_addr, size,
                     remote_pe, tspent_ns);
+}
+
+/* ---------------------------------------------------- batch workload -- */
+/* The 4096-put workload: sizes log-uniform on [2^min_log2, 2^max_log2]
+ * rounded down to a multiple of 4 bytes, int32/f32 50/50, (src, dst) PE pairs
+ * uniform with src != dst, 256-byte aligned source offsets inside the source
+ * region, destinations packed in request order per destination PE (so they
+ * are disjoint and the result is order-independent).  One PCG32 stream
+ * (seed, seq 54) drives every draw, in the order written below. */
+int oracle_batch_generate(uint64_t seed, int nreq, int npes, uint64_t src_region, uint64_t dst_region, int min_log2,
+                          int max_log2, int *src_pe, int *dst_pe, uint64_t *src_off, uint64_t *dst_off,
+                          uint64_t *nbytes, int *dtype) {
+    uint64_t inc = (54u << 1u) | 1u, st = 0;
+    st = st * PCG_MULT + inc;
+    st += seed;
+    st = st * PCG_MULT + inc;
+    uint64_t cursor[1024];
+    if (npes < 2 || npes > 1024) return -2;
+    for (int p = 0; p < npes; ++p) cursor[p] = 0;
+    for (int i = 0; i < nreq; ++i) {
+        uint32_t u[5];
+        for (int k = 0; k < 5; ++k) {
+            u[k] = pcg_out(st);
+            st = st * PCG_MULT + inc;
+        }
+        const double e = (double)min_log2 + (double)(max_log2 - min_log2) * ((double)u[0] / 4294967296.0);
+        uint64_t size = ((uint64_t)exp2(e)) & ~(uint64_t)3;
+        if (size < 4) size = 4;
+        const int s = (int)(u[2] % (uint32_t)npes);
+        const int d = (int)((s + 1 + (int)(u[3] % (uint32_t)(npes - 1))) % npes);
+        if (size > src_region) return -1;
+        const uint64_t slots = (src_region - size) / 256 + 1;
+        src_pe[i] = s;
+        dst_pe[i] = d;
+        dtype[i] = (int)(u[1] & 1u);
+        nbytes[i] = size;
+        src_off[i] = (u[4] % slots) * 256;
+        dst_off[i] = cursor[d];
+        cursor[d] = (cursor[d] + size + 255) / 256 * 256;
+        if (cursor[d] > dst_region) return -1;
+    }
+    return 0;
+}
+
+/* applies the batch to host heap images in request order: heaps[pe] is PE
+ * pe's image; sources at src_base + src_off, targets at dst_base + dst_off */
+void oracle_batch_apply(uint8_t **heaps, int nreq, const int *src_pe, const int *dst_pe, const uint64_t *src_off,
+                        const uint64_t *dst_off, const uint64_t *nbytes, uint64_t src_base, uint64_t dst_base) {
+    for (int i = 0; i < nreq; ++i)
+        memcpy(heaps[dst_pe[i]] + dst_base + dst_off[i], heaps[src_pe[i]] + src_base + src_off[i], nbytes[i]);
 }

@@ -85,6 +85,13 @@ int oracle_trace_row(char *buf, size_t cap, long long tstamp_ns, unsigned long l
                      int type, uint64_t local_addr, uint64_t remote_addr, int size, int remote_pe,
                      long long tspent_ns);
 
+/* ---- the 4096-put batch workload (see oracle.c) ---- */
+int oracle_batch_generate(uint64_t seed, int nreq, int npes, uint64_t src_region, uint64_t dst_region, int min_log2,
+                          int max_log2, int *src_pe, int *dst_pe, uint64_t *src_off, uint64_t *dst_off,
+                          uint64_t *nbytes, int *dtype);
+void oracle_batch_apply(uint8_t **heaps, int nreq, const int *src_pe, const int *dst_pe, const uint64_t *src_off,
+                        const uint64_t *dst_off, const uint64_t *nbytes, uint64_t src_base, uint64_t dst_base);
+
 #ifdef __cplusplus
 }
 #endif

@@ -98,6 +98,48 @@ class Oracle:
         self.lib.oracle_gen_barrier_init(buf, npes)
         return buf
 
+    def batch_generate(self, seed, nreq, npes, src_region, dst_region, min_log2=6, max_log2=22):
+        f = self.lib.oracle_batch_generate
+        f.restype = _I
+        f.argtypes = [_U64, _I, _I, _U64, _U64, _I, _I, _P, _P, _P, _P, _P, _P]
+        out = {k: np.zeros(nreq, dt) for k, dt in [("src_pe", np.int32), ("dst_pe", np.int32), ("src_off", np.uint64),
+                                                   ("dst_off", np.uint64), ("nbytes", np.uint64), ("dtype", np.int32)]}
+        rc = f(seed, nreq, npes, src_region, dst_region, min_log2, max_log2,
+               *[out[k].ctypes.data for k in ("src_pe", "dst_pe", "src_off", "dst_off", "nbytes", "dtype")])
+        if rc != 0:
+            raise ValueError(f"oracle_batch_generate failed ({rc})")
+        return out
+
+    def batch_apply(self, heaps, reqs, src_base, dst_base):
+        f = self.lib.oracle_batch_apply
+        f.restype = None
+        f.argtypes = [_P, _I, _P, _P, _P, _P, _P, _U64, _U64]
+        ptrs = (C.c_void_p * len(heaps))(*[h.ctypes.data for h in heaps])
+        f(ptrs, len(reqs["nbytes"]), *[reqs[k].ctypes.data for k in ("src_pe", "dst_pe", "src_off", "dst_off",
+                                                                      "nbytes")], src_base, dst_base)
+
+    def mlp_num_params(self, c):
+        f = self.lib.oracle_mlp_num_params
+        f.restype, f.argtypes = _I, [_I]
+        return f(c)
+
+    def mlp_forward(self, P, X, c):
+        f = self.lib.oracle_mlp_forward
+        f.restype, f.argtypes = None, [_P, _I, _P, _I, _P]
+        P, X = np.ascontiguousarray(P, np.float64), np.ascontiguousarray(X, np.float64)
+        Y = np.zeros((len(X), c), np.float64)
+        f(P.ctypes.data, c, X.ctypes.data, len(X), Y.ctypes.data)
+        return Y
+
+    def mlp_loss_grad(self, P, X, T, c):
+        f = self.lib.oracle_mlp_loss_grad
+        f.restype, f.argtypes = C.c_double, [_P, _I, _P, _P, _I, _P]
+        P = np.ascontiguousarray(P, np.float64)
+        X, T = np.ascontiguousarray(X, np.float64), np.ascontiguousarray(T, np.float64)
+        g = np.zeros_like(P)
+        loss = f(P.ctypes.data, c, X.ctypes.data, T.ctypes.data, len(X), g.ctypes.data)
+        return loss, g
+
     def trace_row(self, *fields):
         buf = C.create_string_buffer(512)
         self.lib.oracle_trace_row(buf, 512, *fields)

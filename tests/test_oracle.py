@@ -245,3 +245,47 @@ def test_trace_csv_known_answer(oracle):
     names = [oracle.lib.oracle_trace_op_name(i).decode() for i in range(8)]
     assert names == ["put_int", "put_float", "put_double", "get_int", "get_float", "get_double", "wait_int",
                      "wait_until_int"]
+
+
+# ---------------------------------------------------------------- cost model (MLP)
+
+def test_mlp_gradient_matches_finite_differences(oracle):
+    """the oracle's analytic gradient against central differences (double),
+    every parameter of a small model"""
+    rng = np.random.default_rng(0)
+    c = 5
+    npar = oracle.mlp_num_params(c)
+    P = rng.normal(0, 0.5, npar)
+    X = rng.normal(0, 1, (7, 4))
+    T = rng.normal(0, 1, (7, c))
+    loss, g = oracle.mlp_loss_grad(P, X, T, c)
+    h = 1e-6
+    fd = np.zeros(npar)
+    for i in range(npar):
+        e = np.zeros(npar)
+        e[i] = h
+        fd[i] = (oracle.mlp_loss_grad(P + e, X, T, c)[0] - oracle.mlp_loss_grad(P - e, X, T, c)[0]) / (2 * h)
+    np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-8)
+    Y = oracle.mlp_forward(P, X, c)
+    assert abs(loss - np.mean((Y - T) ** 2)) < 1e-12
+
+
+def test_mlp_forward_closed_forms(oracle):
+    """zero weights give the output biases; a single active path is linear"""
+    c = 3
+    npar = oracle.mlp_num_params(c)
+    P = np.zeros(npar)
+    b3 = npar - c
+    P[b3:] = [1.5, -2.0, 0.25]
+    X = np.random.default_rng(1).normal(size=(4, 4))
+    np.testing.assert_array_equal(oracle.mlp_forward(P, X, c), np.tile(P[b3:], (4, 1)))
+    # W1[0][0] = 1, W2[0][0] = 1, W3[j][0] = j + 1: y_j = (j + 1) * relu(x0) + b3_j
+    P[0] = 1.0
+    w2 = 4 * 32 + 32
+    P[w2] = 1.0
+    w3 = w2 + 32 * 32 + 32
+    for j in range(c):
+        P[w3 + j * 32] = j + 1
+    Y = oracle.mlp_forward(P, X, c)
+    want = np.maximum(X[:, :1], 0) * np.arange(1, c + 1) + P[b3:]
+    np.testing.assert_allclose(Y, want, rtol=0, atol=1e-15)
