@@ -143,6 +143,29 @@ SIGNATURES.update({
     "nvshmemx_costmodel_destroy": (None, [_P]),
 })
 
+SIGNATURES.update({
+    "nvshmemh_jacobi_field_bytes": (_SZ, [_I]),
+    "nvshmemh_jacobi_recv_bytes": (_SZ, [_I]),
+    "nvshmemh_jacobi_init": (None, [_P, _I, _I, _I, _U64, _P]),
+    "nvshmemh_jacobi_pack_put": (None, [_P, _P, _I, _I, _I, _I, _P]),
+    "nvshmemh_jacobi_unpack": (None, [_P, _P, _I, _I, _I, _I, _P]),
+    "nvshmemh_jacobi_stencil": (None, [_P, _P, _I, _I, _I, _I, _P]),
+    "nvshmemh_jacobi_run": (_P, [_P, _P, _P, _I, _I, _I, _I, _P]),
+})
+
+
+def jacobi_interior(hip, field_ptr, n):
+    """(n, n, n) interior of one PE's (n+2)^3 field"""
+    a = hip.dtoh(field_ptr, (n + 2) ** 3 * 8, "<f8").reshape(n + 2, n + 2, n + 2)
+    return a[1:-1, 1:-1, 1:-1].copy()
+
+
+def jacobi_block(global_field, pe, n):
+    """PE pe's block of the global 2n^3 field (2x2x2 PE grid, pe = (x<<2)|(y<<1)|z)"""
+    cx, cy, cz = pe >> 2, (pe >> 1) & 1, pe & 1
+    return global_field[cx * n:(cx + 1) * n, cy * n:(cy + 1) * n, cz * n:(cz + 1) * n]
+
+
 NCAND = 54
 
 

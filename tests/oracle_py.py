@@ -140,6 +140,17 @@ class Oracle:
         loss = f(P.ctypes.data, c, X.ctypes.data, T.ctypes.data, len(X), g.ctypes.data)
         return loss, g
 
+    def jacobi(self, G, steps, seed):
+        """global G^3 field after `steps` periodic Jacobi steps (float64)"""
+        init, run = self.lib.oracle_jacobi_init, self.lib.oracle_jacobi_run
+        init.restype, init.argtypes = None, [_P, _I, _U64]
+        run.restype, run.argtypes = _I, [_P, _P, _I, _I]
+        u = np.zeros(G ** 3, np.float64)
+        v = np.zeros(G ** 3, np.float64)
+        init(u.ctypes.data, G, seed)
+        which = run(u.ctypes.data, v.ctypes.data, G, steps)
+        return (v if which else u).reshape(G, G, G)
+
     def trace_row(self, *fields):
         buf = C.create_string_buffer(512)
         self.lib.oracle_trace_row(buf, 512, *fields)

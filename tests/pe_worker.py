@@ -92,6 +92,20 @@ def main():
         out["violations"] = int(hip.dtoh(viol, 8, np.uint64)[0])
         out["slots"] = hip.dtoh(slots, n * 4, np.int32).tolist()
         lib.nvshmem_barrier_all()
+    elif scenario == "jacobi":
+        n, steps, outdir = int(sys.argv[2]), int(sys.argv[3]), Path(sys.argv[4])
+        fb = lib.nvshmemh_jacobi_field_bytes(n)
+        u, v = lib.nvshmalloc(fb), lib.nvshmalloc(fb)
+        recv = lib.nvshmalloc(lib.nvshmemh_jacobi_recv_bytes(n))
+        hip.memset(u, 0, fb)
+        hip.memset(v, 0, fb)
+        lib.nvshmemh_jacobi_init(u, n, me, 1, 42, None)
+        hip.sync()
+        lib.nvshmem_barrier_all()
+        res = lib.nvshmemh_jacobi_run(u, v, recv, n, steps, 0, 0, None)
+        hip.sync()
+        lib.nvshmem_barrier_all()
+        np.save(outdir / f"jacobi_pe{me}.npy", na.jacobi_interior(hip, res, n))
     out["errors"] = lib.nvshmemx_get_last_error()
     lib.nvstop_pes()
     print(json.dumps(out), flush=True)

@@ -1,0 +1,77 @@
+"""3-D 7-point Jacobi with halo exchange through the symmetric heap, 8 PEs in
+a periodic 2x2x2 grid, bit-exact against the oracle's host replay of the whole
+undecomposed domain.  Local mode (all 8 PEs in one process, kernel boundaries
+as the barrier) at 16^3 per PE for 100 steps and at the full 256^3 per PE for
+3 steps; process mode (8 processes, nvshmem_barrier_all_offload per step) at
+16^3 for 40 steps."""
+import json
+import os
+import subprocess
+import sys
+import uuid
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import nvshmem_amd as na
+
+pytestmark = pytest.mark.gpu
+HERE = Path(__file__).resolve().parent
+SEED = 42
+
+
+def _run_local(lib, hip, n, steps, chunk):
+    fb = lib.nvshmemh_jacobi_field_bytes(n)
+    u, v = lib.nvshmalloc(fb), lib.nvshmalloc(fb)
+    recv = lib.nvshmalloc(lib.nvshmemh_jacobi_recv_bytes(n))
+    for pe in range(8):
+        for p in (u, v):
+            hip.memset(lib.nvshmem_ptr(p, pe), 0, fb)
+    lib.nvshmemh_jacobi_init(u, n, 0, 8, SEED, None)
+    out = lib.nvshmemh_jacobi_run(u, v, recv, n, steps, 0 if chunk is None else chunk, 0, None)
+    hip.sync()
+    assert lib.nvshmemx_get_last_error() == 0
+    return out
+
+
+@pytest.mark.parametrize("n,steps,chunk", [(16, 100, None), (16, 7, 5), (256, 3, 64)])
+def test_jacobi_local_bitexact(local_pes, hip, oracle, n, steps, chunk):
+    heap = "16M" if n == 16 else "320M"
+    lib = local_pes.start(8, heap=heap)
+    out = _run_local(lib, hip, n, steps, chunk)
+    want = oracle.jacobi(2 * n, steps, SEED)
+    for pe in range(8):
+        got = na.jacobi_interior(hip, lib.nvshmem_ptr(out, pe), n)
+        blk = na.jacobi_block(want, pe, n)
+        if not np.array_equal(got.view(np.uint64), blk.view(np.uint64)):
+            bad = np.argwhere(got != blk)
+            raise AssertionError(f"PE {pe}: {len(bad)} cells differ, first {bad[0]}: {got[tuple(bad[0])]!r} vs "
+                                 f"{blk[tuple(bad[0])]!r}")
+
+
+def test_jacobi_processes_bitexact(tmp_path, oracle):
+    n, steps = 16, 40
+    name = f"nvshmem_jacobi_{os.getpid()}_{uuid.uuid4().hex[:8]}"
+    procs = [subprocess.Popen([sys.executable, str(HERE / "pe_worker.py"), "jacobi", str(n), str(steps),
+                               str(tmp_path)],
+                              env=na.shm_job_env(r, 8, name, {"NVSHMEM_HEAP_SIZE": "16M",
+                                                              "NVSHMEM_WAIT_TIMEOUT_MS": "5000",
+                                                              "NVSHMEM_BOOTSTRAP_TIMEOUT_S": "60"}),
+                              stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for r in range(8)]
+    try:
+        for p in procs:
+            o, e = p.communicate(timeout=120)
+            assert p.returncode == 0, e[-2000:]
+            assert json.loads([l for l in o.splitlines() if l.startswith("{")][-1])["errors"] == 0
+    finally:
+        for p in procs:
+            if p.poll() is None:
+                p.kill()
+        shm = Path("/dev/shm") / name
+        if shm.exists():
+            shm.unlink()
+    want = oracle.jacobi(2 * n, steps, SEED)
+    for pe in range(8):
+        got = np.load(tmp_path / f"jacobi_pe{pe}.npy")
+        np.testing.assert_array_equal(got.view(np.uint64), na.jacobi_block(want, pe, n).view(np.uint64))
