@@ -198,6 +198,89 @@ def learned_worker(n_batches, train_steps):
     print(json.dumps(out), flush=True)
 
 
+JACOBI_N, JACOBI_STEPS = 256, 100
+
+
+def jacobi_worker(layout, name=None):
+    """3-D 7-point Jacobi, 8 PEs (2x2x2, periodic), 256^3 float64 cells per
+    PE, 100 steps; each step: 6 face puts into the neighbours' receive slots,
+    the barrier (kernel boundary in local mode, nvshmem_barrier_all_offload
+    between processes), unpack, stencil.  Prints one JSON line (PE 0)."""
+    import numpy as np
+    import nvshmem_amd as na
+    lib, hip = na.load(), na.Hip()
+    if layout == "local":
+        os.environ["NVSHMEM_HEAP_SIZE"] = "320M"
+        lib.nvshmemx_start_local_pes(8, 1)
+        pes_here = list(range(8))
+    else:
+        lib.nvstart_pes()
+        pes_here = [lib.nvshmem_my_pe()]
+    n = JACOBI_N
+    fb = lib.nvshmemh_jacobi_field_bytes(n)
+    u, v = lib.nvshmalloc(fb), lib.nvshmalloc(fb)
+    recv = lib.nvshmalloc(lib.nvshmemh_jacobi_recv_bytes(n))
+    for pe in pes_here:
+        for p in (u, v):
+            hip.memset(lib.nvshmem_ptr(p, pe), 0, fb)
+    pe0 = pes_here[0]
+    lib.nvshmemh_jacobi_init(u, n, pe0, len(pes_here), 42, None)
+    hip.sync()
+
+    def total(field):
+        return sum(float(na.jacobi_interior(hip, lib.nvshmem_ptr(field, pe), n).sum(dtype=np.float64))
+                   for pe in pes_here)
+    s0 = total(u)
+    lib.nvshmem_barrier_all()
+    ms = timed(hip, lambda: lib.nvshmemh_jacobi_run(u, v, recv, n, JACOBI_STEPS, 64, 0, None), 1, 0) / JACOBI_STEPS
+    res = u if JACOBI_STEPS % 2 == 0 else v
+    lib.nvshmem_barrier_all()
+    s1 = total(res)
+    err = lib.nvshmemx_get_last_error()
+    out = {"layout": layout, "pes": 8, "cells_per_pe": n ** 3, "steps": JACOBI_STEPS, "ms_per_step": ms,
+           "sum_before": s0, "sum_after": s1, "errors": err}
+    lib.nvstop_pes()
+    if pe0 == 0:
+        print(json.dumps(out), flush=True)
+
+
+def run_jacobi_leg():
+    import nvshmem_amd as na
+    env = {k: v for k, v in os.environ.items() if not k.startswith("NVSHMEM_")}
+    out = {}
+    try:
+        p = subprocess.run([sys.executable, __file__, "--jacobi-worker", "local"], capture_output=True, text=True,
+                           timeout=600, env=env)
+        lines = [l for l in p.stdout.splitlines() if l.startswith("{")]
+        out["local"] = json.loads(lines[-1]) if lines else {"error": p.stderr[-800:]}
+        name = f"nvshmem_jacobi_{os.getpid()}_{uuid.uuid4().hex[:8]}"
+        procs = [subprocess.Popen([sys.executable, __file__, "--jacobi-worker", "process"],
+                                  env=na.shm_job_env(r, 8, name, {"NVSHMEM_HEAP_SIZE": "320M"}), stdout=subprocess.PIPE,
+                                  stderr=subprocess.PIPE, text=True) for r in range(8)]
+        outs = [pr.communicate(timeout=600) for pr in procs]
+        shm = Path("/dev/shm") / name
+        if shm.exists():
+            shm.unlink()
+        lines = [l for o, _ in outs for l in o.splitlines() if l.startswith("{")]
+        out["process"] = json.loads(lines[-1]) if lines else {"error": outs[0][1][-800:]}
+    except Exception as exc:
+        out["error"] = str(exc)[-500:]
+    for k in ("local", "process"):
+        r = out.get(k)
+        if isinstance(r, dict) and "ms_per_step" in r:
+            # stencil: one read of u and one write of v per cell; halo: 6 faces
+            # of n^2 doubles read, written remotely, read back (unpack) and written
+            cells = 8 * JACOBI_N ** 3
+            halo = 8 * 6 * JACOBI_N ** 2 * 8 * 4
+            r["hbm_gbps"] = round((cells * 16 + halo) / (r["ms_per_step"] * 1e-3) / 1e9, 1)
+            r["ms_per_step"] = round(r["ms_per_step"], 4)
+            if k == "local":  # all 8 PEs' cells: the stencil conserves the total up to rounding
+                r["rel_sum_drift"] = abs(r["sum_after"] - r["sum_before"]) / abs(r["sum_before"])
+            r.pop("sum_before", None)
+            r.pop("sum_after", None)
+    return out
+
+
 def run_learned_leg(n_batches, train_steps):
     env = {k: v for k, v in os.environ.items() if not k.startswith("NVSHMEM_")}
     try:
@@ -279,6 +362,8 @@ def main():
     ap.add_argument("--learned-batches", type=int, default=1000)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-jacobi", action="store_true")
+    ap.add_argument("--jacobi-worker", choices=["local", "process"], default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pe_worker:
         worker(args.pe_worker)
@@ -288,6 +373,9 @@ def main():
         return
     if args.learned_worker:
         learned_worker(args.learned_batches, args.train_steps)
+        return
+    if args.jacobi_worker:
+        jacobi_worker(args.jacobi_worker)
         return
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: all PEs share one GPU by design; --gpus {args.gpus} is not supported",
@@ -373,6 +461,7 @@ def main():
 
     batched = None if args.no_batch else run_batch_leg()
     learned = None if args.no_learned else run_learned_leg(args.learned_batches, args.train_steps)
+    jacobi = None if args.no_jacobi else run_jacobi_leg()
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     r = result
     line = {
@@ -404,6 +493,7 @@ def main():
         "device_errors": r["errors"],
         "batched": batched,
         "learned": learned,
+        "jacobi": jacobi,
     }
     print(json.dumps(line), flush=True)
     if not (r["put_ok"] and r["get_ok"]) or r["errors"]:

@@ -33,6 +33,7 @@ const int kPieces[9] = {1, 2, 4, 8, 16, 32, 64, 128, 256};
 const int kVec[3] = {4, 8, 16};
 constexpr int kNumCand = 54;
 constexpr size_t kPool = (size_t)256 << 20;
+constexpr size_t kMaxPieces = 65536;
 
 void candidate_of(int i, nvshmemx_batch_choice_t *c) {
     c->pieces = kPieces[i % 9];
@@ -85,14 +86,19 @@ extern "C" int nvshmemx_tune_measure(int npts, const double *log2_bytes, const i
         size_t size = (size_t)floor(exp2(log2_bytes[p]));
         size = std::max<size_t>(4, size & ~(size_t)3);
         const size_t stride = (size + 32 + 255) / 256 * 256;
-        const size_t k = (size_t)std::max<long long>(1, std::min<long long>(concurrency[p], (long long)(kPool / stride)));
+        /* the most pieces any candidate makes of one request bounds how many
+         * requests a point replays, keeping every plan <= kMaxPieces pieces
+         * (identical for all 54 candidates of the point) */
+        const size_t max_parts = std::max<size_t>(1, std::min<size_t>(1024, size / 256));
+        long long kcap = std::min<long long>((long long)(kPool / stride), (long long)(kMaxPieces / max_parts));
+        const size_t k = (size_t)std::max<long long>(1, std::min<long long>(concurrency[p], std::max(1LL, kcap)));
         const size_t doff = align_class[p] == 0 ? 0 : (align_class[p] == 1 ? 8 : 4);
         reqs.resize(k);
         for (size_t i = 0; i < k; ++i) reqs[i] = resolved_req{src + i * stride, dst + i * stride + doff, size, 0};
         for (int c = 0; c < kNumCand; ++c) {
             nvshmemx_batch_choice_t ch;
             candidate_of(c, &ch);
-            plan_fill(plan, reqs, nullptr, &ch, 1, workgroups, block, NT_BOTH);
+            plan_fill(plan, reqs, nullptr, &ch, 1, workgroups, block, NT_BOTH, false);
             plan_launch(plan, nullptr);
             NVSHMEMI_HIP(hipEventRecord(e0, nullptr));
             for (int r = 0; r < reps; ++r) plan_launch(plan, nullptr);

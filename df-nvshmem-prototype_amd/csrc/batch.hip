@@ -136,7 +136,8 @@ namespace nvshmemi {
  * they are large enough, so the autotuner can rebuild per candidate without
  * allocating. */
 void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, const nvshmemx_batch_choice_t *choices,
-               const nvshmemx_batch_choice_t *one_choice, int nsets, int wgs_per_set, int block, int nontemporal) {
+               const nvshmemx_batch_choice_t *one_choice, int nsets, int wgs_per_set, int block, int nontemporal,
+               bool sort_pieces) {
     std::vector<std::vector<piece_t>> wg(nsets), wave(nsets);
     size_t total = 0;
     for (size_t i = 0; i < reqs.size(); ++i) {
@@ -153,8 +154,10 @@ void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, co
     std::vector<queue_t> queues(nsets);
     auto by_size = [](const piece_t &a, const piece_t &b) { return a.nbytes > b.nbytes; };
     for (int s = 0; s < nsets; ++s) {
-        std::stable_sort(wg[s].begin(), wg[s].end(), by_size);
-        std::stable_sort(wave[s].begin(), wave[s].end(), by_size);
+        if (sort_pieces) {
+            std::stable_sort(wg[s].begin(), wg[s].end(), by_size);
+            std::stable_sort(wave[s].begin(), wave[s].end(), by_size);
+        }
         queues[s].wg_begin = (unsigned)all.size();
         all.insert(all.end(), wg[s].begin(), wg[s].end());
         queues[s].wg_end = queues[s].wave_begin = (unsigned)all.size();
@@ -236,7 +239,7 @@ extern "C" int nvshmemx_batch_plan_create(const nvshmemx_put_request_t *reqs, in
     const int wgs_per_set = owner ? (L.workgroups > 0 ? std::max(1, L.workgroups / nsets) : g.wgs_per_pe)
                                   : (L.workgroups > 0 ? L.workgroups : 1024);
     nvshmemx_batch_plan *p = new nvshmemx_batch_plan();
-    plan_fill(p, rr, choices, nullptr, nsets, wgs_per_set, L.block, L.nontemporal);
+    plan_fill(p, rr, choices, nullptr, nsets, wgs_per_set, L.block, L.nontemporal, true);
     *out = p;
     return 0;
 }

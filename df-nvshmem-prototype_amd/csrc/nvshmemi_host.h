@@ -108,7 +108,8 @@ struct resolved_req {
     int set;
 };
 void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, const nvshmemx_batch_choice_t *choices,
-               const nvshmemx_batch_choice_t *one_choice, int nsets, int wgs_per_set, int block, int nontemporal);
+               const nvshmemx_batch_choice_t *one_choice, int nsets, int wgs_per_set, int block, int nontemporal,
+               bool sort_pieces);
 void plan_launch(nvshmemx_batch_plan *p, hipStream_t s);
 nvshmemx_batch_plan *plan_new();
 

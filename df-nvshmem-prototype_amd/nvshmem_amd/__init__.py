@@ -279,15 +279,19 @@ def gen_requests(lib, seed, nreq, npes, src_region, dst_region, min_log2=6, max_
 def make_requests(lib, reqs, sym_base, src_base, dst_base):
     """PutRequest array: sources at PE src_pe's heap + src_base + src_off (local
     addresses), targets at symmetric sym_base + dst_base + dst_off on dst_pe."""
+    import numpy as np
     n = len(reqs["nbytes"])
-    arr = (PutRequest * n)()
-    for i in range(n):
-        s = int(reqs["src_pe"][i])
-        arr[i].target = sym_base + dst_base + int(reqs["dst_off"][i])
-        arr[i].source = lib.nvshmem_ptr(sym_base + src_base + int(reqs["src_off"][i]), s)
-        arr[i].nbytes = int(reqs["nbytes"][i])
-        arr[i].pe = int(reqs["dst_pe"][i])
-        arr[i].elem_bytes = 4
+    npes = lib.nvshmem_n_pes()
+    src_bases = np.array([lib.nvshmem_ptr(sym_base + src_base, s) for s in range(npes)], np.uint64)
+    rec = np.zeros(n, np.dtype([("target", "<u8"), ("source", "<u8"), ("nbytes", "<u8"), ("pe", "<i4"),
+                                ("elem_bytes", "<i4")]))
+    rec["target"] = np.uint64(sym_base + dst_base) + reqs["dst_off"].astype(np.uint64)
+    rec["source"] = src_bases[reqs["src_pe"]] + reqs["src_off"].astype(np.uint64)
+    rec["nbytes"] = reqs["nbytes"]
+    rec["pe"] = reqs["dst_pe"]
+    rec["elem_bytes"] = 4
+    arr = (PutRequest * n).from_buffer(rec)
+    arr._keepalive = rec
     return arr
 
 
