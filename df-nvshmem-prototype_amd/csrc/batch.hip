@@ -7,21 +7,28 @@
  *  - each request is split into pieces by its choice (pieces x vector width x
  *    wavefront/workgroup mode); piece boundaries fall on 256-byte offsets of the
  *    request, so every piece keeps the request's relative alignment;
- *  - pieces go into device queues (one workgroup queue and one wavefront queue
- *    per queue set), each sorted by size, largest first, so the dynamic
- *    schedule below approximates longest-processing-time-first;
- *  - one persistent kernel drains the queues: a workgroup takes whole
- *    workgroup pieces (one atomic ticket per piece, broadcast through LDS),
- *    then each wavefront takes wavefront pieces independently;
- *  - the ticket counters reset themselves: the last workgroup of a queue set
- *    to finish zeroes them, so a plan replays from a captured hipGraph with no
- *    memset node.
+ *  - the pieces are assigned to workers on the host, longest first, each to the
+ *    currently least-loaded worker (LPT list scheduling): workgroup pieces to
+ *    workgroups (all the workgroup's wavefronts share a piece), wavefront
+ *    pieces to single wavefronts.  A piece costs its bytes plus a fixed
+ *    per-piece charge standing for the first access's latency.  Per-worker
+ *    lists are laid out contiguously with offset tables;
+ *  - one kernel runs every list: workgroup b copies its workgroup pieces with
+ *    all its threads, then each of its wavefronts copies its own wavefront
+ *    pieces.  No atomics and no per-launch state, so a plan replays from a
+ *    captured hipGraph as is.
  *
- * Queue sets: SHARED = one set served by every workgroup; OWNER = one set per
- * PE in local mode, served only by that PE's workgroups (workgroup-partitioned
- * PEs, each executing the puts it initiated).
+ * A plan is static: the assignment is computed once and reused every step,
+ * which is what the batched-exchange pattern needs.  (A first version handed
+ * pieces out with a shared atomic ticket; on MI355X the ticket counter became
+ * the bottleneck as the grid grew, which the host-side assignment removes.)
+ *
+ * Worker sets: SHARED = one set of workers serving every request; OWNER = one
+ * set per PE in local mode, serving only the requests whose source is in that
+ * PE's heap (workgroup-partitioned PEs, each executing the puts it initiated).
  */
 #include <algorithm>
+#include <queue>
 #include <vector>
 
 #include "nvshmem.h"
@@ -39,12 +46,9 @@ struct piece_t {
     unsigned int pad;
 };
 
-struct queue_t {
-    unsigned int wg_begin, wg_end, wave_begin, wave_end;
-};
-
-
-constexpr int kCtrStride = 16; /* counters of one queue set: 64 bytes apart */
+/* per-piece charge in the LPT cost, in bytes: about one access latency of a
+ * wavefront streaming 4 x 1 KiB */
+constexpr size_t kPieceCharge = 4096;
 
 template <int NT>
 __device__ __forceinline__ void copy_piece(const piece_t &p, unsigned lane, unsigned n) {
@@ -56,38 +60,15 @@ __device__ __forceinline__ void copy_piece(const piece_t &p, unsigned lane, unsi
 }
 
 template <int NT>
-__global__ void __launch_bounds__(1024) k_batch(const piece_t *pieces, const queue_t *queues, unsigned int *ctr,
-                                                int wgs_per_set) {
-    const int set = (int)blockIdx.x / wgs_per_set;
-    const queue_t q = queues[set];
-    unsigned int *c = ctr + set * kCtrStride;
-    __shared__ unsigned int s_ticket;
-    for (;;) {
-        if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(&c[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        const unsigned int j = q.wg_begin + s_ticket;
-        __syncthreads();
-        if (j >= q.wg_end) break;
-        copy_piece<NT>(pieces[j], threadIdx.x, blockDim.x);
-    }
-    const unsigned int lane = threadIdx.x & 63u;
-    for (;;) {
-        unsigned int t = 0;
-        if (lane == 0) t = __hip_atomic_fetch_add(&c[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        t = __builtin_amdgcn_readfirstlane(t);
-        const unsigned int j = q.wave_begin + t;
-        if (j >= q.wave_end) break;
-        copy_piece<NT>(pieces[j], lane, 64);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned int done = __hip_atomic_fetch_add(&c[2], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (done == (unsigned int)wgs_per_set - 1) {
-            __hip_atomic_store(&c[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&c[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&c[2], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+__global__ void __launch_bounds__(1024) k_batch(const piece_t *__restrict__ pieces,
+                                                const unsigned int *__restrict__ wg_off,
+                                                const unsigned int *__restrict__ wave_off) {
+    const unsigned int b = blockIdx.x;
+    const unsigned int wg_end = wg_off[b + 1];
+    for (unsigned int j = wg_off[b]; j < wg_end; ++j) copy_piece<NT>(pieces[j], threadIdx.x, blockDim.x);
+    const unsigned int w = b * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const unsigned int wave_end = wave_off[w + 1];
+    for (unsigned int j = wave_off[w]; j < wave_end; ++j) copy_piece<NT>(pieces[j], threadIdx.x & 63u, 64);
 }
 
 /* one request's pieces under a choice; boundaries on 256-byte offsets */
@@ -108,6 +89,28 @@ static void split_request(const char *src, char *dst, size_t nbytes, const nvshm
     }
 }
 
+/* LPT: pieces (any order) onto `workers` bins; returns per-worker lists.
+ * `base_load[w]` is load a worker already carries; `share` divides a piece's
+ * bytes among the lanes of a workgroup worker's wavefronts. */
+static std::vector<std::vector<piece_t>> lpt(std::vector<piece_t> pieces, int workers, size_t share,
+                                            std::vector<double> &load) {
+    std::vector<std::vector<piece_t>> lists((size_t)workers);
+    std::stable_sort(pieces.begin(), pieces.end(),
+                     [](const piece_t &a, const piece_t &b) { return a.nbytes > b.nbytes; });
+    typedef std::pair<double, int> item;
+    std::priority_queue<item, std::vector<item>, std::greater<item>> heap;
+    for (int w = 0; w < workers; ++w) heap.push(item(load[(size_t)w], w));
+    for (const piece_t &p : pieces) {
+        item top = heap.top();
+        heap.pop();
+        lists[(size_t)top.second].push_back(p);
+        top.first += (double)(p.nbytes / share + kPieceCharge);
+        load[(size_t)top.second] = top.first;
+        heap.push(top);
+    }
+    return lists;
+}
+
 } // namespace nvshmemi
 
 using namespace nvshmemi;
@@ -120,24 +123,36 @@ struct nvshmemx_batch_plan {
     size_t npieces = 0;
     size_t total_bytes = 0;
     size_t piece_capacity = 0;
-    int set_capacity = 0;
+    size_t wg_capacity = 0;
+    size_t wave_capacity = 0;
     piece_t *d_pieces = nullptr;
-    queue_t *d_queues = nullptr;
-    unsigned int *d_ctr = nullptr;
+    unsigned int *d_wg_off = nullptr;
+    unsigned int *d_wave_off = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t capture_stream = nullptr;
     std::vector<piece_t> host_pieces; /* staging, reused across rebuilds */
+    std::vector<unsigned int> host_wg_off, host_wave_off;
 };
 
 namespace nvshmemi {
 
+static void ensure_capacity(void **d, size_t &cap, size_t need, size_t elem) {
+    if (need <= cap) return;
+    if (*d) NVSHMEMI_HIP(hipFree(*d));
+    cap = std::max<size_t>(need, 1024);
+    NVSHMEMI_HIP(hipMalloc(d, cap * elem));
+}
+
 /* (Re)build plan `p` from resolved requests; device buffers are reused when
  * they are large enough, so the autotuner can rebuild per candidate without
- * allocating. */
+ * allocating.  `balance` = false skips the LPT assignment and deals pieces
+ * round-robin (used when every piece is the same size). */
 void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, const nvshmemx_batch_choice_t *choices,
                const nvshmemx_batch_choice_t *one_choice, int nsets, int wgs_per_set, int block, int nontemporal,
-               bool sort_pieces) {
+               bool balance) {
+    if (block < 64 || block > 1024 || block % 64) NVSHMEMI_FATAL("batch block size %d invalid", block);
+    const int waves_per_wg = block / 64;
     std::vector<std::vector<piece_t>> wg(nsets), wave(nsets);
     size_t total = 0;
     for (size_t i = 0; i < reqs.size(); ++i) {
@@ -149,44 +164,77 @@ void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, co
         split_request(r.src, r.dst, r.nbytes, ch, ch.mode ? wg[r.set] : wave[r.set]);
         total += r.nbytes;
     }
+    const size_t G = (size_t)nsets * wgs_per_set, W = G * waves_per_wg;
     std::vector<piece_t> &all = p->host_pieces;
     all.clear();
-    std::vector<queue_t> queues(nsets);
-    auto by_size = [](const piece_t &a, const piece_t &b) { return a.nbytes > b.nbytes; };
+    std::vector<std::vector<piece_t>> wg_lists(G), wave_lists(W);
     for (int s = 0; s < nsets; ++s) {
-        if (sort_pieces) {
-            std::stable_sort(wg[s].begin(), wg[s].end(), by_size);
-            std::stable_sort(wave[s].begin(), wave[s].end(), by_size);
+        const size_t g0 = (size_t)s * wgs_per_set, w0 = g0 * waves_per_wg;
+        std::vector<std::vector<piece_t>> a, b;
+        if (balance) {
+            std::vector<double> wg_load((size_t)wgs_per_set, 0.0);
+            a = lpt(wg[s], wgs_per_set, (size_t)waves_per_wg, wg_load);
+            /* a wavefront starts its own pieces after its workgroup's */
+            std::vector<double> wave_load((size_t)wgs_per_set * waves_per_wg);
+            for (size_t w = 0; w < wave_load.size(); ++w) wave_load[w] = wg_load[w / waves_per_wg];
+            b = lpt(wave[s], wgs_per_set * waves_per_wg, 1, wave_load);
+        } else {
+            a.resize((size_t)wgs_per_set);
+            b.resize((size_t)wgs_per_set * waves_per_wg);
+            for (size_t i = 0; i < wg[s].size(); ++i) a[i % a.size()].push_back(wg[s][i]);
+            for (size_t i = 0; i < wave[s].size(); ++i) b[i % b.size()].push_back(wave[s][i]);
         }
-        queues[s].wg_begin = (unsigned)all.size();
-        all.insert(all.end(), wg[s].begin(), wg[s].end());
-        queues[s].wg_end = queues[s].wave_begin = (unsigned)all.size();
-        all.insert(all.end(), wave[s].begin(), wave[s].end());
-        queues[s].wave_end = (unsigned)all.size();
+        for (size_t i = 0; i < a.size(); ++i) wg_lists[g0 + i] = std::move(a[i]);
+        for (size_t i = 0; i < b.size(); ++i) wave_lists[w0 + i] = std::move(b[i]);
     }
+    std::vector<unsigned int> &wo = p->host_wg_off, &vo = p->host_wave_off;
+    wo.assign(G + 1, 0);
+    vo.assign(W + 1, 0);
+    for (size_t i = 0; i < G; ++i) {
+        wo[i] = (unsigned)all.size();
+        all.insert(all.end(), wg_lists[i].begin(), wg_lists[i].end());
+    }
+    wo[G] = (unsigned)all.size();
+    for (size_t i = 0; i < W; ++i) {
+        vo[i] = (unsigned)all.size();
+        all.insert(all.end(), wave_lists[i].begin(), wave_lists[i].end());
+    }
+    vo[W] = (unsigned)all.size();
     if (all.size() >= 0xFFFFFFF0u) NVSHMEMI_FATAL("batch of %zu pieces is too large", all.size());
-    if (all.size() > p->piece_capacity) {
-        if (p->d_pieces) NVSHMEMI_HIP(hipFree(p->d_pieces));
-        p->piece_capacity = std::max<size_t>(all.size(), 1024);
-        NVSHMEMI_HIP(hipMalloc((void **)&p->d_pieces, p->piece_capacity * sizeof(piece_t)));
-    }
-    if (nsets > p->set_capacity) {
-        if (p->d_queues) NVSHMEMI_HIP(hipFree(p->d_queues));
-        if (p->d_ctr) NVSHMEMI_HIP(hipFree(p->d_ctr));
-        p->set_capacity = nsets;
-        NVSHMEMI_HIP(hipMalloc((void **)&p->d_queues, nsets * sizeof(queue_t)));
-        NVSHMEMI_HIP(hipMalloc((void **)&p->d_ctr, (size_t)nsets * kCtrStride * sizeof(unsigned int)));
-        NVSHMEMI_HIP(hipMemset(p->d_ctr, 0, (size_t)nsets * kCtrStride * sizeof(unsigned int)));
-    }
+    ensure_capacity((void **)&p->d_pieces, p->piece_capacity, all.size(), sizeof(piece_t));
+    ensure_capacity((void **)&p->d_wg_off, p->wg_capacity, G + 1, sizeof(unsigned int));
+    ensure_capacity((void **)&p->d_wave_off, p->wave_capacity, W + 1, sizeof(unsigned int));
     if (!all.empty())
         NVSHMEMI_HIP(hipMemcpy(p->d_pieces, all.data(), all.size() * sizeof(piece_t), hipMemcpyHostToDevice));
-    NVSHMEMI_HIP(hipMemcpy(p->d_queues, queues.data(), nsets * sizeof(queue_t), hipMemcpyHostToDevice));
+    NVSHMEMI_HIP(hipMemcpy(p->d_wg_off, wo.data(), (G + 1) * sizeof(unsigned int), hipMemcpyHostToDevice));
+    NVSHMEMI_HIP(hipMemcpy(p->d_wave_off, vo.data(), (W + 1) * sizeof(unsigned int), hipMemcpyHostToDevice));
     p->nsets = nsets;
     p->wgs_per_set = wgs_per_set;
     p->block = block;
     p->nontemporal = nontemporal & 3;
     p->npieces = all.size();
     p->total_bytes = total;
+}
+
+nvshmemx_batch_plan *plan_new() { return new nvshmemx_batch_plan(); }
+
+void plan_launch(nvshmemx_batch_plan *p, hipStream_t s) {
+    const unsigned grid = (unsigned)(p->nsets * p->wgs_per_set);
+    switch (p->nontemporal) {
+    case NT_BOTH:
+        hipLaunchKernelGGL(k_batch<NT_BOTH>, dim3(grid), dim3(p->block), 0, s, p->d_pieces, p->d_wg_off, p->d_wave_off);
+        break;
+    case NT_LOAD:
+        hipLaunchKernelGGL(k_batch<NT_LOAD>, dim3(grid), dim3(p->block), 0, s, p->d_pieces, p->d_wg_off, p->d_wave_off);
+        break;
+    case NT_STORE:
+        hipLaunchKernelGGL(k_batch<NT_STORE>, dim3(grid), dim3(p->block), 0, s, p->d_pieces, p->d_wg_off,
+                           p->d_wave_off);
+        break;
+    default:
+        hipLaunchKernelGGL(k_batch<NT_NONE>, dim3(grid), dim3(p->block), 0, s, p->d_pieces, p->d_wg_off, p->d_wave_off);
+    }
+    NVSHMEMI_HIP(hipGetLastError());
 }
 
 } // namespace nvshmemi
@@ -237,42 +285,16 @@ extern "C" int nvshmemx_batch_plan_create(const nvshmemx_put_request_t *reqs, in
                                      owner ? source_pe(r.source) : 0};
     }
     const int wgs_per_set = owner ? (L.workgroups > 0 ? std::max(1, L.workgroups / nsets) : g.wgs_per_pe)
-                                  : (L.workgroups > 0 ? L.workgroups : 1024);
+                                  : (L.workgroups > 0 ? L.workgroups : 2048);
     nvshmemx_batch_plan *p = new nvshmemx_batch_plan();
     plan_fill(p, rr, choices, nullptr, nsets, wgs_per_set, L.block, L.nontemporal, true);
     *out = p;
     return 0;
 }
 
-nvshmemx_batch_plan *nvshmemi::plan_new() { return new nvshmemx_batch_plan(); }
-
-static void launch_plan(nvshmemx_batch_plan_t p, hipStream_t s) { nvshmemi::plan_launch(p, s); }
-
-void nvshmemi::plan_launch(nvshmemx_batch_plan *p, hipStream_t s) {
-    const unsigned grid = (unsigned)(p->nsets * p->wgs_per_set);
-    switch (p->nontemporal) {
-    case NT_BOTH:
-        hipLaunchKernelGGL(k_batch<NT_BOTH>, dim3(grid), dim3(p->block), 0, s, p->d_pieces, p->d_queues, p->d_ctr,
-                           p->wgs_per_set);
-        break;
-    case NT_LOAD:
-        hipLaunchKernelGGL(k_batch<NT_LOAD>, dim3(grid), dim3(p->block), 0, s, p->d_pieces, p->d_queues, p->d_ctr,
-                           p->wgs_per_set);
-        break;
-    case NT_STORE:
-        hipLaunchKernelGGL(k_batch<NT_STORE>, dim3(grid), dim3(p->block), 0, s, p->d_pieces, p->d_queues, p->d_ctr,
-                           p->wgs_per_set);
-        break;
-    default:
-        hipLaunchKernelGGL(k_batch<NT_NONE>, dim3(grid), dim3(p->block), 0, s, p->d_pieces, p->d_queues, p->d_ctr,
-                           p->wgs_per_set);
-    }
-    NVSHMEMI_HIP(hipGetLastError());
-}
-
 extern "C" void nvshmemx_batch_plan_launch(nvshmemx_batch_plan_t p, hipStream_t s) {
     if (!p) NVSHMEMI_FATAL("nvshmemx_batch_plan_launch: null plan");
-    launch_plan(p, s);
+    plan_launch(p, s);
 }
 
 extern "C" int nvshmemx_batch_plan_capture(nvshmemx_batch_plan_t p, int steps) {
@@ -281,7 +303,7 @@ extern "C" int nvshmemx_batch_plan_capture(nvshmemx_batch_plan_t p, int steps) {
     if (p->graph) NVSHMEMI_HIP(hipGraphDestroy(p->graph));
     if (!p->capture_stream) NVSHMEMI_HIP(hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking));
     NVSHMEMI_HIP(hipStreamBeginCapture(p->capture_stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < steps; ++i) launch_plan(p, p->capture_stream);
+    for (int i = 0; i < steps; ++i) plan_launch(p, p->capture_stream);
     NVSHMEMI_HIP(hipStreamEndCapture(p->capture_stream, &p->graph));
     NVSHMEMI_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
     return 0;
@@ -302,8 +324,8 @@ extern "C" void nvshmemx_batch_plan_destroy(nvshmemx_batch_plan_t p) {
     if (p->graph) NVSHMEMI_HIP(hipGraphDestroy(p->graph));
     if (p->capture_stream) NVSHMEMI_HIP(hipStreamDestroy(p->capture_stream));
     if (p->d_pieces) NVSHMEMI_HIP(hipFree(p->d_pieces));
-    if (p->d_queues) NVSHMEMI_HIP(hipFree(p->d_queues));
-    if (p->d_ctr) NVSHMEMI_HIP(hipFree(p->d_ctr));
+    if (p->d_wg_off) NVSHMEMI_HIP(hipFree(p->d_wg_off));
+    if (p->d_wave_off) NVSHMEMI_HIP(hipFree(p->d_wave_off));
     delete p;
 }
 

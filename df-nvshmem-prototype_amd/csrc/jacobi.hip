@@ -130,12 +130,13 @@ __global__ void __launch_bounds__(256) k_stencil(const double *u_sym, double *v_
     const int j = (blockIdx.y % jb) * 4 + (threadIdx.x >> 6) + 1;
     const int i0 = (blockIdx.y / jb) * chunk + 1;
     if (k > n || j > n) return;
-    const double *u = (const double *)nvshmemx_ptr_d(u_sym, g.pe);
-    double *v = (double *)nvshmemx_ptr_d(v_sym, g.pe);
+    const double *__restrict__ u = (const double *)nvshmemx_ptr_d(u_sym, g.pe);
+    double *__restrict__ v = (double *)nvshmemx_ptr_d(v_sym, g.pe);
     const size_t sx = (size_t)(n + 2) * (n + 2), sy = (size_t)(n + 2);
     size_t c = g.idx(i0, j, k);
     double xm = u[c - sx], x0 = u[c];
     const int i1 = i0 + chunk - 1 < n ? i0 + chunk - 1 : n;
+#pragma unroll 4
     for (int i = i0; i <= i1; ++i, c += sx) {
         const double xp = u[c + sx];
         const double s = ((((xm + xp) + u[c - sy]) + u[c + sy]) + u[c - 1]) + u[c + 1];

@@ -109,7 +109,7 @@ struct resolved_req {
 };
 void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, const nvshmemx_batch_choice_t *choices,
                const nvshmemx_batch_choice_t *one_choice, int nsets, int wgs_per_set, int block, int nontemporal,
-               bool sort_pieces);
+               bool balance);
 void plan_launch(nvshmemx_batch_plan *p, hipStream_t s);
 nvshmemx_batch_plan *plan_new();
 
