@@ -228,7 +228,7 @@ def jacobi_worker(layout, name=None):
     hip.sync()
 
     def total(field):
-        return sum(float(na.jacobi_interior(hip, lib.nvshmem_ptr(field, pe), n).sum(dtype=np.float64))
+        return sum(float(na.jacobi_interior(hip, lib.nvshmem_ptr(field, pe), n, lib).sum(dtype=np.float64))
                    for pe in pes_here)
     s0 = total(u)
     lib.nvshmem_barrier_all()

@@ -23,10 +23,23 @@ __constant__ nvshmemi_device_state_t nvshmemi_dstate;
 
 using namespace nvshmemi;
 
+/* NVSHMEM_IDEALIZE (the reference's IDEALIZE=1 / _ENABLE_IDEALIZE_COMM_,
+ * Makefile:14-16): the reference API's p, g, put, get, quiet and waits compile
+ * to nothing (g returns 0), to time an application's compute alone.  The
+ * nvshmemx_ extensions keep working. */
+#ifdef NVSHMEM_IDEALIZE
+#define NVSHMEMI_IDEAL(...) return __VA_ARGS__
+#else
+#define NVSHMEMI_IDEAL(...) \
+    do {                    \
+    } while (0)
+#endif
+
 /* ------------------------------------------------------------ quiet -- */
 
 extern "C" __device__ void nvshmem_quiet(void) {
     /* reference nvshmem_device.cu:431-436 */
+    NVSHMEMI_IDEAL();
     fence_acq_rel();
 }
 
@@ -34,6 +47,7 @@ extern "C" __device__ void nvshmem_quiet(void) {
 
 #define NVSHMEMI_DEF_P(Name, Type, OpType)                                            \
     extern "C" __device__ void nvshmem_##Name##_p(Type *target, const Type value, int pe) { \
+        NVSHMEMI_IDEAL();                                                             \
         NVSHMEMI_TRACE(OpType, (const void *)0, target, 1, pe, 0);                   \
         store_relaxed((Type *)translate(target, pe), value);                          \
     }
@@ -43,6 +57,7 @@ NVSHMEMI_DEF_P(double, double, NVSHMEMI_OP_PUT_DOUBLE)
 
 #define NVSHMEMI_DEF_G(Name, Type, OpType)                                            \
     extern "C" __device__ Type nvshmem_##Name##_g(Type *source, int pe) {               \
+        NVSHMEMI_IDEAL((Type)0);                                                      \
         NVSHMEMI_TRACE(OpType, (const void *)0, source, 1, pe, 0);                   \
         return load_relaxed((const Type *)translate(source, pe));                     \
     }
@@ -54,6 +69,7 @@ NVSHMEMI_DEF_G(double, double, NVSHMEMI_OP_GET_DOUBLE)
 
 #define NVSHMEMI_DEF_PUT(Name, Type, OpType)                                                \
     extern "C" __device__ void nvshmem_##Name##_put(Type *target, Type *source, int len, int pe) { \
+        NVSHMEMI_IDEAL();                                                                   \
         NVSHMEMI_TRACE(OpType, source, target, len, pe, 0);                                \
         if (len <= 0) return;                                                               \
         copy<16, 4>(translate(target, pe), source, (size_t)len * sizeof(Type), 0, 1);   \
@@ -64,6 +80,7 @@ NVSHMEMI_DEF_PUT(double, double, NVSHMEMI_OP_PUT_DOUBLE)
 
 #define NVSHMEMI_DEF_GET(Name, Type, OpType)                                                \
     extern "C" __device__ void nvshmem_##Name##_get(Type *target, Type *source, int len, int pe) { \
+        NVSHMEMI_IDEAL();                                                                   \
         NVSHMEMI_TRACE(OpType, target, source, len, pe, 0);                                \
         if (len <= 0) return;                                                               \
         copy<16, 4>(target, translate(source, pe), (size_t)len * sizeof(Type), 0, 1);   \
@@ -76,6 +93,7 @@ NVSHMEMI_DEF_GET(double, double, NVSHMEMI_OP_GET_DOUBLE)
 
 extern "C" __device__ void nvshmem_int_wait(int *ivar, int cmp_value) {
     /* reference nvshmem_device.cu:542-562: block while *ivar == cmp_value */
+    NVSHMEMI_IDEAL();
     NVSHMEMI_TRACE_TIMED_BEGIN();
     wait_until(ivar, CMP_NE, cmp_value);
     NVSHMEMI_TRACE_TIMED_END(NVSHMEMI_OP_WAIT_INT, ivar, (const void *)0, 1, my_pe());
@@ -85,6 +103,7 @@ extern "C" __device__ void nvshmem_int_wait_until(int *ivar, int cmp, int cmp_va
     /* reference nvshmem_device.cu:564-584: block until *ivar == cmp_value;
      * the comparison operator `cmp` is ignored, as in the reference */
     (void)cmp;
+    NVSHMEMI_IDEAL();
     NVSHMEMI_TRACE_TIMED_BEGIN();
     wait_until(ivar, CMP_EQ, cmp_value);
     NVSHMEMI_TRACE_TIMED_END(NVSHMEMI_OP_WAIT_UNTIL_INT, ivar, (const void *)0, 1, my_pe());

@@ -42,6 +42,17 @@ namespace {
 
 constexpr uint64_t kMult = 6364136223846793005ULL;
 
+/* Row layout: a row of n + 2 cells (k = 0 .. n+1) sits at offset kKOff in a
+ * row of n + kRowPad doubles, so interior cell k = 1 starts on a 128-byte
+ * line whenever n is a multiple of 16 (rows are then whole lines long). */
+constexpr int kKOff = 15;
+constexpr int kRowPad = 32;
+
+/* stencil tile: kTJ rows x 64 columns per workgroup of 256 threads, each
+ * thread owning one column of kTJ / 4 rows */
+constexpr int kTJ = 16;
+constexpr int kTK = 64;
+
 __device__ __forceinline__ uint32_t out32(uint64_t old) {
     const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
     const uint32_t rot = (uint32_t)(old >> 59u);
@@ -71,8 +82,9 @@ struct geom {
         cy = (pe >> 1) & 1;
         cz = pe & 1;
     }
+    __device__ __forceinline__ size_t row_stride() const { return (size_t)n + kRowPad; }
     __device__ __forceinline__ size_t idx(int i, int j, int k) const {
-        return ((size_t)i * (n + 2) + j) * (size_t)(n + 2) + k;
+        return ((size_t)i * (n + 2) + j) * row_stride() + (size_t)(k + kKOff);
     }
     /* neighbour PE across side d (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z), periodic 2x2x2 */
     __device__ __forceinline__ int neighbour(int d) const {
@@ -123,26 +135,90 @@ __global__ void k_unpack(double *u_sym, const double *recv_sym, int n, int pe0, 
 }
 
 /* thread (k, j) walks planes i in [i0, i0 + chunk) keeping x-1, x, x+1 in registers */
+/* Workgroup (k tile, j tile x i chunk, PE): walks planes i0..i1 of a
+ * kTJ x kTK tile.  Each plane (tile plus a one-cell ring in j and k) is read
+ * from memory once into a double-buffered LDS ring; plane i+1 is fetched
+ * while plane i is consumed.  Thread (wave w, lane l) owns column k0 + l of
+ * rows j0 + 4w .. j0 + 4w + 3 and keeps their x-1 values in registers. */
 __global__ void __launch_bounds__(256) k_stencil(const double *u_sym, double *v_sym, int n, int pe0, int chunk) {
     const geom g(n, pe0 + (int)blockIdx.z);
-    const int k = blockIdx.x * 64 + (threadIdx.x & 63) + 1;
-    const int jb = (int)(gridDim.y / ((n + chunk - 1) / chunk));
-    const int j = (blockIdx.y % jb) * 4 + (threadIdx.x >> 6) + 1;
-    const int i0 = (blockIdx.y / jb) * chunk + 1;
-    if (k > n || j > n) return;
+    const int jtiles = (n + kTJ - 1) / kTJ;
+    const int k0 = (int)blockIdx.x * kTK + 1;
+    const int j0 = (int)(blockIdx.y % jtiles) * kTJ + 1;
+    const int i0 = (int)(blockIdx.y / jtiles) * chunk + 1;
+    const int i1 = i0 + chunk - 1 < n ? i0 + chunk - 1 : n;
     const double *__restrict__ u = (const double *)nvshmemx_ptr_d(u_sym, g.pe);
     double *__restrict__ v = (double *)nvshmemx_ptr_d(v_sym, g.pe);
-    const size_t sx = (size_t)(n + 2) * (n + 2), sy = (size_t)(n + 2);
-    size_t c = g.idx(i0, j, k);
-    double xm = u[c - sx], x0 = u[c];
-    const int i1 = i0 + chunk - 1 < n ? i0 + chunk - 1 : n;
-#pragma unroll 4
-    for (int i = i0; i <= i1; ++i, c += sx) {
-        const double xp = u[c + sx];
-        const double s = ((((xm + xp) + u[c - sy]) + u[c + sy]) + u[c - 1]) + u[c + 1];
-        v[c] = 0.25 * x0 + 0.125 * s;
-        xm = x0;
-        x0 = xp;
+    const size_t sy = g.row_stride(), sx = (size_t)(n + 2) * sy;
+    __shared__ double tile[3][kTJ + 2][kTK + 2];
+    const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
+    const int kcol = k0 + lane;
+    const int jlim = j0 + kTJ < n + 1 ? j0 + kTJ : n + 1; /* last row to load (halo included) */
+    const int klim = k0 + kTK < n + 1 ? k0 + kTK : n + 1; /* last column to load */
+    constexpr int RPW = kTJ / 4;                          /* rows per wavefront */
+    constexpr int NR = (kTJ + 2 + 3) / 4;                 /* tile rows a wavefront loads */
+    const bool has_halo = threadIdx.x < 2 * (kTJ + 2);
+    const int hr = (int)(threadIdx.x >> 1), hhi = (int)(threadIdx.x & 1u);
+    const int hk = hhi ? k0 + kTK : k0 - 1;
+
+    /* plane i of the tile into registers (issued early), then into LDS */
+    double pre[NR], preh = 0.0;
+    auto issue = [&](int i) {
+        const size_t pb = (size_t)i * sx + kKOff;
+#pragma unroll
+        for (int rr = 0; rr < NR; ++rr) {
+            const int r = wv + 4 * rr, j = j0 - 1 + r;
+            pre[rr] = (r < kTJ + 2 && j <= jlim && kcol <= klim) ? u[pb + (size_t)j * sy + kcol] : 0.0;
+        }
+        if (has_halo) {
+            const int j = j0 - 1 + hr;
+            preh = (j <= jlim && hk <= n + 1) ? u[pb + (size_t)j * sy + hk] : 0.0;
+        }
+    };
+    auto commit = [&](int buf) {
+#pragma unroll
+        for (int rr = 0; rr < NR; ++rr) {
+            const int r = wv + 4 * rr;
+            if (r < kTJ + 2) tile[buf][r][lane + 1] = pre[rr];
+        }
+        if (has_halo) tile[buf][hr][hhi ? kTK + 1 : 0] = preh;
+    };
+
+    double xm[RPW];
+    bool ok[RPW];
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        const int j = j0 + wv * RPW + q;
+        ok[q] = kcol <= n && j <= n;
+        xm[q] = ok[q] ? u[(size_t)(i0 - 1) * sx + (size_t)j * sy + kcol + kKOff] : 0.0;
+    }
+    issue(i0);
+    commit(0);
+    issue(i0 + 1);
+    commit(1);
+    __syncthreads();
+    int bc = 0, bn = 1, bf = 2; /* planes i, i + 1, and the one being fetched */
+    for (int i = i0; i <= i1; ++i) {
+        const bool fetch = i + 2 <= i1 + 1;
+        if (fetch) issue(i + 2);
+        const size_t vb = (size_t)i * sx + kKOff + kcol;
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) {
+            const int r = wv * RPW + q + 1;
+            const double x0 = tile[bc][r][lane + 1];
+            const double xp = tile[bn][r][lane + 1];
+            const double s = ((((xm[q] + xp) + tile[bc][r - 1][lane + 1]) + tile[bc][r + 1][lane + 1]) +
+                              tile[bc][r][lane]) +
+                             tile[bc][r][lane + 2];
+            if (ok[q]) v[vb + (size_t)(j0 + r - 1) * sy] = 0.25 * x0 + 0.125 * s;
+            xm[q] = x0;
+        }
+        if (fetch) commit(bf);
+        __syncthreads();
+        const int t = bc;
+        bc = bn;
+        bn = bf;
+        bf = t;
     }
 }
 
@@ -172,7 +248,13 @@ void check_n(int n) {
 extern "C" {
 
 /* bytes of each symmetric allocation: u / v and the receive area */
-size_t nvshmemh_jacobi_field_bytes(int n) { return (size_t)(n + 2) * (n + 2) * (n + 2) * sizeof(double); }
+size_t nvshmemh_jacobi_field_bytes(int n) { return (size_t)(n + 2) * (n + 2) * (size_t)(n + kRowPad) * sizeof(double); }
+
+/* field layout: cell (i, j, k) is double number ((i * (n+2) + j) * row_stride + k + koff) */
+void nvshmemh_jacobi_layout(int n, int *row_stride, int *koff) {
+    *row_stride = n + kRowPad;
+    *koff = kKOff;
+}
 size_t nvshmemh_jacobi_recv_bytes(int n) { return (size_t)2 * 6 * n * n * sizeof(double); }
 
 /* pe0..pe0+npes_here-1 are the PEs this call works for (1 in process mode,
@@ -205,9 +287,9 @@ void nvshmemh_jacobi_unpack(double *u_sym, double *recv_sym, int n, int pe0, int
 
 void nvshmemh_jacobi_stencil(double *u_sym, double *v_sym, int n, int pe0, int npes_here, int chunk, hipStream_t s) {
     if (chunk <= 0) chunk = n;
-    const int jb = (n + 3) / 4, ic = (n + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_stencil, dim3((unsigned)((n + 63) / 64), (unsigned)(jb * ic), npes_here), dim3(256), 0, s,
-                       u_sym, v_sym, n, pe0, chunk);
+    const int jt = (n + kTJ - 1) / kTJ, ic = (n + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_stencil, dim3((unsigned)((n + kTK - 1) / kTK), (unsigned)(jt * ic), npes_here), dim3(256), 0,
+                       s, u_sym, v_sym, n, pe0, chunk);
     NVSHMEMI_HIP(hipGetLastError());
 }
 

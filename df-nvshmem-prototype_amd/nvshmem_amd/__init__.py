@@ -28,8 +28,10 @@ ERR_TRACE_OVERFLOW = 0x8
 ERR_TRACE_CONFLICT = 0x10
 
 
-def lib_path(trace: bool = False) -> Path:
-    return (BUILD_DIR / "trace" if trace else BUILD_DIR) / "libnvshmem_amd.so"
+def lib_path(trace: bool = False, variant: str = "") -> Path:
+    """build/libnvshmem_amd.so, or the tracing / idealized-communication build"""
+    sub = "trace" if trace else variant
+    return (BUILD_DIR / sub if sub else BUILD_DIR) / "libnvshmem_amd.so"
 
 
 class LibraryMissing(RuntimeError):
@@ -145,6 +147,7 @@ SIGNATURES.update({
 
 SIGNATURES.update({
     "nvshmemh_jacobi_field_bytes": (_SZ, [_I]),
+    "nvshmemh_jacobi_layout": (None, [_I, _P, _P]),
     "nvshmemh_jacobi_recv_bytes": (_SZ, [_I]),
     "nvshmemh_jacobi_init": (None, [_P, _I, _I, _I, _U64, _P]),
     "nvshmemh_jacobi_pack_put": (None, [_P, _P, _I, _I, _I, _I, _P]),
@@ -154,10 +157,21 @@ SIGNATURES.update({
 })
 
 
-def jacobi_interior(hip, field_ptr, n):
-    """(n, n, n) interior of one PE's (n+2)^3 field"""
-    a = hip.dtoh(field_ptr, (n + 2) ** 3 * 8, "<f8").reshape(n + 2, n + 2, n + 2)
-    return a[1:-1, 1:-1, 1:-1].copy()
+def jacobi_interior(hip, field_ptr, n, lib=None):
+    """(n, n, n) interior of one PE's field (layout from nvshmemh_jacobi_layout)"""
+    rs, ko = C.c_int(), C.c_int()
+    (lib or _loaded_lib()).nvshmemh_jacobi_layout(n, C.byref(rs), C.byref(ko))
+    a = hip.dtoh(field_ptr, (n + 2) * (n + 2) * rs.value * 8, "<f8").reshape(n + 2, n + 2, rs.value)
+    return a[1:-1, 1:-1, ko.value + 1:ko.value + 1 + n].copy()
+
+
+_LIBS = {}
+
+
+def _loaded_lib():
+    if "default" not in _LIBS:
+        _LIBS["default"] = load()
+    return _LIBS["default"]
 
 
 def jacobi_block(global_field, pe, n):
@@ -301,8 +315,8 @@ class CopyConfig(C.Structure):
                 ("mode", _I)]
 
 
-def load(trace: bool = False) -> C.CDLL:
-    path = lib_path(trace)
+def load(trace: bool = False, variant: str = "") -> C.CDLL:
+    path = lib_path(trace, variant)
     if not path.exists():
         raise LibraryMissing(f"{path} is not built; run __graft_entry__.build() (make -C {PKG_DIR})")
     lib = C.CDLL(str(path), mode=C.RTLD_GLOBAL)

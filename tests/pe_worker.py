@@ -105,7 +105,7 @@ def main():
         res = lib.nvshmemh_jacobi_run(u, v, recv, n, steps, 0, 0, None)
         hip.sync()
         lib.nvshmem_barrier_all()
-        np.save(outdir / f"jacobi_pe{me}.npy", na.jacobi_interior(hip, res, n))
+        np.save(outdir / f"jacobi_pe{me}.npy", na.jacobi_interior(hip, res, n, lib))
     out["errors"] = lib.nvshmemx_get_last_error()
     lib.nvstop_pes()
     print(json.dumps(out), flush=True)

@@ -42,7 +42,7 @@ def test_jacobi_local_bitexact(local_pes, hip, oracle, n, steps, chunk):
     out = _run_local(lib, hip, n, steps, chunk)
     want = oracle.jacobi(2 * n, steps, SEED)
     for pe in range(8):
-        got = na.jacobi_interior(hip, lib.nvshmem_ptr(out, pe), n)
+        got = na.jacobi_interior(hip, lib.nvshmem_ptr(out, pe), n, lib)
         blk = na.jacobi_block(want, pe, n)
         if not np.array_equal(got.view(np.uint64), blk.view(np.uint64)):
             bad = np.argwhere(got != blk)
