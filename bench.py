@@ -121,12 +121,15 @@ def batch_worker():
 def learned_worker(n_batches, train_steps):
     """Learned config choice on the uneven-put workload (64 workgroup PEs):
     1. exhaustive autotuner: every one of the 54 candidates measured at every
-       point of a 990-point feature grid (53,460 samples);
+       point of a 1188-point feature grid (64,152 samples);
     2. the MLP cost model trained on those samples on the GPU;
     3. `n_batches` held-out batches (seeds 1000..) of 4096 puts each, planned
        with the MLP's choices, the table's choices (exhaustive autotuner) and
        the library's untuned rule, each timed as one fused launch; the per-call
-       path is timed on the first 10.  Prints one JSON line."""
+       path is timed on the first 10;
+    4. 100 batches each of 16 and 128 puts from the same size distribution,
+       where how many puts share the launch decides the best split.
+    Prints one JSON line."""
     import ctypes as C
     import numpy as np
     import nvshmem_amd as na
@@ -164,23 +167,48 @@ def learned_worker(n_batches, train_steps):
         lib.nvshmemx_batch_plan_destroy(plan)
         return ms
 
-    rec = {"mlp": [], "table": [], "rule": [], "per_call": []}
+    rec = {"mlp": [], "table": [], "rule": [], "one_piece": [], "per_call": []}
+    choose_ms = []
     bad = 0
     t0 = time.time()
     for b in range(n_batches):
         reqs = na.gen_requests(lib, 1000 + b, BATCH_REQS, BATCH_NPES, BATCH_SRC, BATCH_DST)
         arr = na.make_requests(lib, reqs, base, 0, BATCH_SRC)
         n = len(arr)
-        ch_mlp = model.choose(arr, n, n)
+        tc = time.perf_counter()
+        ch_mlp = model.choose(arr, n, 0)          # per-request effective concurrency
+        choose_ms.append((time.perf_counter() - tc) * 1e3)
         ch_tab = (na.BatchChoice * n)()
-        lib.nvshmemx_tune_table_choose(feats.ctypes.data, times.ctypes.data, len(times), arr, n, n, ch_tab)
+        lib.nvshmemx_tune_table_choose(feats.ctypes.data, times.ctypes.data, len(times), arr, n, 0, ch_tab)
+        one = (na.BatchChoice * n)()
+        for i in range(n):
+            one[i].pieces, one[i].vec_bytes, one[i].mode = 1, 16, 1
         rec["mlp"].append(timed_plan(arr, n, ch_mlp))
         rec["table"].append(timed_plan(arr, n, ch_tab))
         rec["rule"].append(timed_plan(arr, n, None))
+        rec["one_piece"].append(timed_plan(arr, n, one))
         if b < 10:
             rec["per_call"].append(timed(hip, lambda: lib.nvshmemx_put_requests_on_stream(arr, n, None), 1, 0))
         if b < 3:
             bad += int(lib.nvshmemh_verify_requests(arr, n))
+    # fewer concurrent puts from the same size distribution: the concurrency
+    # input is what the untuned rule cannot see
+    small = {}
+    for nreq in (16, 128):
+        rs = {"mlp": [], "table": [], "rule": []}
+        for b in range(100):
+            reqs = na.gen_requests(lib, 5000 + 1000 * nreq + b, nreq, BATCH_NPES, BATCH_SRC, BATCH_DST)
+            arr = na.make_requests(lib, reqs, base, 0, BATCH_SRC)
+            ch_mlp = model.choose(arr, nreq, 0)
+            ch_tab = (na.BatchChoice * nreq)()
+            lib.nvshmemx_tune_table_choose(feats.ctypes.data, times.ctypes.data, len(times), arr, nreq, 0, ch_tab)
+            rs["mlp"].append(timed_plan(arr, nreq, ch_mlp))
+            rs["table"].append(timed_plan(arr, nreq, ch_tab))
+            rs["rule"].append(timed_plan(arr, nreq, None))
+        small[str(nreq)] = {"mean_ms": {k: round(float(np.mean(v)), 4) for k, v in rs.items()},
+                            "mlp_over_table": round(float(np.mean(rs["mlp"]) / np.mean(rs["table"])), 4),
+                            "mlp_gain_over_rule": round(float(np.mean(rs["rule"]) / np.mean(rs["mlp"])), 4)}
+    out["fewer_puts"] = small
     out["eval_seconds"] = round(time.time() - t0, 1)
     m, t, r = (np.array(rec[k]) for k in ("mlp", "table", "rule"))
     pc = np.array(rec["per_call"])
@@ -189,7 +217,9 @@ def learned_worker(n_batches, train_steps):
         "mlp_over_table": round(float(np.mean(m) / np.mean(t)), 4),
         "mlp_within_5pct_of_table_frac": round(float(np.mean(m <= 1.05 * t)), 3),
         "mlp_gain_over_rule": round(float(np.mean(r) / np.mean(m)), 4),
+        "mlp_gain_over_one_piece": round(float(np.mean(rec["one_piece"]) / np.mean(m)), 4),
         "mlp_speedup_vs_per_call": round(float(np.mean(pc) / np.mean(m[:len(pc)])), 2),
+        "mlp_choose_ms_per_batch": round(float(np.median(choose_ms)), 3),
         "bitexact": bad == 0,
         "errors": lib.nvshmemx_get_last_error(),
     })

@@ -29,6 +29,7 @@
  */
 #include <algorithm>
 #include <queue>
+#include <stdlib.h>
 #include <vector>
 
 #include "nvshmem.h"
@@ -47,8 +48,11 @@ struct piece_t {
 };
 
 /* per-piece charge in the LPT cost, in bytes: about one access latency of a
- * wavefront streaming 4 x 1 KiB */
-constexpr size_t kPieceCharge = 4096;
+ * wavefront streaming 4 x 1 KiB (NVSHMEM_BATCH_PIECE_CHARGE overrides it) */
+static double piece_charge() {
+    const char *e = getenv("NVSHMEM_BATCH_PIECE_CHARGE");
+    return e && *e ? atof(e) : 4096.0;
+}
 
 template <int NT>
 __device__ __forceinline__ void copy_piece(const piece_t &p, unsigned lane, unsigned n) {
@@ -100,11 +104,12 @@ static std::vector<std::vector<piece_t>> lpt(std::vector<piece_t> pieces, int wo
     typedef std::pair<double, int> item;
     std::priority_queue<item, std::vector<item>, std::greater<item>> heap;
     for (int w = 0; w < workers; ++w) heap.push(item(load[(size_t)w], w));
+    const double charge = piece_charge();
     for (const piece_t &p : pieces) {
         item top = heap.top();
         heap.pop();
         lists[(size_t)top.second].push_back(p);
-        top.first += (double)(p.nbytes / share + kPieceCharge);
+        top.first += (double)(p.nbytes / share) + charge;
         load[(size_t)top.second] = top.first;
         heap.push(top);
     }
@@ -144,13 +149,25 @@ static void ensure_capacity(void **d, size_t &cap, size_t need, size_t elem) {
     NVSHMEMI_HIP(hipMalloc(d, cap * elem));
 }
 
+/* Fisher-Yates with a fixed xorshift64 stream: a reproducible order */
+static void scatter(std::vector<piece_t> &v) {
+    unsigned long long x = 0x9E3779B97F4A7C15ull;
+    for (size_t i = v.size(); i > 1; --i) {
+        x ^= x << 13, x ^= x >> 7, x ^= x << 17;
+        std::swap(v[i - 1], v[(size_t)(x % i)]);
+    }
+}
+
 /* (Re)build plan `p` from resolved requests; device buffers are reused when
  * they are large enough, so the autotuner can rebuild per candidate without
- * allocating.  `balance` = false skips the LPT assignment and deals pieces
- * round-robin (used when every piece is the same size). */
+ * allocating.  ASSIGN_LPT balances uneven pieces; ASSIGN_ROUND_ROBIN deals
+ * pieces in request order (same-size pieces); ASSIGN_SCATTERED deals them in
+ * a fixed pseudo-random order, so that pieces running at the same time are not
+ * address-adjacent -- as in a mixed batch, whose LPT order interleaves the
+ * pieces of unrelated requests. */
 void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, const nvshmemx_batch_choice_t *choices,
                const nvshmemx_batch_choice_t *one_choice, int nsets, int wgs_per_set, int block, int nontemporal,
-               bool balance) {
+               int assign) {
     if (block < 64 || block > 1024 || block % 64) NVSHMEMI_FATAL("batch block size %d invalid", block);
     const int waves_per_wg = block / 64;
     std::vector<std::vector<piece_t>> wg(nsets), wave(nsets);
@@ -171,7 +188,7 @@ void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, co
     for (int s = 0; s < nsets; ++s) {
         const size_t g0 = (size_t)s * wgs_per_set, w0 = g0 * waves_per_wg;
         std::vector<std::vector<piece_t>> a, b;
-        if (balance) {
+        if (assign == ASSIGN_LPT) {
             std::vector<double> wg_load((size_t)wgs_per_set, 0.0);
             a = lpt(wg[s], wgs_per_set, (size_t)waves_per_wg, wg_load);
             /* a wavefront starts its own pieces after its workgroup's */
@@ -179,6 +196,7 @@ void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, co
             for (size_t w = 0; w < wave_load.size(); ++w) wave_load[w] = wg_load[w / waves_per_wg];
             b = lpt(wave[s], wgs_per_set * waves_per_wg, 1, wave_load);
         } else {
+            if (assign == ASSIGN_SCATTERED) scatter(wg[s]), scatter(wave[s]);
             a.resize((size_t)wgs_per_set);
             b.resize((size_t)wgs_per_set * waves_per_wg);
             for (size_t i = 0; i < wg[s].size(); ++i) a[i % a.size()].push_back(wg[s][i]);
@@ -287,7 +305,7 @@ extern "C" int nvshmemx_batch_plan_create(const nvshmemx_put_request_t *reqs, in
     const int wgs_per_set = owner ? (L.workgroups > 0 ? std::max(1, L.workgroups / nsets) : g.wgs_per_pe)
                                   : (L.workgroups > 0 ? L.workgroups : 2048);
     nvshmemx_batch_plan *p = new nvshmemx_batch_plan();
-    plan_fill(p, rr, choices, nullptr, nsets, wgs_per_set, L.block, L.nontemporal, true);
+    plan_fill(p, rr, choices, nullptr, nsets, wgs_per_set, L.block, L.nontemporal, ASSIGN_LPT);
     *out = p;
     return 0;
 }

@@ -332,17 +332,25 @@ extern "C" int nvshmemx_costmodel_choose(nvshmemx_costmodel_t m, const nvshmemx_
                                          int concurrency, nvshmemx_batch_choice_t *out) {
     if (n < 1) return 0;
     std::vector<float> X((size_t)n * IN), Y((size_t)n * m->C);
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) total += reqs[i].nbytes;
     for (int i = 0; i < n; ++i) {
         const void *dst = g.initialized ? nvshmem_ptr(reqs[i].target, reqs[i].pe) : reqs[i].target;
-        nvshmemx_tune_features(log2((double)std::max<size_t>(1, reqs[i].nbytes)), reqs[i].elem_bytes, concurrency,
+        nvshmemx_tune_features(log2((double)std::max<size_t>(1, reqs[i].nbytes)), reqs[i].elem_bytes,
+                               nvshmemx_effective_concurrency(reqs[i].nbytes, total, concurrency),
                                nvshmemx_align_class(reqs[i].source, dst), &X[(size_t)i * IN]);
     }
     nvshmemx_costmodel_predict(m, X.data(), n, Y.data());
+    const size_t cap = balance_cap(total);
     for (int i = 0; i < n; ++i) {
         const float *y = &Y[(size_t)i * m->C];
         int best = 0;
-        for (int j = 1; j < m->C; ++j)
-            if (y[j] < y[best]) best = j;
+        if (m->C == nvshmemx_tune_candidate_count()) {
+            best = pick_candidate(y, reqs[i].nbytes, cap, true);
+        } else {
+            for (int j = 1; j < m->C; ++j)
+                if (y[j] < y[best]) best = j;
+        }
         nvshmemx_tune_candidate(best, &out[i]);
     }
     return 0;

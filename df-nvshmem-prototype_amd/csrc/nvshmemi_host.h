@@ -107,9 +107,14 @@ struct resolved_req {
     size_t nbytes;
     int set;
 };
+enum { ASSIGN_LPT = 0, ASSIGN_ROUND_ROBIN = 1, ASSIGN_SCATTERED = 2 };
+/* autotune.hip: largest balanced piece of a batch (0: no cap) and the
+ * constrained choice over the 54 candidates' costs (times or log-times) */
+size_t balance_cap(size_t batch_bytes);
+int pick_candidate(const float *cost, size_t nbytes, size_t cap, bool log_cost);
 void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, const nvshmemx_batch_choice_t *choices,
                const nvshmemx_batch_choice_t *one_choice, int nsets, int wgs_per_set, int block, int nontemporal,
-               bool balance);
+               int assign);
 void plan_launch(nvshmemx_batch_plan *p, hipStream_t s);
 nvshmemx_batch_plan *plan_new();
 

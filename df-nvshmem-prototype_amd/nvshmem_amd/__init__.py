@@ -132,6 +132,7 @@ SIGNATURES.update({
     "nvshmemx_tune_features": (None, [C.c_double, _I, _I, _I, _P]),
     "nvshmemx_tune_measure": (_I, [_I, _P, _P, _P, _P, _I, _I, _I, _P]),
     "nvshmemx_tune_table_choose": (_I, [_P, _P, _I, _P, _I, _I, _P]),
+    "nvshmemx_effective_concurrency": (_I, [_SZ, _SZ, _I]),
     "nvshmemx_costmodel_create": (_I, [_I, _U64, C.POINTER(_P)]),
     "nvshmemx_costmodel_num_params": (_I, [_P]),
     "nvshmemx_costmodel_get_params": (None, [_P, _P]),
@@ -193,7 +194,7 @@ def candidates(lib):
     return out
 
 
-def tune_grid(half_octaves=True, widths=(4, 8), concurrency=(1, 8, 64, 512, 4096), aligns=(0, 1, 2),
+def tune_grid(half_octaves=True, widths=(4, 8), concurrency=(1, 8, 64, 512, 4096, 32768), aligns=(0, 1, 2),
               min_log2=6, max_log2=22):
     """feature points of the autotuner's exhaustive table"""
     import numpy as np
@@ -205,7 +206,7 @@ def tune_grid(half_octaves=True, widths=(4, 8), concurrency=(1, 8, 64, 512, 4096
             "concurrency": np.array([p[2] for p in pts], np.int32), "align": np.array([p[3] for p in pts], np.int32)}
 
 
-def tune_measure(lib, grid, reps=3, workgroups=1024, block=256):
+def tune_measure(lib, grid, reps=3, workgroups=2048, block=256):
     """times_us[npts, 54] from on-device measurement of every candidate"""
     import numpy as np
     n = len(grid["log2_bytes"])

@@ -148,6 +148,11 @@ void nvshmemx_tune_features(double log2_bytes, int elem_bytes, int concurrency, 
  * scratch buffers, HIP-event timed); times_us[p * 54 + c].  Returns 0. */
 int nvshmemx_tune_measure(int npts, const double *log2_bytes, const int *elem_bytes, const int *concurrency,
                           const int *align_class, int reps, int workgroups, int block, float *times_us);
+/* Concurrency feature of a request inside a batch of `batch_bytes`: the
+ * number of same-size requests carrying that many bytes (clamped to
+ * [1, 32768]); `given` > 0 is returned unchanged.  The choose functions below
+ * use it when their `concurrency` argument is <= 0. */
+int nvshmemx_effective_concurrency(size_t nbytes, size_t batch_bytes, int given);
 /* Per request: the measured-best candidate of the nearest measured point. */
 int nvshmemx_tune_table_choose(const float *features, const float *times_us, int npts,
                                const nvshmemx_put_request_t *reqs, int nreq, int concurrency,
