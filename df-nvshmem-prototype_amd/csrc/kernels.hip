@@ -17,25 +17,31 @@
  * resets the counter and advances the generation with release.  A wait that
  * outlives the wall-clock budget records NVSHMEMI_ERR_BARRIER_TIMEOUT.
  */
+#include <algorithm>
+
 #include "nvshmemi_device.h"
 #include "nvshmemi_host.h"
 
 namespace nvshmemi {
 
 /* MODE 0: all lanes of the grid interleave over the whole range (grid-stride);
- * MODE 1: each workgroup streams one contiguous slice. */
+ * MODE 1: each workgroup streams one contiguous slice;
+ * MODE 2: tiles of U * blockDim vectors alternating between `streams` address
+ * runs, normally one tile per workgroup. */
 template <int W, int U, int NT, int MODE>
-__global__ void __launch_bounds__(1024) k_copy(char *dst, const char *src, size_t nbytes) {
+__global__ void __launch_bounds__(1024) k_copy(char *dst, const char *src, size_t nbytes, int streams) {
     if constexpr (MODE == 0) {
         const size_t lane = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
         const size_t n = (size_t)gridDim.x * blockDim.x;
         copy<W, U, NT>(dst, src, nbytes, lane, n);
-    } else {
+    } else if constexpr (MODE == 1) {
         copy_chunked<W, U, NT>(dst, src, nbytes, blockIdx.x, gridDim.x, threadIdx.x, blockDim.x);
+    } else {
+        copy_tiled<W, U, NT>(dst, src, nbytes, (size_t)streams, blockIdx.x, gridDim.x, threadIdx.x, blockDim.x);
     }
 }
 
-typedef void (*copy_kernel_t)(char *, const char *, size_t);
+typedef void (*copy_kernel_t)(char *, const char *, size_t, int);
 
 template <int W, int NT, int MODE> static copy_kernel_t pick_unroll(int u) {
     switch (u) {
@@ -62,23 +68,34 @@ template <int MODE> static copy_kernel_t pick_width(const CopyConfig &c) {
 }
 
 static copy_kernel_t pick_copy_kernel(const CopyConfig &c) {
+    if (c.mode == 2) return pick_width<2>(c);
     return c.mode == 1 ? pick_width<1>(c) : pick_width<0>(c);
 }
 
-/* Launch shape by size, from a sweep of {interleaved, chunked} x block
- * {256,512,1024} x grid {2K..32K} x unroll {2,4,8} x NT {none,load,store,both}
- * on MI355X (copies rotated over a 1 GiB footprint so the 256 MB MALL cannot
- * serve them).  Non-temporal loads and stores won at every size >= 64 MiB. */
+/* Launch shape by size, from sweeps on MI355X of {interleaved, chunked,
+ * tiled} x block {256,512,1024} x grid x unroll {1,2,4,8} x streams x NT
+ * (copies rotated over a 1 GiB footprint so the 256 MB MALL cannot serve
+ * them; means of three passes):
+ *   >= 512 MiB        tiled, 512 threads x 1 access (8 KiB tiles), 2 runs: 6.5 TB/s at 1 GiB
+ *   128 .. 512 MiB    interleaved, 16384 x 512, 8 in flight:             6.4 TB/s at 256 MiB
+ *   32 .. 128 MiB     interleaved, 4096 x 512, 8 in flight:              5.9 TB/s at 64 MiB
+ *   below             interleaved, grid sized to the copy (<= 4096)
+ * Non-temporal loads and stores won at every size >= 8 MiB. */
 CopyConfig default_copy_config(size_t nbytes) {
     CopyConfig c;
+    c.streams = 1;
     c.vec_bytes = 16;
     c.nontemporal = NT_BOTH;
     if (nbytes >= ((size_t)512 << 20)) {
-        c.mode = 1, c.block = 1024, c.workgroups = 8192, c.unroll = 8;
+        c.mode = 2, c.block = 512, c.workgroups = 0, c.unroll = 1, c.streams = 2;
         return c;
     }
     if (nbytes >= ((size_t)128 << 20)) {
         c.mode = 0, c.block = 512, c.workgroups = 16384, c.unroll = 8;
+        return c;
+    }
+    if (nbytes >= ((size_t)32 << 20)) {
+        c.mode = 0, c.block = 512, c.workgroups = 4096, c.unroll = 8;
         return c;
     }
     c.mode = 0;
@@ -93,12 +110,40 @@ CopyConfig default_copy_config(size_t nbytes) {
     return c;
 }
 
+/* Mode 2 grid: one tile per workgroup at the widest access the relative
+ * alignment allows (tiles are rounded up to whole runs), capped so the grid
+ * stays a valid launch; the kernel's tile loop covers any remainder. */
+static size_t tiled_grid(const void *dst, const void *src, size_t nbytes, const CopyConfig &c) {
+    const uintptr_t rel = (uintptr_t)dst ^ (uintptr_t)src;
+    size_t w = (size_t)(c.vec_bytes >= 16 ? 16 : (c.vec_bytes >= 8 ? 8 : 4));
+    while (w > 1 && (rel & (w - 1))) w >>= 1;
+    const size_t tile_bytes = w * (size_t)c.unroll * (size_t)c.block;
+    const size_t tiles = (nbytes + tile_bytes - 1) / tile_bytes;
+    const size_t streams = (size_t)c.streams;
+    size_t g = (tiles + streams - 1) / streams * streams;
+    if (g < 1) g = 1;
+    if (g > ((size_t)1 << 30)) g = (size_t)1 << 30;
+    return g;
+}
+
 void launch_copy(void *dst, const void *src, size_t nbytes, const CopyConfig &cfg, hipStream_t s) {
     if (cfg.block < 64 || cfg.block > 1024 || cfg.block % 64 != 0)
         NVSHMEMI_FATAL("copy block size %d must be a multiple of 64 in [64, 1024]", cfg.block);
-    if (cfg.workgroups < 1) NVSHMEMI_FATAL("copy grid size %d must be positive", cfg.workgroups);
+    if (cfg.mode < 0 || cfg.mode > 2) NVSHMEMI_FATAL("copy mode %d must be 0, 1 or 2", cfg.mode);
+    if (cfg.mode == 2 && (cfg.streams < 1 || cfg.streams > 64))
+        NVSHMEMI_FATAL("copy streams %d must be in [1, 64]", cfg.streams);
+    size_t grid = (size_t)(cfg.workgroups > 0 ? cfg.workgroups : 0);
+    if (grid == 0) {
+        if (cfg.mode == 2) {
+            grid = tiled_grid(dst, src, nbytes, cfg);
+        } else { /* one pass of `unroll` accesses per lane, at most 16384 workgroups */
+            const size_t per_wg = (size_t)cfg.block * 16 * (size_t)(cfg.unroll > 0 ? cfg.unroll : 1);
+            grid = std::min<size_t>(16384, std::max<size_t>(1, (nbytes + per_wg - 1) / per_wg));
+        }
+    }
     copy_kernel_t k = pick_copy_kernel(cfg);
-    hipLaunchKernelGGL(k, dim3(cfg.workgroups), dim3(cfg.block), 0, s, (char *)dst, (const char *)src, nbytes);
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(cfg.block), 0, s, (char *)dst, (const char *)src, nbytes,
+                       cfg.streams);
     NVSHMEMI_HIP(hipGetLastError());
 }
 

@@ -208,6 +208,55 @@ __device__ __forceinline__ void copy_w_chunked(char *dst, const char *src, size_
     if (lo < hi) stream_vectors<UNROLL, NT>((V *)(dst + sp.head), (const V *)(src + sp.head), lo, hi, lane, n);
 }
 
+/* Tiled copy: the vectors are cut into tiles of UNROLL * n (one load per lane
+ * per UNROLL step), and the tiles into `streams` equal runs; workgroup-sized
+ * worker `part` of `parts` takes tiles part, part + parts, ...  in an order
+ * that alternates between the runs (tile t -> run t % streams, position
+ * t / streams).  With one tile per workgroup the hardware dispatches the grid
+ * in tile order, so the device sweeps `streams` address windows at once.
+ * Part 0 also copies the unaligned head and tail bytes. */
+template <int W, int UNROLL, int NT>
+__device__ __forceinline__ void copy_w_tiled(char *dst, const char *src, size_t nbytes, size_t streams, size_t part,
+                                             size_t parts, size_t lane, size_t n) {
+    typedef typename vec<W>::t V;
+    const split<W> sp(dst, nbytes);
+    if (W > 1 && part == 0) {
+        for (size_t i = lane; i < sp.head; i += n) dst[i] = src[i];
+        for (size_t i = lane; i < sp.tail; i += n) dst[sp.tail_start + i] = src[sp.tail_start + i];
+    }
+    V *d = (V *)(dst + sp.head);
+    const V *s = (const V *)(src + sp.head);
+    const size_t tile = (size_t)UNROLL * n;
+    const size_t tiles = (sp.nvec + tile - 1) / tile;
+    const size_t run = (tiles + streams - 1) / streams;
+    for (size_t t = part; t < run * streams; t += parts) {
+        const size_t base = ((t % streams) * run + t / streams) * tile + lane;
+        V r[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u)
+            if (base + u * n < sp.nvec) r[u] = vload<(NT & NT_LOAD) != 0>(s + base + u * n);
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u)
+            if (base + u * n < sp.nvec) vstore<(NT & NT_STORE) != 0>(d + base + u * n, r[u]);
+    }
+}
+
+template <int MAXW, int UNROLL, int NT = NT_NONE>
+__device__ __forceinline__ void copy_tiled(void *dstv, const void *srcv, size_t nbytes, size_t streams, size_t part,
+                                           size_t parts, size_t lane, size_t n) {
+    char *dst = (char *)dstv;
+    const char *src = (const char *)srcv;
+    const uintptr_t rel = (uintptr_t)dst ^ (uintptr_t)src;
+    if (MAXW >= 16 && (rel & 15) == 0)
+        copy_w_tiled<(MAXW >= 16 ? 16 : 1), UNROLL, NT>(dst, src, nbytes, streams, part, parts, lane, n);
+    else if (MAXW >= 8 && (rel & 7) == 0)
+        copy_w_tiled<(MAXW >= 8 ? 8 : 1), UNROLL, NT>(dst, src, nbytes, streams, part, parts, lane, n);
+    else if (MAXW >= 4 && (rel & 3) == 0)
+        copy_w_tiled<(MAXW >= 4 ? 4 : 1), UNROLL, NT>(dst, src, nbytes, streams, part, parts, lane, n);
+    else if ((rel & 1) == 0) copy_w_tiled<2, UNROLL, NT>(dst, src, nbytes, streams, part, parts, lane, n);
+    else copy_w_tiled<1, UNROLL, NT>(dst, src, nbytes, streams, part, parts, lane, n);
+}
+
 /* Pick the widest access width <= MAXW that the relative alignment of dst and
  * src allows, then copy.  Every lane of the cooperating group must call this
  * with identical arguments. */

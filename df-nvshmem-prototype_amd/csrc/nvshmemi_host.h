@@ -93,7 +93,8 @@ struct CopyConfig {
     int vec_bytes;  /* 4, 8 or 16 */
     int unroll;     /* 1, 2, 4 or 8 */
     int nontemporal; /* NT_* bits: 1 loads, 2 stores */
-    int mode;        /* 0 grid-interleaved, 1 contiguous slice per workgroup */
+    int mode;        /* 0 grid-interleaved, 1 contiguous slice per workgroup, 2 tiles over interleaved streams */
+    int streams;     /* mode 2: address streams the tiles alternate between */
 };
 CopyConfig default_copy_config(size_t nbytes);
 void launch_copy(void *dst, const void *src, size_t nbytes, const CopyConfig &cfg, hipStream_t s);

@@ -313,7 +313,7 @@ def make_requests(lib, reqs, sym_base, src_base, dst_base):
 class CopyConfig(C.Structure):
     """nvshmemx_copy_config_t (include/nvshmemx.h)."""
     _fields_ = [("workgroups", _I), ("block", _I), ("vec_bytes", _I), ("unroll", _I), ("nontemporal", _I),
-                ("mode", _I)]
+                ("mode", _I), ("streams", _I)]
 
 
 def load(trace: bool = False, variant: str = "") -> C.CDLL:

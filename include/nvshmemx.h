@@ -77,7 +77,10 @@ typedef struct nvshmemx_copy_config {
     int vec_bytes;   /* widest access: 4, 8 or 16 */
     int unroll;      /* independent accesses in flight per lane: 1, 2, 4 or 8 */
     int nontemporal; /* bit 0: non-temporal loads, bit 1: non-temporal stores */
-    int mode;        /* 0: lanes of the whole grid interleave; 1: one contiguous slice per workgroup */
+    int mode;        /* 0: lanes of the whole grid interleave; 1: one contiguous slice per workgroup;
+                        2: tiles of unroll * block accesses alternating between `streams` address runs
+                           (workgroups 0: one tile per workgroup) */
+    int streams;     /* mode 2: 1..64 address runs */
 } nvshmemx_copy_config_t;
 
 void nvshmemx_putmem_config_on_stream(void *target, const void *source, size_t nbytes, int pe,

@@ -51,7 +51,7 @@ def _check_region(hip, oracle, lib, dst_dev_region, image, nbytes_region):
         raise AssertionError(f"{bad.size} bytes differ, first at {bad[0]} (got {got[bad[0]]}, want {image[bad[0]]})")
 
 
-def _run_put(lib, hip, oracle, *, path, dtype, nelem, toff, soff, pe_src=0, pe_dst=1, seed=1, chunk=0):
+def _run_put(lib, hip, oracle, *, path, dtype, nelem, toff, soff, pe_src=0, pe_dst=1, seed=1, chunk=0, cfg=None):
     """put `nelem` elements of `dtype` from a local buffer (PE pe_src's heap) into
     PE pe_dst's symmetric buffer at byte offset toff; compare with the oracle."""
     w = na.DTYPE_BYTES[dtype]
@@ -68,6 +68,8 @@ def _run_put(lib, hip, oracle, *, path, dtype, nelem, toff, soff, pe_src=0, pe_d
     target = sym + GUARD + toff  # symmetric address
     if path == "grid":
         getattr(lib, f"nvshmemx_{dtype}_put_on_stream")(target, src_local, nelem, pe_dst, None)
+    elif path == "config":
+        lib.nvshmemx_putmem_config_on_stream(target, src_local, nbytes, pe_dst, C.byref(cfg), None)
     elif path == "thread":
         lib.nvshmemh_thread_putget(0, na.DTYPES[dtype], target, src_local, nelem, chunk, pe_dst, None)
     elif path in ("wave", "block"):
@@ -126,6 +128,33 @@ def test_grid_putget_sizes(local_pes, hip, oracle, dtype, op):
         fn = _run_put if op == "put" else _run_get
         fn(lib, hip, oracle, path="grid", dtype=dtype, nelem=nb // w, toff=0, soff=0, seed=_seed(dtype, op, nb))
         nb *= 2
+
+
+@pytest.mark.parametrize("mode,block,unroll,streams,workgroups", [
+    (0, 256, 4, 0, 7), (0, 1024, 8, 0, 0), (1, 512, 8, 0, 13), (1, 64, 1, 0, 0),
+    (2, 512, 1, 2, 0), (2, 256, 2, 1, 0), (2, 1024, 1, 4, 0), (2, 64, 8, 3, 0), (2, 256, 4, 5, 3)])
+def test_copy_configs(local_pes, hip, oracle, mode, block, unroll, streams, workgroups):
+    """every copy mode and launch shape of nvshmemx_putmem_config_on_stream on
+    empty, ragged and relatively misaligned puts (mode 2 = tiles alternating
+    between address runs; workgroups 0 = one tile per workgroup)"""
+    lib = local_pes.start(2, heap="256M")
+    cfg = na.CopyConfig(workgroups, block, 16, unroll, 3, mode, streams)
+    for dtype in ["int", "double"]:
+        w = na.DTYPE_BYTES[dtype]
+        for nelem in [0, 1, 17, 4099, 65536 + 3, (3 << 20) // w + 5]:
+            for toff, soff in [(0, 0), (w, 0), (8, 12) if w == 4 else (8, 0)]:
+                lib.nvshmcleanup()
+                _run_put(lib, hip, oracle, path="config", dtype=dtype, nelem=nelem, toff=toff, soff=soff, cfg=cfg,
+                         seed=_seed("config", mode, block, dtype, nelem, toff, soff))
+
+
+def test_large_put_default_shape(local_pes, hip, oracle):
+    """a 512 MiB + 3-element float put, misaligned by 4 bytes, on the library's
+    default shape for large copies"""
+    lib = local_pes.start(2, heap="1100M")
+    lib.nvshmcleanup()
+    _run_put(lib, hip, oracle, path="grid", dtype="float", nelem=(512 << 20) // 4 + 3, toff=4, soff=0,
+             seed=_seed("large"))
 
 
 @pytest.mark.parametrize("path", ["grid", "thread", "wave", "block"])
