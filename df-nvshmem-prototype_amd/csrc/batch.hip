@@ -299,6 +299,7 @@ extern "C" int nvshmemx_batch_plan_create(const nvshmemx_put_request_t *reqs, in
     for (int i = 0; i < nreq; ++i) {
         const nvshmemx_put_request_t &r = reqs[i];
         if (r.pe < 0 || r.pe >= g.npes) NVSHMEMI_FATAL("request %d: PE %d out of range [0, %d)", i, r.pe, g.npes);
+        check_symmetric(r.target, r.nbytes, "nvshmemx_batch_plan_create");
         rr[(size_t)i] = resolved_req{(const char *)r.source, (char *)nvshmem_ptr(r.target, r.pe), r.nbytes,
                                      owner ? source_pe(r.source) : 0};
     }

@@ -360,10 +360,29 @@ static void check_pe(int pe, const char *fn) {
     if (pe < 0 || pe >= g.npes) NVSHMEMI_FATAL("%s: PE %d out of range [0, %d)", fn, pe, g.npes);
 }
 
+namespace nvshmemi {
+/* A symmetric range must lie inside one PE's heap: in process mode this
+ * process's heap, in local mode any PE's slot of the one allocation.  A
+ * pointer from outside (a private hipMalloc, a host pointer) would otherwise
+ * translate to an address outside every peer's mapping. */
+void check_symmetric(const void *addr, size_t nbytes, const char *fn) {
+    const char *a = (const char *)addr;
+    const char *lo = g.mode == NVSHMEMI_MODE_LOCAL ? g.heap_base[0] : g.heap_base[g.mype];
+    const size_t span = g.mode == NVSHMEMI_MODE_LOCAL ? g.heap_stride * (size_t)g.npes : g.heap_bytes;
+    if (a < lo || (size_t)(a - lo) >= span)
+        NVSHMEMI_FATAL("%s: %p is not a symmetric address (heap %p + %zu bytes)", fn, addr, (const void *)lo, span);
+    const size_t off = g.mode == NVSHMEMI_MODE_LOCAL ? (size_t)(a - lo) & (g.heap_stride - 1) : (size_t)(a - lo);
+    if (nbytes > g.heap_bytes - off)
+        NVSHMEMI_FATAL("%s: %zu bytes at symmetric offset %zu run past the %zu-byte heap", fn, nbytes, off,
+                       g.heap_bytes);
+}
+} // namespace nvshmemi
+
 extern "C" void nvshmemx_putmem_on_stream(void *target, const void *source, size_t nbytes, int pe, hipStream_t s) {
     require_init("nvshmemx_putmem_on_stream");
     check_pe(pe, "nvshmemx_putmem_on_stream");
     if (nbytes == 0) return;
+    check_symmetric(target, nbytes, "nvshmemx_putmem_on_stream");
     launch_copy(nvshmem_ptr(target, pe), source, nbytes, default_copy_config(nbytes), s);
 }
 
@@ -371,6 +390,7 @@ extern "C" void nvshmemx_getmem_on_stream(void *target, const void *source, size
     require_init("nvshmemx_getmem_on_stream");
     check_pe(pe, "nvshmemx_getmem_on_stream");
     if (nbytes == 0) return;
+    check_symmetric(source, nbytes, "nvshmemx_getmem_on_stream");
     launch_copy(target, nvshmem_ptr((void *)source, pe), nbytes, default_copy_config(nbytes), s);
 }
 
@@ -379,6 +399,7 @@ extern "C" void nvshmemx_putmem_config_on_stream(void *target, const void *sourc
     require_init("nvshmemx_putmem_config_on_stream");
     check_pe(pe, "nvshmemx_putmem_config_on_stream");
     if (nbytes == 0) return;
+    check_symmetric(target, nbytes, "nvshmemx_putmem_config_on_stream");
     CopyConfig c = default_copy_config(nbytes);
     if (cfg) {
         if (cfg->workgroups > 0) c.workgroups = cfg->workgroups;

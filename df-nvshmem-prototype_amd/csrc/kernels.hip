@@ -76,7 +76,8 @@ static copy_kernel_t pick_copy_kernel(const CopyConfig &c) {
  * tiled} x block {256,512,1024} x grid x unroll {1,2,4,8} x streams x NT
  * (copies rotated over a 1 GiB footprint so the 256 MB MALL cannot serve
  * them; means of three passes):
- *   >= 512 MiB        tiled, 512 threads x 1 access (8 KiB tiles), 2 runs: 6.5 TB/s at 1 GiB
+ *   >= 512 MiB        tiled, 256 threads x 2 accesses (8 KiB tiles), 1 run: 6.5 TB/s at 1 GiB,
+ *                     also into a peer process's IPC-mapped heap
  *   128 .. 512 MiB    interleaved, 16384 x 512, 8 in flight:             6.4 TB/s at 256 MiB
  *   32 .. 128 MiB     interleaved, 4096 x 512, 8 in flight:              5.9 TB/s at 64 MiB
  *   below             interleaved, grid sized to the copy (<= 4096)
@@ -87,7 +88,7 @@ CopyConfig default_copy_config(size_t nbytes) {
     c.vec_bytes = 16;
     c.nontemporal = NT_BOTH;
     if (nbytes >= ((size_t)512 << 20)) {
-        c.mode = 2, c.block = 512, c.workgroups = 0, c.unroll = 1, c.streams = 2;
+        c.mode = 2, c.block = 256, c.workgroups = 0, c.unroll = 2, c.streams = 1;
         return c;
     }
     if (nbytes >= ((size_t)128 << 20)) {

@@ -97,6 +97,8 @@ struct CopyConfig {
     int streams;     /* mode 2: address streams the tiles alternate between */
 };
 CopyConfig default_copy_config(size_t nbytes);
+/* host_runtime.hip: fatal unless [addr, addr + nbytes) lies in one PE's heap */
+void check_symmetric(const void *addr, size_t nbytes, const char *fn);
 void launch_copy(void *dst, const void *src, size_t nbytes, const CopyConfig &cfg, hipStream_t s);
 
 /* batch.hip: plans over requests whose targets are already resolved to this

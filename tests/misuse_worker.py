@@ -1,0 +1,34 @@
+"""Makes one misuse of the host API that the library must refuse before any
+kernel is launched (argv[1] names the case), then reports "launched" if the
+call returned.  Launched by test_gpu_rma.py in its own process, since a refused
+call ends the process with an error message."""
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT / "df-nvshmem-prototype_amd"))
+
+import nvshmem_amd as na  # noqa: E402
+
+
+def main(case):
+    lib, hip = na.load(), na.Hip()
+    assert lib.nvshmemx_start_local_pes(2, 1) == 0
+    sym = lib.nvshmalloc(1 << 20)
+    private = hip.malloc(1 << 20)
+    print("case", case, flush=True)
+    if case == "get_private_source":
+        lib.nvshmemx_getmem_on_stream(private, private, 4096, 1, None)
+    elif case == "put_private_target":
+        lib.nvshmemx_putmem_on_stream(private, sym, 4096, 1, None)
+    elif case == "put_past_heap_end":
+        lib.nvshmemx_putmem_on_stream(sym, private, int(lib.nvshmemx_heap_size()) + (4 << 20), 1, None)
+    elif case == "symmetric_ok":
+        lib.nvshmemx_putmem_on_stream(sym, private, 4096, 1, None)
+    hip.sync()
+    print("launched", flush=True)
+    lib.nvstop_pes()
+
+
+if __name__ == "__main__":
+    main(sys.argv[1])

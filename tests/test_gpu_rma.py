@@ -7,6 +7,10 @@ destination is the oracle's put/get applied to a host image of the same heap
 region (canary bytes around the target must survive untouched).  Bit-exact.
 """
 import ctypes as C
+import os
+import subprocess
+import sys
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -312,3 +316,19 @@ def test_wait_timeout_is_reported(local_pes, hip, until):
         assert lib.nvshmemx_get_last_error() == 0
     finally:
         lib.nvshmemx_set_wait_timeout_ms(5000)
+
+
+@pytest.mark.parametrize("case", ["get_private_source", "put_private_target", "put_past_heap_end", "symmetric_ok"])
+def test_non_symmetric_ranges_are_refused(case):
+    """host launchers check that the symmetric side of a copy lies in a PE's
+    heap before launching: a private or out-of-heap range ends the process with
+    a message instead of a device access outside every mapping"""
+    env = {k: v for k, v in os.environ.items() if not k.startswith("NVSHMEM_")}
+    env["NVSHMEM_HEAP_SIZE"] = "64M"
+    p = subprocess.run([sys.executable, str(Path(__file__).with_name("misuse_worker.py")), case], env=env,
+                       capture_output=True, text=True, timeout=120)
+    if case == "symmetric_ok":
+        assert p.returncode == 0 and "launched" in p.stdout, p.stderr[-2000:]
+    else:
+        assert p.returncode != 0 and "launched" not in p.stdout
+        assert "symmetric" in p.stderr or "heap" in p.stderr, p.stderr[-2000:]
