@@ -378,6 +378,20 @@ def cpu_baseline():
         return {"value": None, "unit": "GB/s", "cores": 2, "kind": "port", "sample": f"failed: {exc}"}
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per put launch from the committed FETCH_SIZE / WRITE_SIZE
+    profile of this kernel (scaled by a plain copy of known size), or None when
+    the profile was taken of a different kernel."""
+    f = Path(__file__).resolve().parent / "profiles" / "put_1GiB_f32_traffic.json"
+    try:
+        prof = json.loads(f.read_text())
+    except (OSError, ValueError):
+        return None
+    if prof.get("kernel") != kernel or prof.get("algorithmic_bytes") != 2 * NBYTES:
+        return None
+    return int(prof["traffic_bytes"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -412,6 +426,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    import ctypes as C
     import numpy as np
     import nvshmem_amd as na
 
@@ -477,10 +492,14 @@ def main():
             put_ok = peer["pe1_put_bitexact"]
             err |= peer["pe1_errors"]
         rtt_ns = s.astype(np.float64) * 1e6 / khz
+        dcfg = na.CopyConfig()
+        lib.nvshmemx_default_copy_config(NBYTES, C.byref(dcfg))
+        put_kernel = f"k_copy<16, {dcfg.unroll}, {dcfg.nontemporal}, {dcfg.mode}>"
         put_gbps = 2 * NBYTES / (put_ms * 1e-3) / 1e9
         result = dict(
             put_ms=put_ms, put_gbps=put_gbps, get_gbps=2 * NBYTES / (get_ms * 1e-3) / 1e9,
             memcpy_gbps=2 * NBYTES / (memcpy_ms * 1e-3) / 1e9, put_ok=put_ok, get_ok=get_ok, errors=err,
+            put_kernel=put_kernel,
             rtt_p50=float(np.percentile(rtt_ns, 50)), rtt_p99=float(np.percentile(rtt_ns, 99)))
     finally:
         if child is not None and child.poll() is None:
@@ -512,7 +531,8 @@ def main():
                                   else "in one process on one MI355X"),
                    "bytes_per_put": NBYTES, "layout": args.layout, "parallelism": "1 GPU, 2 PEs"},
         "roofline": {"bound": "hbm", "achieved": round(r["put_gbps"], 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                     "frac": round(r["put_gbps"] / PEAK_HBM_GBPS, 4), "traffic": None},
+                     "frac": round(r["put_gbps"] / PEAK_HBM_GBPS, 4), "traffic": profiled_traffic(r["put_kernel"]),
+                     "kernel": r["put_kernel"]},
         "cpu_baseline": cpu,
         "get_gbps": round(r["get_gbps"], 1),
         "hip_memcpy_d2d_gbps": round(r["memcpy_gbps"], 1),

@@ -414,6 +414,18 @@ extern "C" void nvshmemx_putmem_config_on_stream(void *target, const void *sourc
     launch_copy(nvshmem_ptr(target, pe), source, nbytes, c, s);
 }
 
+extern "C" void nvshmemx_default_copy_config(size_t nbytes, nvshmemx_copy_config_t *out) {
+    if (!out) NVSHMEMI_FATAL("nvshmemx_default_copy_config: null output");
+    const CopyConfig c = default_copy_config(nbytes);
+    out->workgroups = c.workgroups;
+    out->block = c.block;
+    out->vec_bytes = c.vec_bytes;
+    out->unroll = c.unroll;
+    out->nontemporal = c.nontemporal;
+    out->mode = c.mode;
+    out->streams = c.streams;
+}
+
 #define NVSHMEMI_DEF_HOST_TYPED(Name, Type)                                                              \
     extern "C" void nvshmemx_##Name##_put_on_stream(Type *target, const Type *source, size_t len, int pe, \
                                                     hipStream_t s) {                                     \

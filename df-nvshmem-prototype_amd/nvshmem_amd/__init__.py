@@ -77,6 +77,7 @@ SIGNATURES = {
     "nvshmemx_float_get_on_stream": (None, [_P, _P, _SZ, _I, _P]),
     "nvshmemx_double_get_on_stream": (None, [_P, _P, _SZ, _I, _P]),
     "nvshmemx_putmem_config_on_stream": (None, [_P, _P, _SZ, _I, _P, _P]),
+    "nvshmemx_default_copy_config": (None, [_SZ, _P]),
     "nvshmemx_barrier_all_on_stream": (None, [_P]),
     "nvshmemx_trace_flush": (None, []),
     "nvshmemx_trace_enabled": (_I, []),

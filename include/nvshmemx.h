@@ -85,6 +85,8 @@ typedef struct nvshmemx_copy_config {
 
 void nvshmemx_putmem_config_on_stream(void *target, const void *source, size_t nbytes, int pe,
                                       const nvshmemx_copy_config_t *cfg, hipStream_t stream);
+/* The launch shape nvshmemx_{put,get}mem_on_stream use for `nbytes`. */
+void nvshmemx_default_copy_config(size_t nbytes, nvshmemx_copy_config_t *out);
 
 /* Stream-ordered device barrier on an explicit stream. */
 void nvshmemx_barrier_all_on_stream(hipStream_t stream);
