@@ -151,6 +151,16 @@ class Oracle:
         which = run(u.ctypes.data, v.ctypes.data, G, steps)
         return (v if which else u).reshape(G, G, G)
 
+    def jacobi_step(self, u):
+        """one periodic Jacobi step of a G^3 float64 field"""
+        u = np.ascontiguousarray(u, np.float64)
+        G = u.shape[0]
+        v = np.empty_like(u)
+        step = self.lib.oracle_jacobi_step
+        step.restype, step.argtypes = None, [_P, _P, _I]
+        step(u.ctypes.data, v.ctypes.data, G)
+        return v
+
     def trace_row(self, *fields):
         buf = C.create_string_buffer(512)
         self.lib.oracle_trace_row(buf, 512, *fields)

@@ -289,3 +289,46 @@ def test_mlp_forward_closed_forms(oracle):
     Y = oracle.mlp_forward(P, X, c)
     want = np.maximum(X[:, :1], 0) * np.arange(1, c + 1) + P[b3:]
     np.testing.assert_allclose(Y, want, rtol=0, atol=1e-15)
+
+
+def test_jacobi_plane_waves_are_eigenvectors(oracle):
+    """cos(2 pi k.x / G) is an eigenvector of the periodic 7-point update with
+    eigenvalue 1/4 + 1/4 (cos 2 pi kx/G + cos 2 pi ky/G + cos 2 pi kz/G): pins the
+    coefficients, the three axes (distinct k per axis) and the periodic wrap"""
+    G = 8
+    x, y, z = np.meshgrid(np.arange(G), np.arange(G), np.arange(G), indexing="ij")
+    for k in [(1, 2, 3), (3, 0, 1), (0, 0, 4), (2, 5, 7)]:
+        u = np.cos(2 * np.pi * (k[0] * x + k[1] * y + k[2] * z) / G)
+        lam = 0.25 + 0.25 * sum(np.cos(2 * np.pi * ki / G) for ki in k)
+        np.testing.assert_allclose(oracle.jacobi_step(u), lam * u, rtol=0, atol=1e-14)
+
+
+def test_jacobi_exact_special_cases(oracle):
+    """a constant (dyadic) field is a fixed point, bit for bit; a unit spike at a
+    corner spreads to exactly 1/4 on itself and 1/8 on its six periodic
+    neighbours and nothing else"""
+    G = 6
+    c = np.full((G, G, G), 0.375)
+    np.testing.assert_array_equal(oracle.jacobi_step(c), c)
+    u = np.zeros((G, G, G))
+    u[0, 0, G - 1] = 1.0
+    want = np.zeros((G, G, G))
+    want[0, 0, G - 1] = 0.25
+    for d in [(1, 0, G - 1), (G - 1, 0, G - 1), (0, 1, G - 1), (0, G - 1, G - 1), (0, 0, 0), (0, 0, G - 2)]:
+        want[d] = 0.125
+    np.testing.assert_array_equal(oracle.jacobi_step(u), want)
+
+
+def test_jacobi_run_conserves_sum_and_stays_in_range(oracle):
+    """the update's weights sum to one, so the periodic sum is conserved (up to
+    rounding) and values stay in the initial [0, 1) range"""
+    G = 12
+    u0 = oracle.jacobi(G, 0, 42)
+    assert u0.min() >= 0.0 and u0.max() < 1.0
+    assert np.all(u0 * 2.0 ** 53 == np.floor(u0 * 2.0 ** 53))
+    u5 = oracle.jacobi(G, 5, 42)
+    assert abs(u5.sum() - u0.sum()) < 1e-12 * u0.sum()
+    assert u5.min() >= u0.min() and u5.max() <= u0.max()
+    # steps compose: 5 steps = 2 steps then 3 more
+    u2 = oracle.jacobi(G, 2, 42)
+    np.testing.assert_array_equal(oracle.jacobi_step(oracle.jacobi_step(oracle.jacobi_step(u2))), u5)
