@@ -85,25 +85,26 @@ size_t balance_cap(size_t batch_bytes) {
     return cap >= 4096 ? cap : 0;
 }
 
-/* Time-equivalent piece bytes of candidate `c` on a request of `nbytes`
- * (a wavefront moves a quarter of a 256-thread workgroup's bytes per unit
- * time, so its pieces count four times). */
-static size_t piece_cost(size_t nbytes, int c) {
+/* Contiguous bytes of one piece of candidate `c` on a request of `nbytes`. */
+static size_t piece_bytes(size_t nbytes, int c) {
     nvshmemx_batch_choice_t ch;
     candidate_of(c, &ch);
-    const size_t parts = std::min<size_t>((size_t)ch.pieces * (ch.mode ? 1 : 4), std::max<size_t>(1, nbytes / 256));
-    const size_t per = ((nbytes + parts - 1) / parts + 255) / 256 * 256;
-    return per * (ch.mode ? 1 : 4);
+    const size_t parts = std::min<size_t>((size_t)ch.pieces * (ch.mode ? 1 : 4), std::max<size_t>(1, nbytes / kPieceQuantum));
+    return ((nbytes + parts - 1) / parts + kPieceQuantum - 1) / kPieceQuantum * kPieceQuantum;
 }
+
+/* Time-equivalent piece bytes (a wavefront moves a quarter of a 256-thread
+ * workgroup's bytes per unit time, so its pieces count four times). */
+static size_t piece_cost(size_t nbytes, int c) { return piece_bytes(nbytes, c) * (c / 27 ? 1 : 4); }
 
 /* Constrained choice over the 54 candidates' costs (times, or log-times when
  * `log_cost`).  Only candidates whose pieces fit under `cap` compete (when
  * none fits: the candidate with the smallest pieces).  Among the competitors
  * whose cost is within kTieTolerance of the cheapest, the one with the
- * largest pieces, then the widest accesses, wins: measured costs carry a few
- * per cent of noise, and fewer pieces and instructions mean fewer descriptors
- * and fewer scattered accesses in the fused launch (the one-standard-error
- * rule of model selection). */
+ * longest contiguous pieces, then the widest accesses, wins: measured costs
+ * carry a few per cent of noise, and longer pieces and wider accesses mean
+ * fewer descriptors, fewer instructions and fewer scattered accesses in the
+ * fused launch (the one-standard-error rule of model selection). */
 constexpr double kTieTolerance = 0.05;
 int pick_candidate(const float *cost, size_t nbytes, size_t cap, bool log_cost) {
     int best = -1, finest = 0;
@@ -116,10 +117,12 @@ int pick_candidate(const float *cost, size_t nbytes, size_t cap, bool log_cost) 
     if (best < 0) return finest;
     const double limit = log_cost ? (double)cost[best] + log1p(kTieTolerance) : (double)cost[best] * (1.0 + kTieTolerance);
     int pick = best;
+    size_t pb_pick = piece_bytes(nbytes, pick);
     for (int c = 0; c < kNumCand; ++c) {
         if ((cap && pc[c] > cap) || (double)cost[c] > limit) continue;
+        const size_t pb = piece_bytes(nbytes, c);
         const int vc = kVec[(c / 9) % 3], vp = kVec[(pick / 9) % 3];
-        if (pc[c] > pc[pick] || (pc[c] == pc[pick] && (vc > vp || (vc == vp && cost[c] < cost[pick])))) pick = c;
+        if (pb > pb_pick || (pb == pb_pick && (vc > vp || (vc == vp && cost[c] < cost[pick])))) pick = c, pb_pick = pb;
     }
     return pick;
 }
@@ -171,7 +174,7 @@ extern "C" int nvshmemx_tune_measure(int npts, const double *log2_bytes, const i
         for (int c = 0; c < kNumCand; ++c) {
             nvshmemx_batch_choice_t ch;
             candidate_of(c, &ch);
-            const size_t parts = std::min<size_t>((size_t)ch.pieces * (ch.mode ? 1 : 4), std::max<size_t>(1, size / 256));
+            const size_t parts = std::min<size_t>((size_t)ch.pieces * (ch.mode ? 1 : 4), std::max<size_t>(1, size / kPieceQuantum));
             const size_t k = std::max<size_t>(1, std::min(k_full, kMaxPieces / parts));
             reqs.resize(k);
             for (size_t i = 0; i < k; ++i) reqs[i] = resolved_req{src + i * stride, dst + i * stride + doff, size, 0};

@@ -5,7 +5,7 @@
  * (nvshmem_device.cu:487-512).  A batch of put requests is turned into a plan:
  *
  *  - each request is split into pieces by its choice (pieces x vector width x
- *    wavefront/workgroup mode); piece boundaries fall on 256-byte offsets of the
+ *    wavefront/workgroup mode); piece boundaries fall on 4 KiB offsets of the
  *    request, so every piece keeps the request's relative alignment;
  *  - the pieces are assigned to workers on the host, longest first, each to the
  *    currently least-loaded worker (LPT list scheduling): workgroup pieces to
@@ -48,11 +48,8 @@ struct piece_t {
 };
 
 /* per-piece charge in the LPT cost, in bytes: about one access latency of a
- * wavefront streaming 4 x 1 KiB (NVSHMEM_BATCH_PIECE_CHARGE overrides it) */
-static double piece_charge() {
-    const char *e = getenv("NVSHMEM_BATCH_PIECE_CHARGE");
-    return e && *e ? atof(e) : 4096.0;
-}
+ * wavefront streaming 4 x 1 KiB (0 to 64 KiB measured the same) */
+constexpr double kPieceCharge = 4096.0;
 
 template <int NT>
 __device__ __forceinline__ void copy_piece(const piece_t &p, unsigned lane, unsigned n) {
@@ -75,11 +72,11 @@ __global__ void __launch_bounds__(1024) k_batch(const piece_t *__restrict__ piec
     for (unsigned int j = wave_off[w]; j < wave_end; ++j) copy_piece<NT>(pieces[j], threadIdx.x & 63u, 64);
 }
 
-/* one request's pieces under a choice; boundaries on 256-byte offsets */
+/* one request's pieces under a choice; boundaries on kPieceQuantum offsets */
 static void split_request(const char *src, char *dst, size_t nbytes, const nvshmemx_batch_choice_t &ch,
                           std::vector<piece_t> &out) {
     if (nbytes == 0) return;
-    const size_t quantum = 256;
+    const size_t quantum = kPieceQuantum;
     size_t want = (size_t)std::max(1, std::min(ch.pieces, 256));
     if (ch.mode == 0) want *= 4; /* a workgroup's worth of wavefronts per piece */
     const size_t max_parts = std::max<size_t>(1, nbytes / quantum);
@@ -104,7 +101,7 @@ static std::vector<std::vector<piece_t>> lpt(std::vector<piece_t> pieces, int wo
     typedef std::pair<double, int> item;
     std::priority_queue<item, std::vector<item>, std::greater<item>> heap;
     for (int w = 0; w < workers; ++w) heap.push(item(load[(size_t)w], w));
-    const double charge = piece_charge();
+    const double charge = kPieceCharge;
     for (const piece_t &p : pieces) {
         item top = heap.top();
         heap.pop();
