@@ -63,12 +63,16 @@ def worker(name):
     lib.nvshmem_barrier_all()                       # 3: start ping-pong
     lib.nvshmemh_pingpong_process(flag, 0, 0, PINGPONG_ITERS, None, 0, None, None)
     hip.sync()
+    lib.nvshmem_barrier_all()                       # 3b: barrier episodes
+    lib.nvshmemh_barrier_latency(BARRIER_ITERS, None, None)
+    hip.sync()
     err = lib.nvshmemx_get_last_error()
     lib.nvshmem_barrier_all()                       # 4: done
     lib.nvstop_pes()
     print(json.dumps({"pe1_put_bitexact": ok, "pe1_errors": err}), flush=True)
 
 
+BARRIER_ITERS, BARRIER_SKIP = 10000, 10     # barrier episodes timed on the device; the first few absorb launch skew
 BATCH_NPES, BATCH_WGS, BATCH_SRC, BATCH_DST, BATCH_REQS, BATCH_SEED = 64, 4, 32 << 20, 64 << 20, 4096, 7
 
 
@@ -482,6 +486,11 @@ def main():
             lib.nvshmemh_pingpong_local(flag, 0, 1, PINGPONG_ITERS, None, 0, samples, None)
         hip.sync()
         s = hip.dtoh(samples, PINGPONG_ITERS * 8, np.int64)
+        lib.nvshmem_barrier_all()                                        # 3b
+        bsamples = hip.malloc(BARRIER_ITERS * 8)
+        lib.nvshmemh_barrier_latency(BARRIER_ITERS, bsamples, None)
+        hip.sync()
+        bs = hip.dtoh(bsamples, BARRIER_ITERS * 8, np.int64)[BARRIER_SKIP:]
         khz = lib.nvshmemx_wallclock_khz()
         err = lib.nvshmemx_get_last_error()
         lib.nvshmem_barrier_all()                                        # 4
@@ -492,6 +501,7 @@ def main():
             put_ok = peer["pe1_put_bitexact"]
             err |= peer["pe1_errors"]
         rtt_ns = s.astype(np.float64) * 1e6 / khz
+        bar_ns = bs.astype(np.float64) * 1e6 / khz
         dcfg = na.CopyConfig()
         lib.nvshmemx_default_copy_config(NBYTES, C.byref(dcfg))
         put_kernel = f"k_copy<16, {dcfg.unroll}, {dcfg.nontemporal}, {dcfg.mode}>"
@@ -500,7 +510,8 @@ def main():
             put_ms=put_ms, put_gbps=put_gbps, get_gbps=2 * NBYTES / (get_ms * 1e-3) / 1e9,
             memcpy_gbps=2 * NBYTES / (memcpy_ms * 1e-3) / 1e9, put_ok=put_ok, get_ok=get_ok, errors=err,
             put_kernel=put_kernel,
-            rtt_p50=float(np.percentile(rtt_ns, 50)), rtt_p99=float(np.percentile(rtt_ns, 99)))
+            rtt_p50=float(np.percentile(rtt_ns, 50)), rtt_p99=float(np.percentile(rtt_ns, 99)),
+            bar_p50=float(np.percentile(bar_ns, 50)), bar_p99=float(np.percentile(bar_ns, 99)))
     finally:
         if child is not None and child.poll() is None:
             child.kill()
@@ -539,6 +550,9 @@ def main():
         "p_wait_rtt_ns": {"p50": round(r["rtt_p50"], 1), "p99": round(r["rtt_p99"], 1),
                           "iters": PINGPONG_ITERS, "between": "2 processes" if args.layout == "process" else
                           "2 workgroup PEs"},
+        "barrier_ns": {"p50": round(r["bar_p50"], 1), "p99": round(r["bar_p99"], 1),
+                       "episodes": BARRIER_ITERS - BARRIER_SKIP,
+                       "between": "2 processes" if args.layout == "process" else "2 workgroup PEs"},
         "bitexact": {"put": r["put_ok"], "get": r["get_ok"]},
         "device_errors": r["errors"],
         "batched": batched,

@@ -189,6 +189,24 @@ __global__ void k_pingpong_process(int *flag_sym, int me, int peer, int initiato
     pingpong_body(flag_sym, me, peer, initiator, iters, payload_sym, payload_len, samples);
 }
 
+/* Barrier latency: `iters` episodes of the offload barrier's protocol (the
+ * counter and generation words in PE 0's internal region), one lane per
+ * workgroup, each workgroup one arrival: launch one workgroup per process in
+ * process mode, one per PE in local mode.  samples[i] = wall-clock ticks of
+ * episode i as seen by workgroup 0 of this launch. */
+__global__ void k_barrier_latency(int iters, long long *samples) {
+    if (threadIdx.x != 0) return;
+    char *root = nvshmemi_dstate.heap_base[0];
+    int *count = (int *)(root + NVSHMEMI_OFF_BAR_COUNT);
+    int *gen = (int *)(root + NVSHMEMI_OFF_BAR_SENSE);
+    for (int i = 0; i < iters; ++i) {
+        if (failed()) break;
+        const long long t0 = (long long)wall_clock64();
+        nvshmemi::gen_barrier(count, gen, nvshmemi_dstate.npes, NVSHMEMI_ERR_BARRIER_TIMEOUT);
+        if (blockIdx.x == 0 && samples) samples[i] = (long long)wall_clock64() - t0;
+    }
+}
+
 /* local mode: workgroup 0 is PE a (initiator), workgroup 1 is PE b */
 __global__ void k_pingpong_local(int *flag_sym, int pe_a, int pe_b, int iters, int *payload_sym, int payload_len,
                                  long long *samples) {
@@ -473,6 +491,19 @@ void nvshmemh_pingpong_process(int *flag_sym, int peer, int initiator, int iters
                                long long *samples, hipStream_t s) {
     hipLaunchKernelGGL(k_pingpong_process, dim3(1), dim3(64), 0, s, flag_sym, nvshmem_my_pe(), peer, initiator, iters,
                        payload_sym, payload_len, samples);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+void nvshmemh_barrier_latency(int iters, long long *samples, hipStream_t s) {
+    if (nvshmemx_mode() == NVSHMEMX_MODE_LOCAL) {
+        /* every arrival must be resident at once: one workgroup per local PE */
+        void *args[] = {&iters, &samples};
+        nvshmemx_collective_launch((const void *)k_barrier_latency, dim3(nvshmem_n_pes()), dim3(64), args, 0, s);
+        return;
+    }
+    /* one arrival per process, a plain launch (cooperative launches are
+     * exclusive to one process of the device at a time) */
+    hipLaunchKernelGGL(k_barrier_latency, dim3(1), dim3(64), 0, s, iters, samples);
     NVSHMEMI_HIP(hipGetLastError());
 }
 

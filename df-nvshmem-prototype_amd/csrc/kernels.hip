@@ -150,25 +150,7 @@ void launch_copy(void *dst, const void *src, size_t nbytes, const CopyConfig &cf
 
 __global__ void k_offload_barrier(int *count, int *gen, int npes) {
     if (threadIdx.x != 0) return;
-    const int g0 = load_acquire(gen);
-    const int old = __hip_atomic_fetch_add(count, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == npes - 1) {
-        store_relaxed(count, 0);
-        store_release(gen, g0 + 1);
-        return;
-    }
-    const uint64_t budget = nvshmemi_dstate.wait_timeout_ticks;
-    const uint64_t t0 = wall_clock64();
-    int backoff = 1;
-    while (load_relaxed(gen) == g0) {
-        for (int i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(1);
-        if (backoff < 8) backoff <<= 1;
-        if ((uint64_t)(wall_clock64() - t0) > budget) {
-            set_error(NVSHMEMI_ERR_BARRIER_TIMEOUT);
-            break;
-        }
-    }
-    fence_acquire();
+    gen_barrier(count, gen, npes, NVSHMEMI_ERR_BARRIER_TIMEOUT);
 }
 
 void launch_offload_barrier(hipStream_t s) {

@@ -121,6 +121,35 @@ __device__ __forceinline__ int wait_until(int *ivar, int op, int ref) {
     return v;
 }
 
+/* One episode of the generation barrier on (count, gen) among `n` arrivals,
+ * called by one lane per arrival: read the generation (it cannot advance
+ * before this arrival), arrive with an agent-scope fetch-add; the last arriver
+ * resets the count and advances the generation with release, the others poll
+ * the generation with back-off until it moves or the wait budget runs out
+ * (then `timeout_bit` is set and the episode is abandoned).  Ends with an
+ * acquire, so what every arrival published before arriving is visible. */
+__device__ __forceinline__ void gen_barrier(int *count, int *gen, int n, int timeout_bit) {
+    const int g0 = load_acquire(gen);
+    const int old = __hip_atomic_fetch_add(count, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == n - 1) {
+        store_relaxed(count, 0);
+        store_release(gen, g0 + 1);
+        return;
+    }
+    const uint64_t budget = nvshmemi_dstate.wait_timeout_ticks;
+    const uint64_t t0 = wall_clock64();
+    int backoff = 1;
+    while (load_relaxed(gen) == g0) {
+        for (int i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(1);
+        if (backoff < 8) backoff <<= 1;
+        if ((uint64_t)(wall_clock64() - t0) > budget) {
+            set_error(timeout_bit);
+            break;
+        }
+    }
+    fence_acquire();
+}
+
 /* ---------------------------------------------------------- bulk copies -- */
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));

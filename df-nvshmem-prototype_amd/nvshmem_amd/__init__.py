@@ -88,6 +88,7 @@ SIGNATURES = {
     "nvshmemh_coop_putget": (None, [_I, _I, _P, _P, _SZ, _SZ, _I, _I, _P]),
     "nvshmemh_pg": (None, [_I, _I, _P, _P, _P, _I, _I, _P]),
     "nvshmemh_pingpong_process": (None, [_P, _I, _I, _I, _P, _I, _P, _P]),
+    "nvshmemh_barrier_latency": (None, [_I, _P, _P]),
     "nvshmemh_pingpong_local": (None, [_P, _I, _I, _I, _P, _I, _P, _P]),
     "nvshmemh_order_local": (None, [_P, _P, _P, _P, _I, _I, _I, _I, _P, _P]),
     "nvshmemh_wg_barrier_check": (None, [_P, _I, _P, _I, _P]),

@@ -332,3 +332,23 @@ def test_non_symmetric_ranges_are_refused(case):
     else:
         assert p.returncode != 0 and "launched" not in p.stdout
         assert "symmetric" in p.stderr or "heap" in p.stderr, p.stderr[-2000:]
+
+
+@pytest.mark.parametrize("npes", [2, 8])
+def test_barrier_latency_episodes(local_pes, hip, npes):
+    """the offload barrier's protocol run back to back by one lane per PE:
+    every episode completes (no time-out), each takes a positive time"""
+    lib = local_pes.start(npes)
+    iters = 2000
+    samples = hip.malloc(iters * 8)
+    try:
+        lib.nvshmemh_barrier_latency(iters, samples, None)
+        hip.sync()
+        ticks = hip.dtoh(samples, iters * 8, np.int64)
+    finally:
+        hip.free(samples)
+    assert lib.nvshmemx_get_last_error() == 0
+    assert (ticks > 0).all()
+    ns = ticks * 1e6 / lib.nvshmemx_wallclock_khz()
+    print(f"{npes} PEs: barrier p50 {np.percentile(ns, 50):.0f} ns, p99 {np.percentile(ns, 99):.0f} ns")
+    assert np.percentile(ns, 50) < 100_000
