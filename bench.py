@@ -315,6 +315,80 @@ def run_jacobi_leg():
     return out
 
 
+SWEEP_REPS = 100
+
+
+def sweep_worker():
+    """{int, float, double} x {put, get} x 2^3..2^30 bytes, 2 PEs as 2
+    processes; PE 0 times every launch (HIP events, 100 per point) and checks
+    every point bit-exact on the device; PE 1 owns the target heap.  Sources
+    are PCG32 words seeded per (PE, dtype, size).  Prints one JSON line (PE 0)."""
+    import ctypes as C
+    import numpy as np
+    import nvshmem_amd as na
+    lib, hip = na.load(), na.Hip()
+    lib.nvstart_pes()
+    me = lib.nvshmem_my_pe()
+    top = 1 << 30
+    dst = lib.nvshmalloc(top)
+    lib.nvshmem_barrier_all()
+    if me == 0:
+        src, back = hip.malloc(top), hip.malloc(top)
+        e0, e1 = hip.event(), hip.event()
+        out = {"reps": SWEEP_REPS, "bitexact": True, "points": {}}
+        for di, (dtype, w) in enumerate((("int", 4), ("float", 4), ("double", 8))):
+            put = getattr(lib, f"nvshmemx_{dtype}_put_on_stream")
+            get = getattr(lib, f"nvshmemx_{dtype}_get_on_stream")
+            for op in ("put", "get"):
+                out["points"][f"{op}_{dtype}"] = []
+            for lg in range(3, 31):
+                size = 1 << lg
+                n = size // w
+                lib.nvshmemh_fill_pcg32(src, (size + 3) // 4, 1000003 * me + 1009 * di + lg, 54, None)
+                for op in ("put", "get"):
+                    fn = (lambda: put(dst, src, n, 1, None)) if op == "put" else (lambda: get(back, dst, n, 1, None))
+                    fn()
+                    t = []
+                    for _ in range(SWEEP_REPS):
+                        hip.record(e0, None)
+                        fn()
+                        hip.record(e1, None)
+                        hip.sync()
+                        t.append(hip.elapsed_ms(e0, e1))
+                    req = (na.PutRequest * 1)()
+                    req[0].target, req[0].source, req[0].nbytes, req[0].pe, req[0].elem_bytes = (
+                        dst, src if op == "put" else back, size, 1, w)
+                    ok = int(lib.nvshmemh_verify_requests(req, 1)) == 0
+                    out["bitexact"] = out["bitexact"] and ok
+                    t = np.array(t)
+                    out["points"][f"{op}_{dtype}"].append(
+                        [size, round(2 * size / (t.mean() * 1e-3) / 1e9, 2), round(float(np.percentile(t, 50)) * 1e3, 2),
+                         round(float(np.percentile(t, 99)) * 1e3, 2)])
+        out["errors"] = lib.nvshmemx_get_last_error()
+    lib.nvshmem_barrier_all()
+    lib.nvstop_pes()
+    if me == 0:
+        print(json.dumps(out), flush=True)
+
+
+def run_sweep_leg():
+    import nvshmem_amd as na
+    name = f"nvshmem_sweep_{os.getpid()}_{uuid.uuid4().hex[:8]}"
+    try:
+        procs = [subprocess.Popen([sys.executable, __file__, "--sweep-worker"],
+                                  env=na.shm_job_env(r, 2, name, {"NVSHMEM_HEAP_SIZE": HEAP}), stdout=subprocess.PIPE,
+                                  stderr=subprocess.PIPE, text=True) for r in range(2)]
+        outs = [pr.communicate(timeout=600) for pr in procs]
+        lines = [l for o, _ in outs for l in o.splitlines() if l.startswith("{")]
+        return json.loads(lines[-1]) if lines else {"error": outs[0][1][-800:]}
+    except Exception as exc:
+        return {"error": str(exc)[-500:]}
+    finally:
+        shm = Path("/dev/shm") / name
+        if shm.exists():
+            shm.unlink()
+
+
 def run_learned_leg(n_batches, train_steps):
     env = {k: v for k, v in os.environ.items() if not k.startswith("NVSHMEM_")}
     try:
@@ -412,6 +486,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--jacobi-worker", choices=["local", "process"], default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--sweep-worker", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pe_worker:
         worker(args.pe_worker)
@@ -421,6 +497,9 @@ def main():
         return
     if args.learned_worker:
         learned_worker(args.learned_batches, args.train_steps)
+        return
+    if args.sweep_worker:
+        sweep_worker()
         return
     if args.jacobi_worker:
         jacobi_worker(args.jacobi_worker)
@@ -522,6 +601,7 @@ def main():
     batched = None if args.no_batch else run_batch_leg()
     learned = None if args.no_learned else run_learned_leg(args.learned_batches, args.train_steps)
     jacobi = None if args.no_jacobi else run_jacobi_leg()
+    sweep = None if args.no_sweep else run_sweep_leg()
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     r = result
     line = {
@@ -558,6 +638,7 @@ def main():
         "batched": batched,
         "learned": learned,
         "jacobi": jacobi,
+        "sweep": sweep,
     }
     print(json.dumps(line), flush=True)
     if not (r["put_ok"] and r["get_ok"]) or r["errors"]:
