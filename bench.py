@@ -438,7 +438,9 @@ def timed(hip, fn, steps, warmup):
 
 def cpu_baseline():
     """Same put semantics as host MPI-3 RMA (oracle/mpi_rma_baseline.c), 2 ranks
-    on 2 host cores, bounded to ~10 s: 1 GiB float MPI_Put + flush, repeated."""
+    on 2 host cores: 1 GiB float MPI_Put + flush repeated for ~10 s (the
+    reported value), then the p+wait round trip and put/get of 8 B..1 GiB
+    (100 operations or 0.15 s per size), ~10 s more."""
     exe = ROOT / "oracle" / "build" / "rma_baseline"
     mpirun = Path("/opt/conda/bin/mpirun")
     if not exe.exists() or not mpirun.exists():
@@ -446,14 +448,30 @@ def cpu_baseline():
                 "sample": f"unavailable ({exe.name} or mpirun missing)"}
     cmd = [str(mpirun), "-n", "2", "-bind-to", "core", str(exe), "putget", "put", "float", str(NBYTES), str(NBYTES),
            "1000", "10"]
+    def run(args, timeout=240):
+        out = subprocess.run([str(mpirun), "-n", "2", "-bind-to", "core", str(exe)] + args, capture_output=True,
+                             text=True, timeout=timeout, cwd=ROOT)
+        return [json.loads(l) for l in out.stdout.splitlines() if l.startswith("{")]
+
     try:
-        out = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd=ROOT)
-        rec = [json.loads(l) for l in out.stdout.splitlines() if l.startswith("{")][-1]
-        return {"value": round(rec["gbps"], 2), "unit": "GB/s", "cores": 2, "kind": "port",
-                "sample": f"{rec['reps']} x 1 GiB float MPI_Put+MPI_Win_flush over a shared window, 2 ranks "
-                          f"(bit-exact: {rec['bitexact']})"}
+        rec = run(cmd[6:])[-1]
+        res = {"value": round(rec["gbps"], 2), "unit": "GB/s", "cores": 2, "kind": "port",
+               "sample": f"{rec['reps']} x 1 GiB float MPI_Put+MPI_Win_flush over a shared window, 2 ranks "
+                         f"(bit-exact: {rec['bitexact']})"}
     except Exception as exc:  # report, never fail the GPU measurement
         return {"value": None, "unit": "GB/s", "cores": 2, "kind": "port", "sample": f"failed: {exc}"}
+    try:  # the same semantics at the other sizes and the p+wait round trip, ~10 s more
+        pp = run(["pingpong", str(PINGPONG_ITERS)])[-1]
+        res["pingpong_rtt_ns"] = {"p50": pp["rtt_p50_ns"], "p99": pp["rtt_p99_ns"], "iters": pp["iters"]}
+        res["sweep"] = {}
+        for op in ("put", "get"):
+            recs = run(["putget", op, "float", "8", str(NBYTES), "100", "0.15"])
+            res["sweep"][f"{op}_float"] = [[r["bytes"], round(r["gbps"], 3), round(r["p50_ns"] / 1e3, 2),
+                                            round(r["p99_ns"] / 1e3, 2)] for r in recs]
+            res["sweep"][f"{op}_float_bitexact"] = all(r["bitexact"] for r in recs)
+    except Exception as exc:
+        res["sweep_error"] = str(exc)[-300:]
+    return res
 
 
 def profiled_traffic(kernel):
