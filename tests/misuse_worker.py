@@ -1,6 +1,6 @@
 """Makes one misuse of the host API that the library must refuse before any
-kernel is launched (argv[1] names the case), then reports "launched" if the
-call returned.  Launched by test_gpu_rma.py in its own process, since a refused
+kernel is launched or any state is touched (argv[1] names the case), then
+reports "launched" if the call returned.  Launched by test_gpu_rma.py in its own process, since a refused
 call ends the process with an error message."""
 import sys
 from pathlib import Path
@@ -13,7 +13,15 @@ import nvshmem_amd as na  # noqa: E402
 
 def main(case):
     lib, hip = na.load(), na.Hip()
+    if case == "too_many_pes":
+        lib.nvshmemx_start_local_pes(129, 1)
+        print("launched", flush=True)
+        return
     assert lib.nvshmemx_start_local_pes(2, 1) == 0
+    if case == "heap_exhausted":
+        lib.nvshmalloc(int(lib.nvshmemx_heap_size()) + 1)
+        print("launched", flush=True)
+        return
     sym = lib.nvshmalloc(1 << 20)
     private = hip.malloc(1 << 20)
     print("case", case, flush=True)

@@ -2,9 +2,11 @@
 exports every host entry point declared in include/*.h, the static library
 carries them too, and a caller written against the reference's headers
 compiles and links with relocatable device code."""
+import os
 import re
 import shutil
 import subprocess
+import sys
 from pathlib import Path
 
 import pytest
@@ -87,3 +89,27 @@ int main() {
     p = subprocess.run(cmd, capture_output=True, text=True)
     assert p.returncode == 0, p.stderr[-3000:]
     assert exe.exists()
+
+
+_BEFORE_INIT = {
+    "nvshmalloc": "lib.nvshmalloc(64)",
+    "nvshmem_ptr": "lib.nvshmem_ptr(None, 0)",
+    "nvshmem_barrier_all": "lib.nvshmem_barrier_all()",
+    "nvshmem_barrier_all_offload": "lib.nvshmem_barrier_all_offload()",
+    "nvshmemx_putmem_on_stream": "lib.nvshmemx_putmem_on_stream(None, None, 4, 0, None)",
+    "nvshmemx_batch_plan_create": "lib.nvshmemx_batch_plan_create(None, 0, None, None, None)",
+    "nvstop_pes": "lib.nvstop_pes()",
+}
+
+
+@pytest.mark.parametrize("fn", sorted(_BEFORE_INIT))
+def test_calls_before_init_are_refused(fn):
+    """library calls before nvstart_pes end the process with the reference's
+    error convention ([file:line] [pe] message, non-zero exit) instead of
+    touching uninitialised state; no GPU is needed to reach the check"""
+    code = ("import sys; sys.path.insert(0, %r); import nvshmem_amd as na; lib = na.load(); %s; print('returned')"
+            % (str(ROOT / "df-nvshmem-prototype_amd"), _BEFORE_INIT[fn]))
+    env = {k: v for k, v in os.environ.items() if not k.startswith("NVSHMEM_")}
+    p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120, env=env)
+    assert p.returncode != 0 and "returned" not in p.stdout
+    assert f"{fn} called before nvstart_pes" in p.stderr, p.stderr[-1000:]
