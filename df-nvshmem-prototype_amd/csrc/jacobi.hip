@@ -134,13 +134,21 @@ __global__ void k_unpack(double *u_sym, const double *recv_sym, int n, int pe0, 
     u[g.idx(i, j, k)] = r[((size_t)parity * 6 + (size_t)s) * (size_t)n * n + cell];
 }
 
-/* thread (k, j) walks planes i in [i0, i0 + chunk) keeping x-1, x, x+1 in registers */
 /* Workgroup (k tile, j tile x i chunk, PE): walks planes i0..i1 of a
  * kTJ x kTK tile.  Each plane (tile plus a one-cell ring in j and k) is read
- * from memory once into a double-buffered LDS ring; plane i+1 is fetched
- * while plane i is consumed.  Thread (wave w, lane l) owns column k0 + l of
- * rows j0 + 4w .. j0 + 4w + 3 and keeps their x-1 values in registers. */
-__global__ void __launch_bounds__(256) k_stencil(const double *u_sym, double *v_sym, int n, int pe0, int chunk) {
+ * from memory once into a three-plane LDS ring; plane i+2 is fetched into
+ * registers while plane i is consumed.  Thread (wave w, lane l) owns column
+ * k0 + l of rows j0 + 4w .. j0 + 4w + 3 and keeps their x-1 values in
+ * registers.
+ *
+ * FUSED: the halo is not in u but in this PE's receive slots of parity
+ * `pin` (a halo cell is read straight from the slot of its face), and every
+ * boundary value of v is also written into the receive slot of parity `pout`
+ * of the neighbour across that face -- the next step's halo exchange rides on
+ * this step's stores, so a step is one kernel plus the barrier. */
+template <bool FUSED>
+__global__ void __launch_bounds__(256) k_stencil_t(const double *u_sym, double *v_sym, double *recv_sym, int n, int pe0,
+                                                   int chunk, int pin, int pout) {
     const geom g(n, pe0 + (int)blockIdx.z);
     const int jtiles = (n + kTJ - 1) / kTJ;
     const int k0 = (int)blockIdx.x * kTK + 1;
@@ -149,7 +157,8 @@ __global__ void __launch_bounds__(256) k_stencil(const double *u_sym, double *v_
     const int i1 = i0 + chunk - 1 < n ? i0 + chunk - 1 : n;
     const double *__restrict__ u = (const double *)nvshmemx_ptr_d(u_sym, g.pe);
     double *__restrict__ v = (double *)nvshmemx_ptr_d(v_sym, g.pe);
-    const size_t sy = g.row_stride(), sx = (size_t)(n + 2) * sy;
+    const double *__restrict__ rin = FUSED ? (const double *)nvshmemx_ptr_d(recv_sym, g.pe) : nullptr;
+    const size_t sy = g.row_stride(), sx = (size_t)(n + 2) * sy, nn = (size_t)n * n;
     __shared__ double tile[3][kTJ + 2][kTK + 2];
     const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
     const int kcol = k0 + lane;
@@ -161,18 +170,36 @@ __global__ void __launch_bounds__(256) k_stencil(const double *u_sym, double *v_
     const int hr = (int)(threadIdx.x >> 1), hhi = (int)(threadIdx.x & 1u);
     const int hk = hhi ? k0 + kTK : k0 - 1;
 
+    /* cell (i, j, k) of the (n+2)^3 block, 0 <= i, j, k <= n+1 */
+    auto at = [&](int i, int j, int k) -> double {
+        if constexpr (FUSED) {
+            const bool hx = i == 0 || i == n + 1, hy = j == 0 || j == n + 1, hz = k == 0 || k == n + 1;
+            if (hx || hy || hz) {
+                if ((int)hx + (int)hy + (int)hz > 1) return 0.0; /* edges and corners: never used */
+                if (hx) return rin[((size_t)pin * 6 + (i ? 1 : 0)) * nn + (size_t)(j - 1) * n + (k - 1)];
+                if (hy) return rin[((size_t)pin * 6 + (j ? 3 : 2)) * nn + (size_t)(i - 1) * n + (k - 1)];
+                return rin[((size_t)pin * 6 + (k ? 5 : 4)) * nn + (size_t)(i - 1) * n + (j - 1)];
+            }
+        }
+        return u[(size_t)i * sx + (size_t)j * sy + (size_t)(k + kKOff)];
+    };
+    /* value of interior boundary cell (i, j, k) into the neighbour across face d */
+    auto send = [&](int d, size_t idx, double val) {
+        double *dst = (double *)nvshmemx_ptr_d(recv_sym, g.neighbour(d));
+        dst[((size_t)pout * 6 + (size_t)(d ^ 1)) * nn + idx] = val;
+    };
+
     /* plane i of the tile into registers (issued early), then into LDS */
     double pre[NR], preh = 0.0;
     auto issue = [&](int i) {
-        const size_t pb = (size_t)i * sx + kKOff;
 #pragma unroll
         for (int rr = 0; rr < NR; ++rr) {
             const int r = wv + 4 * rr, j = j0 - 1 + r;
-            pre[rr] = (r < kTJ + 2 && j <= jlim && kcol <= klim) ? u[pb + (size_t)j * sy + kcol] : 0.0;
+            pre[rr] = (r < kTJ + 2 && j <= jlim && kcol <= klim) ? at(i, j, kcol) : 0.0;
         }
         if (has_halo) {
             const int j = j0 - 1 + hr;
-            preh = (j <= jlim && hk <= n + 1) ? u[pb + (size_t)j * sy + hk] : 0.0;
+            preh = (j <= jlim && hk <= n + 1) ? at(i, j, hk) : 0.0;
         }
     };
     auto commit = [&](int buf) {
@@ -190,7 +217,7 @@ __global__ void __launch_bounds__(256) k_stencil(const double *u_sym, double *v_
     for (int q = 0; q < RPW; ++q) {
         const int j = j0 + wv * RPW + q;
         ok[q] = kcol <= n && j <= n;
-        xm[q] = ok[q] ? u[(size_t)(i0 - 1) * sx + (size_t)j * sy + kcol + kKOff] : 0.0;
+        xm[q] = ok[q] ? at(i0 - 1, j, kcol) : 0.0;
     }
     issue(i0);
     commit(0);
@@ -210,7 +237,19 @@ __global__ void __launch_bounds__(256) k_stencil(const double *u_sym, double *v_
             const double s = ((((xm[q] + xp) + tile[bc][r - 1][lane + 1]) + tile[bc][r + 1][lane + 1]) +
                               tile[bc][r][lane]) +
                              tile[bc][r][lane + 2];
-            if (ok[q]) v[vb + (size_t)(j0 + r - 1) * sy] = 0.25 * x0 + 0.125 * s;
+            const double val = 0.25 * x0 + 0.125 * s;
+            const int j = j0 + r - 1;
+            if (ok[q]) {
+                v[vb + (size_t)j * sy] = val;
+                if constexpr (FUSED) {
+                    if (i == 1) send(0, (size_t)(j - 1) * n + (kcol - 1), val);
+                    if (i == n) send(1, (size_t)(j - 1) * n + (kcol - 1), val);
+                    if (j == 1) send(2, (size_t)(i - 1) * n + (kcol - 1), val);
+                    if (j == n) send(3, (size_t)(i - 1) * n + (kcol - 1), val);
+                    if (kcol == 1) send(4, (size_t)(i - 1) * n + (j - 1), val);
+                    if (kcol == n) send(5, (size_t)(i - 1) * n + (j - 1), val);
+                }
+            }
             xm[q] = x0;
         }
         if (fetch) commit(bf);
@@ -288,27 +327,43 @@ void nvshmemh_jacobi_unpack(double *u_sym, double *recv_sym, int n, int pe0, int
 void nvshmemh_jacobi_stencil(double *u_sym, double *v_sym, int n, int pe0, int npes_here, int chunk, hipStream_t s) {
     if (chunk <= 0) chunk = n;
     const int jt = (n + kTJ - 1) / kTJ, ic = (n + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_stencil, dim3((unsigned)((n + kTK - 1) / kTK), (unsigned)(jt * ic), npes_here), dim3(256), 0,
-                       s, u_sym, v_sym, n, pe0, chunk);
+    hipLaunchKernelGGL(k_stencil_t<false>, dim3((unsigned)((n + kTK - 1) / kTK), (unsigned)(jt * ic), npes_here),
+                       dim3(256), 0, s, u_sym, v_sym, (double *)nullptr, n, pe0, chunk, 0, 0);
     NVSHMEMI_HIP(hipGetLastError());
 }
 
-/* `steps` full steps; returns the buffer holding the result (u or v).  In
- * process mode each step's barrier is nvshmem_barrier_all_offload; in local
- * mode the launches for all PEs are ordered on the stream. */
+/* One fused step: halo from this PE's receive slots of parity `pin`, v's
+ * boundary into the neighbours' slots of parity `pout`. */
+void nvshmemh_jacobi_stencil_fused(double *u_sym, double *v_sym, double *recv_sym, int n, int pe0, int npes_here,
+                                   int chunk, int pin, int pout, hipStream_t s) {
+    if (chunk <= 0) chunk = n;
+    const int jt = (n + kTJ - 1) / kTJ, ic = (n + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_stencil_t<true>, dim3((unsigned)((n + kTK - 1) / kTK), (unsigned)(jt * ic), npes_here),
+                       dim3(256), 0, s, u_sym, v_sym, recv_sym, n, pe0, chunk, pin & 1, pout & 1);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+/* `steps` full steps; returns the buffer holding the result (u or v).  The
+ * current field's faces are put once into the neighbours' slots of the first
+ * step's parity; then every step is one fused stencil (halo in from the slots
+ * of its parity, faces out to the slots of the next) followed by the barrier:
+ * nvshmem_barrier_all_offload between processes, the kernel boundary in
+ * local mode (every PE's work is in the same launch). */
 double *nvshmemh_jacobi_run(double *u_sym, double *v_sym, double *recv_sym, int n, int steps, int chunk, int first_step,
                             hipStream_t s) {
     const bool local = nvshmemx_mode() == NVSHMEMX_MODE_LOCAL;
     const int pe0 = local ? 0 : nvshmem_my_pe();
     const int here = local ? nvshmem_n_pes() : 1;
     if (nvshmem_n_pes() != 8) NVSHMEMI_FATAL("jacobi needs exactly 8 PEs (2x2x2), found %d", nvshmem_n_pes());
+    check_n(n);
     double *a = u_sym, *b = v_sym;
+    if (steps <= 0) return a;
+    nvshmemh_jacobi_pack_put(a, recv_sym, n, pe0, here, first_step & 1, s);
+    if (!local) nvshmemx_barrier_all_on_stream(s);
     for (int t = 0; t < steps; ++t) {
         const int parity = (first_step + t) & 1;
-        nvshmemh_jacobi_pack_put(a, recv_sym, n, pe0, here, parity, s);
+        nvshmemh_jacobi_stencil_fused(a, b, recv_sym, n, pe0, here, chunk, parity, parity ^ 1, s);
         if (!local) nvshmemx_barrier_all_on_stream(s);
-        nvshmemh_jacobi_unpack(a, recv_sym, n, pe0, here, parity, s);
-        nvshmemh_jacobi_stencil(a, b, n, pe0, here, chunk, s);
         double *tmp = a;
         a = b;
         b = tmp;

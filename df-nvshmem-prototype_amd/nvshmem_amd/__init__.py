@@ -156,6 +156,7 @@ SIGNATURES.update({
     "nvshmemh_jacobi_pack_put": (None, [_P, _P, _I, _I, _I, _I, _P]),
     "nvshmemh_jacobi_unpack": (None, [_P, _P, _I, _I, _I, _I, _P]),
     "nvshmemh_jacobi_stencil": (None, [_P, _P, _I, _I, _I, _I, _P]),
+    "nvshmemh_jacobi_stencil_fused": (None, [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P]),
     "nvshmemh_jacobi_run": (_P, [_P, _P, _P, _I, _I, _I, _I, _P]),
 })
 

@@ -3,7 +3,9 @@ a periodic 2x2x2 grid, bit-exact against the oracle's host replay of the whole
 undecomposed domain.  Local mode (all 8 PEs in one process, kernel boundaries
 as the barrier) at 16^3 per PE for 100 steps and at the full 256^3 per PE for
 3 steps; process mode (8 processes, nvshmem_barrier_all_offload per step) at
-16^3 for 40 steps."""
+16^3 for 40 steps.  A step is normally one fused kernel (halo read from the
+receive slots, faces written into the neighbours' slots); the separate
+pack / unpack / stencil kernels are checked as well."""
 import json
 import os
 import subprocess
@@ -35,7 +37,7 @@ def _run_local(lib, hip, n, steps, chunk):
     return out
 
 
-@pytest.mark.parametrize("n,steps,chunk", [(16, 100, None), (16, 7, 5), (256, 3, 64)])
+@pytest.mark.parametrize("n,steps,chunk", [(16, 100, None), (16, 7, 5), (48, 5, 7), (256, 3, 64)])
 def test_jacobi_local_bitexact(local_pes, hip, oracle, n, steps, chunk):
     heap = "16M" if n == 16 else "320M"
     lib = local_pes.start(8, heap=heap)
@@ -48,6 +50,34 @@ def test_jacobi_local_bitexact(local_pes, hip, oracle, n, steps, chunk):
             bad = np.argwhere(got != blk)
             raise AssertionError(f"PE {pe}: {len(bad)} cells differ, first {bad[0]}: {got[tuple(bad[0])]!r} vs "
                                  f"{blk[tuple(bad[0])]!r}")
+
+
+@pytest.mark.parametrize("n", [16, 48])
+def test_jacobi_separate_kernels_bitexact(local_pes, hip, oracle, n):
+    """the unfused step (pack+put of u's faces, unpack into u's halo, stencil
+    reading the halo from u) gives the same bits as the fused step; n = 48 has
+    partial stencil tiles in k"""
+    lib = local_pes.start(8, heap="16M")
+    fb = lib.nvshmemh_jacobi_field_bytes(n)
+    u, v = lib.nvshmalloc(fb), lib.nvshmalloc(fb)
+    recv = lib.nvshmalloc(lib.nvshmemh_jacobi_recv_bytes(n))
+    for pe in range(8):
+        for p in (u, v):
+            hip.memset(lib.nvshmem_ptr(p, pe), 0, fb)
+    lib.nvshmemh_jacobi_init(u, n, 0, 8, SEED, None)
+    a, b = u, v
+    steps = 6
+    for t in range(steps):
+        lib.nvshmemh_jacobi_pack_put(a, recv, n, 0, 8, t & 1, None)
+        lib.nvshmemh_jacobi_unpack(a, recv, n, 0, 8, t & 1, None)
+        lib.nvshmemh_jacobi_stencil(a, b, n, 0, 8, 7, None)
+        a, b = b, a
+    hip.sync()
+    assert lib.nvshmemx_get_last_error() == 0
+    want = oracle.jacobi(2 * n, steps, SEED)
+    for pe in range(8):
+        got = na.jacobi_interior(hip, lib.nvshmem_ptr(a, pe), n, lib)
+        np.testing.assert_array_equal(got.view(np.uint64), na.jacobi_block(want, pe, n).view(np.uint64))
 
 
 def test_jacobi_processes_bitexact(tmp_path, oracle):
