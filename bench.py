@@ -112,7 +112,8 @@ def batch_worker():
     lib.nvshmemx_batch_plan_capture(plan, 10)
     out["graph_ms"] = timed(hip, lambda: lib.nvshmemx_batch_plan_graph_launch(plan, None), 3, 1) / 10
     lib.nvshmemx_batch_plan_destroy(plan)
-    owner = plan_of(na.BatchLaunch(0, 256, 3, na.BATCH_OWNER))
+    # each PE's puts on its own 128 workgroups (the default grid split by PE)
+    owner = plan_of(na.BatchLaunch(8192, 0, 2, na.BATCH_OWNER))
     out["owner_ms"] = timed(hip, lambda: lib.nvshmemx_batch_plan_launch(owner, None), 10, 2)
     lib.nvshmemx_batch_plan_destroy(owner)
     out["per_call_ms"] = timed(hip, lambda: lib.nvshmemx_put_requests_on_stream(arr, n, None), 3, 1)

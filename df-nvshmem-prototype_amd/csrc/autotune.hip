@@ -72,17 +72,32 @@ extern "C" int nvshmemx_effective_concurrency(size_t nbytes, size_t batch_bytes,
 }
 
 namespace nvshmemi {
-/* Largest piece the LPT schedule of a batch can absorb.  Any list schedule
- * ends within (mean worker load + largest piece) (Graham's bound), so pieces
- * held to 1/32 of a workgroup's mean share of the batch keep the tail within
- * about 3% of the mean.  The identical-request microbenchmarks cannot see this
- * (all their pieces are equal, so any piece size balances), hence the cap is
- * applied when choosing.  Workgroups = the plans' default grid of 2048.  The
- * cap is 0 (none) below 4 KiB: such batches do not fill the device and the
+size_t resident_threads() {
+    static size_t n = 0;
+    if (!n) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxThreadsPerMultiProcessor, dev) == hipSuccess &&
+            cus > 0 && per_cu > 0)
+            n = (size_t)cus * (size_t)per_cu;
+        else
+            n = (size_t)256 * 2048; /* MI355X: 256 compute units x 2048 threads */
+    }
+    return n;
+}
+
+/* Largest piece the schedule of a batch can absorb, as bytes per thread of
+ * the piece's workers.  Any list schedule ends within (mean load + largest
+ * piece) (Graham's bound), so pieces held to 1/32 of a thread's fair share of
+ * the batch keep the tail within about 3% of the mean.  The identical-request
+ * microbenchmarks cannot see this (all their pieces are equal, so any piece
+ * size balances), hence the cap is applied when choosing.  The cap is 0 (none)
+ * below 16 bytes per thread: such batches do not fill the device and the
  * measured concurrency feature covers them. */
 size_t balance_cap(size_t batch_bytes) {
-    const size_t cap = batch_bytes / (2048 * 32);
-    return cap >= 4096 ? cap : 0;
+    const size_t cap = batch_bytes / resident_threads() / 32;
+    return cap >= 16 ? cap : 0;
 }
 
 /* Contiguous bytes of one piece of candidate `c` on a request of `nbytes`. */
@@ -93,9 +108,11 @@ static size_t piece_bytes(size_t nbytes, int c) {
     return ((nbytes + parts - 1) / parts + kPieceQuantum - 1) / kPieceQuantum * kPieceQuantum;
 }
 
-/* Time-equivalent piece bytes (a wavefront moves a quarter of a 256-thread
- * workgroup's bytes per unit time, so its pieces count four times). */
-static size_t piece_cost(size_t nbytes, int c) { return piece_bytes(nbytes, c) * (c / 27 ? 1 : 4); }
+/* Bytes per thread of one piece: a workgroup piece is shared by the default
+ * batch workgroup's threads, a wavefront piece by 64 lanes. */
+static size_t piece_cost(size_t nbytes, int c) {
+    return piece_bytes(nbytes, c) / (size_t)(c / 27 ? kBatchBlock : 64);
+}
 
 /* Constrained choice over the 54 candidates' costs (times, or log-times when
  * `log_cost`).  Only candidates whose pieces fit under `cap` compete (when
@@ -148,8 +165,8 @@ extern "C" int nvshmemx_tune_measure(int npts, const double *log2_bytes, const i
                                      const int *align_class, int reps, int workgroups, int block, float *times_us) {
     (void)elem_bytes;
     if (npts < 0 || reps < 1) NVSHMEMI_FATAL("nvshmemx_tune_measure: bad arguments");
-    if (workgroups <= 0) workgroups = 2048; /* the batch plans' default grid */
-    if (block <= 0) block = 256;
+    if (workgroups < 0) workgroups = 0; /* 0: sized to the work, as the batch plans' default */
+    if (block <= 0) block = kBatchBlock;
     char *src = nullptr, *dst = nullptr;
     NVSHMEMI_HIP(hipMalloc((void **)&src, kPool + 4096));
     NVSHMEMI_HIP(hipMalloc((void **)&dst, kPool + 4096));
@@ -178,7 +195,7 @@ extern "C" int nvshmemx_tune_measure(int npts, const double *log2_bytes, const i
             const size_t k = std::max<size_t>(1, std::min(k_full, kMaxPieces / parts));
             reqs.resize(k);
             for (size_t i = 0; i < k; ++i) reqs[i] = resolved_req{src + i * stride, dst + i * stride + doff, size, 0};
-            plan_fill(plan, reqs, nullptr, &ch, 1, workgroups, block, NT_BOTH, ASSIGN_SCATTERED);
+            plan_fill(plan, reqs, nullptr, &ch, 1, workgroups, block, kBatchNontemporal, ASSIGN_SCATTERED);
             plan_launch(plan, nullptr);
             NVSHMEMI_HIP(hipEventRecord(e0, nullptr));
             for (int r = 0; r < reps; ++r) plan_launch(plan, nullptr);

@@ -23,9 +23,12 @@
  * pieces out with a shared atomic ticket; on MI355X the ticket counter became
  * the bottleneck as the grid grew, which the host-side assignment removes.)
  *
- * Worker sets: SHARED = one set of workers serving every request; OWNER = one
- * set per PE in local mode, serving only the requests whose source is in that
- * PE's heap (workgroup-partitioned PEs, each executing the puts it initiated).
+ * Worker sets: SHARED = one set of workers serving every request, pieces
+ * assigned by LPT; SOURCE_ORDER (the default) = one set, each workgroup taking
+ * a run of pieces in source-address order (workgroups dispatched together read
+ * neighbouring sources); OWNER = one set per PE in local mode, serving only
+ * the requests whose source is in that PE's heap (workgroup-partitioned PEs,
+ * each executing the puts it initiated).
  */
 #include <algorithm>
 #include <queue>
@@ -113,6 +116,33 @@ static std::vector<std::vector<piece_t>> lpt(std::vector<piece_t> pieces, int wo
     return lists;
 }
 
+/* Source order: pieces sorted by source address and cut into `workers`
+ * consecutive runs of (nearly) equal cost, run w to worker w.  The hardware
+ * dispatches workgroups in index order, so the workgroups running at any
+ * moment read neighbouring sources: puts whose sources overlap (or sit in the
+ * same heap) then share cache lines instead of each streaming its own.  The
+ * last run ends within one piece of the mean (runs close on the first piece
+ * that reaches their share). */
+static std::vector<std::vector<piece_t>> source_runs(std::vector<piece_t> pieces, int workers, size_t share,
+                                                    std::vector<double> &load) {
+    std::vector<std::vector<piece_t>> lists((size_t)workers);
+    std::stable_sort(pieces.begin(), pieces.end(),
+                     [](const piece_t &a, const piece_t &b) { return a.src < b.src; });
+    double total = 0;
+    for (const piece_t &p : pieces) total += (double)(p.nbytes / share) + kPieceCharge;
+    const double target = total / (double)workers;
+    double acc = 0;
+    size_t w = 0;
+    for (const piece_t &p : pieces) {
+        const double c = (double)(p.nbytes / share) + kPieceCharge;
+        lists[w].push_back(p);
+        load[w] += c;
+        acc += c;
+        if (acc >= target * (double)(w + 1) && w + 1 < (size_t)workers) ++w;
+    }
+    return lists;
+}
+
 } // namespace nvshmemi
 
 using namespace nvshmemi;
@@ -157,7 +187,8 @@ static void scatter(std::vector<piece_t> &v) {
 
 /* (Re)build plan `p` from resolved requests; device buffers are reused when
  * they are large enough, so the autotuner can rebuild per candidate without
- * allocating.  ASSIGN_LPT balances uneven pieces; ASSIGN_ROUND_ROBIN deals
+ * allocating.  wgs_per_set <= 0 sizes the grid to the pieces (at most the
+ * default grid).  ASSIGN_LPT balances uneven pieces; ASSIGN_ROUND_ROBIN deals
  * pieces in request order (same-size pieces); ASSIGN_SCATTERED deals them in
  * a fixed pseudo-random order, so that pieces running at the same time are not
  * address-adjacent -- as in a mixed batch, whose LPT order interleaves the
@@ -178,43 +209,68 @@ void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, co
         split_request(r.src, r.dst, r.nbytes, ch, ch.mode ? wg[r.set] : wave[r.set]);
         total += r.nbytes;
     }
+    if (wgs_per_set <= 0) {
+        /* grid sized to the work: one workgroup per workgroup piece or per
+         * workgroup's worth of wavefront pieces, at most the default grid */
+        size_t most = 1;
+        for (int s = 0; s < nsets; ++s)
+            most = std::max(most, wg[s].size() + (wave[s].size() + waves_per_wg - 1) / waves_per_wg);
+        wgs_per_set = (int)std::min<size_t>(std::max(1, kBatchWorkgroups / nsets), most);
+    }
     const size_t G = (size_t)nsets * wgs_per_set, W = G * waves_per_wg;
+    const size_t wps = (size_t)wgs_per_set * waves_per_wg; /* wavefront workers per set */
     std::vector<piece_t> &all = p->host_pieces;
     all.clear();
-    std::vector<std::vector<piece_t>> wg_lists(G), wave_lists(W);
-    for (int s = 0; s < nsets; ++s) {
-        const size_t g0 = (size_t)s * wgs_per_set, w0 = g0 * waves_per_wg;
-        std::vector<std::vector<piece_t>> a, b;
-        if (assign == ASSIGN_LPT) {
-            std::vector<double> wg_load((size_t)wgs_per_set, 0.0);
-            a = lpt(wg[s], wgs_per_set, (size_t)waves_per_wg, wg_load);
-            /* a wavefront starts its own pieces after its workgroup's */
-            std::vector<double> wave_load((size_t)wgs_per_set * waves_per_wg);
-            for (size_t w = 0; w < wave_load.size(); ++w) wave_load[w] = wg_load[w / waves_per_wg];
-            b = lpt(wave[s], wgs_per_set * waves_per_wg, 1, wave_load);
-        } else {
-            if (assign == ASSIGN_SCATTERED) scatter(wg[s]), scatter(wave[s]);
-            a.resize((size_t)wgs_per_set);
-            b.resize((size_t)wgs_per_set * waves_per_wg);
-            for (size_t i = 0; i < wg[s].size(); ++i) a[i % a.size()].push_back(wg[s][i]);
-            for (size_t i = 0; i < wave[s].size(); ++i) b[i % b.size()].push_back(wave[s][i]);
-        }
-        for (size_t i = 0; i < a.size(); ++i) wg_lists[g0 + i] = std::move(a[i]);
-        for (size_t i = 0; i < b.size(); ++i) wave_lists[w0 + i] = std::move(b[i]);
-    }
     std::vector<unsigned int> &wo = p->host_wg_off, &vo = p->host_wave_off;
     wo.assign(G + 1, 0);
     vo.assign(W + 1, 0);
-    for (size_t i = 0; i < G; ++i) {
-        wo[i] = (unsigned)all.size();
-        all.insert(all.end(), wg_lists[i].begin(), wg_lists[i].end());
+    if (assign == ASSIGN_LPT || assign == ASSIGN_SOURCE_ORDER) {
+        std::vector<std::vector<piece_t>> wg_lists(G), wave_lists(W);
+        for (int s = 0; s < nsets; ++s) {
+            const size_t g0 = (size_t)s * wgs_per_set, w0 = g0 * waves_per_wg;
+            std::vector<double> wg_load((size_t)wgs_per_set, 0.0);
+            std::vector<std::vector<piece_t>> a = assign == ASSIGN_LPT
+                                                      ? lpt(wg[s], wgs_per_set, (size_t)waves_per_wg, wg_load)
+                                                      : source_runs(wg[s], wgs_per_set, (size_t)waves_per_wg, wg_load);
+            /* a wavefront starts its own pieces after its workgroup's */
+            std::vector<double> wave_load(wps);
+            for (size_t w = 0; w < wps; ++w) wave_load[w] = wg_load[w / waves_per_wg];
+            std::vector<std::vector<piece_t>> b = lpt(wave[s], (int)wps, 1, wave_load);
+            for (size_t i = 0; i < a.size(); ++i) wg_lists[g0 + i] = std::move(a[i]);
+            for (size_t i = 0; i < b.size(); ++i) wave_lists[w0 + i] = std::move(b[i]);
+        }
+        for (size_t i = 0; i < G; ++i) {
+            wo[i] = (unsigned)all.size();
+            all.insert(all.end(), wg_lists[i].begin(), wg_lists[i].end());
+        }
+        wo[G] = (unsigned)all.size();
+        for (size_t i = 0; i < W; ++i) {
+            vo[i] = (unsigned)all.size();
+            all.insert(all.end(), wave_lists[i].begin(), wave_lists[i].end());
+        }
+        vo[W] = (unsigned)all.size();
+    } else {
+        /* worker i of a set takes pieces i, i + workers, i + 2 * workers, ...
+         * of the set's list, written straight into the flat layout */
+        size_t npieces = 0;
+        for (int s = 0; s < nsets; ++s) {
+            if (assign == ASSIGN_SCATTERED) scatter(wg[s]), scatter(wave[s]);
+            npieces += wg[s].size() + wave[s].size();
+        }
+        all.reserve(npieces);
+        for (int s = 0; s < nsets; ++s)
+            for (size_t i = 0; i < (size_t)wgs_per_set; ++i) {
+                wo[(size_t)s * wgs_per_set + i] = (unsigned)all.size();
+                for (size_t j = i; j < wg[s].size(); j += (size_t)wgs_per_set) all.push_back(wg[s][j]);
+            }
+        wo[G] = (unsigned)all.size();
+        for (int s = 0; s < nsets; ++s)
+            for (size_t i = 0; i < wps; ++i) {
+                vo[(size_t)s * wps + i] = (unsigned)all.size();
+                for (size_t j = i; j < wave[s].size(); j += wps) all.push_back(wave[s][j]);
+            }
+        vo[W] = (unsigned)all.size();
     }
-    wo[G] = (unsigned)all.size();
-    for (size_t i = 0; i < W; ++i) {
-        vo[i] = (unsigned)all.size();
-        all.insert(all.end(), wave_lists[i].begin(), wave_lists[i].end());
-    }
-    vo[W] = (unsigned)all.size();
     if (all.size() >= 0xFFFFFFF0u) NVSHMEMI_FATAL("batch of %zu pieces is too large", all.size());
     ensure_capacity((void **)&p->d_pieces, p->piece_capacity, all.size(), sizeof(piece_t));
     ensure_capacity((void **)&p->d_wg_off, p->wg_capacity, G + 1, sizeof(unsigned int));
@@ -279,7 +335,7 @@ extern "C" int nvshmemx_batch_plan_create(const nvshmemx_put_request_t *reqs, in
                                           const nvshmemx_batch_launch_t *launch, nvshmemx_batch_plan_t *out) {
     require_init("nvshmemx_batch_plan_create");
     if (nreq < 0 || (nreq > 0 && !reqs) || !out) NVSHMEMI_FATAL("nvshmemx_batch_plan_create: bad arguments");
-    nvshmemx_batch_launch_t L = {0, 256, NT_BOTH, NVSHMEMX_BATCH_SHARED};
+    nvshmemx_batch_launch_t L = {0, kBatchBlock, kBatchNontemporal, NVSHMEMX_BATCH_SOURCE_ORDER};
     if (launch) {
         if (launch->workgroups > 0) L.workgroups = launch->workgroups;
         if (launch->block > 0) L.block = launch->block;
@@ -288,6 +344,8 @@ extern "C" int nvshmemx_batch_plan_create(const nvshmemx_put_request_t *reqs, in
     }
     if (L.block < 64 || L.block > 1024 || L.block % 64)
         NVSHMEMI_FATAL("batch block size %d must be a multiple of 64 in [64, 1024]", L.block);
+    if (L.discipline < NVSHMEMX_BATCH_SHARED || L.discipline > NVSHMEMX_BATCH_SOURCE_ORDER)
+        NVSHMEMI_FATAL("unknown batch discipline %d", L.discipline);
     const bool owner = L.discipline == NVSHMEMX_BATCH_OWNER;
     if (owner && g.mode != NVSHMEMI_MODE_LOCAL)
         NVSHMEMI_FATAL("the owner discipline needs local (workgroup-partitioned) PEs");
@@ -301,9 +359,10 @@ extern "C" int nvshmemx_batch_plan_create(const nvshmemx_put_request_t *reqs, in
                                      owner ? source_pe(r.source) : 0};
     }
     const int wgs_per_set = owner ? (L.workgroups > 0 ? std::max(1, L.workgroups / nsets) : g.wgs_per_pe)
-                                  : (L.workgroups > 0 ? L.workgroups : 2048);
+                                  : (L.workgroups > 0 ? L.workgroups : 0 /* sized to the work */);
     nvshmemx_batch_plan *p = new nvshmemx_batch_plan();
-    plan_fill(p, rr, choices, nullptr, nsets, wgs_per_set, L.block, L.nontemporal, ASSIGN_LPT);
+    plan_fill(p, rr, choices, nullptr, nsets, wgs_per_set, L.block, L.nontemporal,
+              L.discipline == NVSHMEMX_BATCH_SOURCE_ORDER ? ASSIGN_SOURCE_ORDER : ASSIGN_LPT);
     *out = p;
     return 0;
 }

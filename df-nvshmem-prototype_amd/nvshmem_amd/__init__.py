@@ -114,7 +114,7 @@ class BatchLaunch(C.Structure):
     _fields_ = [("workgroups", _I), ("block", _I), ("nontemporal", _I), ("discipline", _I)]
 
 
-BATCH_SHARED, BATCH_OWNER = 0, 1
+BATCH_SHARED, BATCH_OWNER, BATCH_SOURCE_ORDER = 0, 1, 2
 
 SIGNATURES.update({
     "nvshmemx_batch_plan_create": (_I, [_P, _I, _P, _P, C.POINTER(_P)]),
@@ -209,7 +209,7 @@ def tune_grid(half_octaves=True, widths=(4, 8), concurrency=(1, 8, 64, 512, 4096
             "concurrency": np.array([p[2] for p in pts], np.int32), "align": np.array([p[3] for p in pts], np.int32)}
 
 
-def tune_measure(lib, grid, reps=3, workgroups=2048, block=256):
+def tune_measure(lib, grid, reps=3, workgroups=0, block=0):
     """times_us[npts, 54] from on-device measurement of every candidate"""
     import numpy as np
     n = len(grid["log2_bytes"])

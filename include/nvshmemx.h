@@ -110,11 +110,12 @@ typedef struct nvshmemx_batch_choice {
 
 #define NVSHMEMX_BATCH_SHARED 0 /* every workgroup of the launch serves every request */
 #define NVSHMEMX_BATCH_OWNER 1  /* local PEs: PE p's workgroups serve the requests whose source is in PE p's heap */
+#define NVSHMEMX_BATCH_SOURCE_ORDER 2 /* as SHARED, but each workgroup takes a run of pieces in source-address order */
 
 typedef struct nvshmemx_batch_launch {
-    int workgroups;  /* fused grid size (0: library default) */
-    int block;       /* threads per workgroup, multiple of 64 (0: 256) */
-    int nontemporal; /* bit 0 loads, bit 1 stores */
+    int workgroups;  /* fused grid size (0: one workgroup per piece, at most 8192) */
+    int block;       /* threads per workgroup, multiple of 64 (0: 1024) */
+    int nontemporal; /* bit 0 loads, bit 1 stores (NULL launch: stores only) */
     int discipline;  /* NVSHMEMX_BATCH_SHARED or NVSHMEMX_BATCH_OWNER */
 } nvshmemx_batch_launch_t;
 
@@ -122,7 +123,8 @@ typedef struct nvshmemx_batch_plan *nvshmemx_batch_plan_t;
 
 /* Build a plan (device-resident piece queues) for `nreq` requests.  `choices`
  * may be NULL (library rule, nvshmemx_batch_default_choice), `launch` may be
- * NULL (defaults).  Returns 0. */
+ * NULL (grid sized to the work, at most 8192 workgroups of 1024 threads,
+ * non-temporal stores, NVSHMEMX_BATCH_SOURCE_ORDER).  Returns 0. */
 int nvshmemx_batch_plan_create(const nvshmemx_put_request_t *reqs, int nreq, const nvshmemx_batch_choice_t *choices,
                                const nvshmemx_batch_launch_t *launch, nvshmemx_batch_plan_t *plan);
 /* One fused launch executing every request of the plan. */
