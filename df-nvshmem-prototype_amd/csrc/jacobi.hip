@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(256) k_stencil_t(const double *u_sym, double *
             const double val = 0.25 * x0 + 0.125 * s;
             const int j = j0 + r - 1;
             if (ok[q]) {
-                v[vb + (size_t)j * sy] = val;
+                __builtin_nontemporal_store(val, &v[vb + (size_t)j * sy]);
                 if constexpr (FUSED) {
                     if (i == 1) send(0, (size_t)(j - 1) * n + (kcol - 1), val);
                     if (i == n) send(1, (size_t)(j - 1) * n + (kcol - 1), val);
