@@ -76,12 +76,13 @@ BARRIER_ITERS, BARRIER_SKIP = 10000, 10     # barrier episodes timed on the devi
 BATCH_NPES, BATCH_WGS, BATCH_SRC, BATCH_DST, BATCH_REQS, BATCH_SEED = 64, 4, 32 << 20, 64 << 20, 4096, 7
 
 
-def batch_worker():
+def batch_worker(counter_pass=False):
     """Uneven-put workload on 64 workgroup-partitioned PEs in one process:
     4096 puts per step, log-uniform 64 B..4 MiB, seed 7.  Times one fused
     launch, the same under a captured hipGraph, the owner discipline (each PE's
     workgroups run the puts it initiated) and the per-call path (one launch per
-    put).  Prints one JSON line."""
+    put).  Prints one JSON line.  counter_pass: only five fused launches and
+    one 1 GiB device-to-device copy (the scale of a counter pass)."""
     import ctypes as C
     import nvshmem_amd as na
     os.environ["NVSHMEM_HEAP_SIZE"] = "96M"
@@ -106,6 +107,17 @@ def batch_worker():
         return plan
 
     plan = plan_of()
+    if counter_pass:
+        for _ in range(5):
+            lib.nvshmemx_batch_plan_launch(plan, None)
+        ref = hip.malloc(2 << 30)
+        hip.check(hip.lib.hipMemcpyAsync(ref + (1 << 30), ref, 1 << 30, hip.D2D, None))
+        hip.sync()
+        out["bitexact"] = int(lib.nvshmemh_verify_requests(arr, n)) == 0
+        lib.nvshmemx_batch_plan_destroy(plan)
+        lib.nvstop_pes()
+        print(json.dumps(out), flush=True)
+        return
     out["fused_ms"] = timed(hip, lambda: lib.nvshmemx_batch_plan_launch(plan, None), 20, 3)
     out["pieces"] = int(lib.nvshmemx_batch_plan_pieces(plan))
     out["bitexact"] = int(lib.nvshmemh_verify_requests(arr, n)) == 0
@@ -499,6 +511,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--pe-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--batch-worker", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--counter-pass", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--learned-worker", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--learned-batches", type=int, default=1000)
     ap.add_argument("--train-steps", type=int, default=3000)
@@ -512,7 +525,7 @@ def main():
         worker(args.pe_worker)
         return
     if args.batch_worker:
-        batch_worker()
+        batch_worker(args.counter_pass)
         return
     if args.learned_worker:
         learned_worker(args.learned_batches, args.train_steps)
