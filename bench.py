@@ -453,7 +453,8 @@ def cpu_baseline():
     """Same put semantics as host MPI-3 RMA (oracle/mpi_rma_baseline.c), 2 ranks
     on 2 host cores: 1 GiB float MPI_Put + flush repeated for ~10 s (the
     reported value), then the p+wait round trip and put/get of 8 B..1 GiB
-    (100 operations or 0.15 s per size), ~10 s more."""
+    (100 operations or 0.15 s per size), ~10 s more; then the Jacobi
+    application and the 4096-put batch as 8 ranks, a few seconds."""
     exe = ROOT / "oracle" / "build" / "rma_baseline"
     mpirun = Path("/opt/conda/bin/mpirun")
     if not exe.exists() or not mpirun.exists():
@@ -484,6 +485,27 @@ def cpu_baseline():
             res["sweep"][f"{op}_float_bitexact"] = all(r["bitexact"] for r in recs)
     except Exception as exc:
         res["sweep_error"] = str(exc)[-300:]
+
+    def run8(args, timeout=300):
+        out = subprocess.run([str(mpirun), "-n", "8", "-bind-to", "core", str(exe)] + args, capture_output=True,
+                             text=True, timeout=timeout, cwd=ROOT)
+        return [json.loads(l) for l in out.stdout.splitlines() if l.startswith("{")]
+
+    try:  # the Jacobi application and the uneven-put batch as 8 host ranks
+        check = run8(["jacobi", "32", "5", "1"])[-1]
+        jac = run8(["jacobi", str(JACOBI_N), "10", "0"])[-1]
+        res["jacobi"] = {"ranks": 8, "cores": 8, "n": JACOBI_N, "steps": jac["steps"],
+                         "ms_per_step": jac["ms_per_step"], "bitexact_small": check["bitexact"],
+                         "sample": "8 ranks x 256^3 f64, 10 steps: face MPI_Put + MPI_Win_fence + update; "
+                                   "bit-exact check against the host replay at 32^3 x 5 steps"}
+        bat = run8(["batch", str(BATCH_REQS), str(BATCH_SEED), "3"])[-1]
+        res["batch"] = {"ranks": 8, "cores": 8, "pes": bat["pes"], "requests": bat["requests"],
+                        "payload_bytes": bat["payload_bytes"], "best_ms": bat["best_ms"], "gbps": bat["gbps"],
+                        "bitexact": bat["bitexact"],
+                        "sample": "the 4096-put step (seed 7) as 8 ranks x 8 logical PEs, MPI_Put from each source "
+                                  "PE's rank + MPI_Win_flush_all, best of 3"}
+    except Exception as exc:
+        res["apps_error"] = str(exc)[-300:]
     return res
 
 

@@ -21,6 +21,11 @@
  *                                  operations per size or until BUDGET_S
  *                                  seconds; the last destination is checked
  *                                  bit-exact against the seeded source
+ *   jacobi N STEPS CHECK (8 ranks)  the 2x2x2 periodic Jacobi application,
+ *                                  N^3 float64 cells per rank; CHECK=1
+ *                                  compares with oracle_jacobi_run
+ *   batch NREQ SEED STEPS           the uneven-put workload, 64 logical PEs
+ *                                  over the ranks (8 ranks x 8 PEs)
  * Output: one JSON object per line on rank 0.
  */
 #define _GNU_SOURCE
@@ -191,21 +196,232 @@ static int do_putget(int rank, const char *op, const char *dtype, size_t min_b, 
     return all_ok ? 0 : 1;
 }
 
+/* ---- config 3: the Jacobi application, 8 ranks in a periodic 2x2x2 grid ----
+ * N^3 float64 cells per rank with a one-cell halo, initial values from the
+ * global PCG32 stream exactly as oracle_jacobi_init lays them out.  A step:
+ * pack the six boundary planes, MPI_Put each into the neighbour's receive
+ * slot (slots alternate by step parity), MPI_Win_fence (the barrier),
+ * unpack the own slots into the halo, update in the oracle's operation order
+ * (compiled without contraction).  With `check`, rank 0 gathers the blocks
+ * and compares them bit for bit with oracle_jacobi_run on the whole domain. */
+static size_t jidx(size_t P, int i, int j, int k) { return ((size_t)i * P + (size_t)j) * P + (size_t)k; }
+
+static void jface(int d, int a, int b, int n, int halo, int *i, int *j, int *k) {
+    const int plane = (d & 1) ? (halo ? n + 1 : n) : (halo ? 0 : 1);
+    if (d < 2) *i = plane, *j = a + 1, *k = b + 1;
+    else if (d < 4) *i = a + 1, *j = plane, *k = b + 1;
+    else *i = a + 1, *j = b + 1, *k = plane;
+}
+
+static int do_jacobi(int rank, int nranks, int n, int steps, int check) {
+    if (nranks != 8) {
+        if (rank == 0) fprintf(stderr, "jacobi needs 8 ranks\n");
+        return 1;
+    }
+    const int cx = rank >> 2, cy = (rank >> 1) & 1, cz = rank & 1;
+    const size_t P = (size_t)n + 2, cells = P * P * P, nn = (size_t)n * n;
+    double *u = (double *)calloc(cells, sizeof(double)), *v = (double *)calloc(cells, sizeof(double));
+    double *send = (double *)malloc(6 * nn * sizeof(double));
+    const uint64_t G = 2 * (uint64_t)n;
+    uint32_t *w = (uint32_t *)malloc(2 * (size_t)n * sizeof(uint32_t));
+    for (int i = 1; i <= n; ++i)
+        for (int j = 1; j <= n; ++j) {
+            const uint64_t gx = (uint64_t)cx * n + (uint64_t)(i - 1), gy = (uint64_t)cy * n + (uint64_t)(j - 1);
+            const uint64_t g = (gx * G + gy) * G + (uint64_t)cz * n;
+            oracle_pcg32_fill_at(w, 2 * (size_t)n, 42, 54, 2 * g);
+            for (int k = 1; k <= n; ++k) {
+                const uint64_t hi = w[2 * (k - 1)], lo = w[2 * (k - 1) + 1];
+                u[jidx(P, i, j, k)] = (double)(((hi << 32) | lo) >> 11) * (1.0 / 9007199254740992.0);
+            }
+        }
+    free(w);
+    double *recv = NULL;
+    MPI_Win win;
+    MPI_Win_allocate((MPI_Aint)(2 * 6 * nn * sizeof(double)), sizeof(double), MPI_INFO_NULL, MPI_COMM_WORLD, &recv,
+                     &win);
+    MPI_Win_fence(0, win);
+    MPI_Barrier(MPI_COMM_WORLD);
+    const double t0 = now_s();
+    for (int t = 0; t < steps; ++t) {
+        const int par = t & 1;
+        for (int d = 0; d < 6; ++d) {
+            for (int a = 0; a < n; ++a)
+                for (int b = 0; b < n; ++b) {
+                    int i, j, k;
+                    jface(d, a, b, n, 0, &i, &j, &k);
+                    send[(size_t)d * nn + (size_t)a * n + b] = u[jidx(P, i, j, k)];
+                }
+            int x = cx, y = cy, z = cz;
+            if (d < 2) x ^= 1;
+            else if (d < 4) y ^= 1;
+            else z ^= 1;
+            const int nb = (x << 2) | (y << 1) | z;
+            MPI_Put(send + (size_t)d * nn, (int)nn, MPI_DOUBLE, nb, (MPI_Aint)(((size_t)par * 6 + (size_t)(d ^ 1)) * nn),
+                    (int)nn, MPI_DOUBLE, win);
+        }
+        MPI_Win_fence(0, win);
+        for (int sd = 0; sd < 6; ++sd)
+            for (int a = 0; a < n; ++a)
+                for (int b = 0; b < n; ++b) {
+                    int i, j, k;
+                    jface(sd, a, b, n, 1, &i, &j, &k);
+                    u[jidx(P, i, j, k)] = recv[((size_t)par * 6 + (size_t)sd) * nn + (size_t)a * n + b];
+                }
+        for (int i = 1; i <= n; ++i)
+            for (int j = 1; j <= n; ++j)
+                for (int k = 1; k <= n; ++k) {
+                    const double sum = ((((u[jidx(P, i - 1, j, k)] + u[jidx(P, i + 1, j, k)]) + u[jidx(P, i, j - 1, k)]) +
+                                         u[jidx(P, i, j + 1, k)]) +
+                                        u[jidx(P, i, j, k - 1)]) +
+                                       u[jidx(P, i, j, k + 1)];
+                    v[jidx(P, i, j, k)] = 0.25 * u[jidx(P, i, j, k)] + 0.125 * sum;
+                }
+        double *tmp = u;
+        u = v;
+        v = tmp;
+    }
+    double dt = now_s() - t0, dt_max = 0;
+    MPI_Reduce(&dt, &dt_max, 1, MPI_DOUBLE, MPI_MAX, 0, MPI_COMM_WORLD);
+    int ok = -1;
+    if (check) {
+        double *blk = (double *)malloc((size_t)n * nn * sizeof(double));
+        for (int i = 1; i <= n; ++i)
+            for (int j = 1; j <= n; ++j)
+                memcpy(blk + ((size_t)(i - 1) * n + (size_t)(j - 1)) * n, u + jidx(P, i, j, 1), (size_t)n * sizeof(double));
+        double *all = rank == 0 ? (double *)malloc(8 * (size_t)n * nn * sizeof(double)) : NULL;
+        MPI_Gather(blk, (int)((size_t)n * nn), MPI_DOUBLE, all, (int)((size_t)n * nn), MPI_DOUBLE, 0, MPI_COMM_WORLD);
+        if (rank == 0) {
+            const size_t Gc = (size_t)G * G * G;
+            double *gu = (double *)malloc(Gc * sizeof(double)), *gv = (double *)malloc(Gc * sizeof(double));
+            oracle_jacobi_init(gu, (int)G, 42);
+            double *res = oracle_jacobi_run(gu, gv, (int)G, steps) ? gv : gu;
+            ok = 1;
+            for (int r = 0; r < 8 && ok; ++r) {
+                const int rx = r >> 2, ry = (r >> 1) & 1, rz = r & 1;
+                for (int i = 0; i < n && ok; ++i)
+                    for (int j = 0; j < n && ok; ++j) {
+                        const size_t gofs = (((size_t)rx * n + i) * G + ((size_t)ry * n + j)) * G + (size_t)rz * n;
+                        ok = memcmp(all + ((size_t)r * n * nn + ((size_t)i * n + j) * n), res + gofs,
+                                    (size_t)n * sizeof(double)) == 0;
+                    }
+            }
+            free(gu);
+            free(gv);
+            free(all);
+        }
+        free(blk);
+    }
+    if (rank == 0) {
+        printf("{\"mode\": \"jacobi\", \"n\": %d, \"steps\": %d, \"ranks\": 8, \"ms_per_step\": %.4f, \"bitexact\": %s}\n",
+               n, steps, dt_max * 1e3 / steps, ok < 0 ? "null" : (ok ? "true" : "false"));
+        fflush(stdout);
+    }
+    MPI_Win_free(&win);
+    free(u);
+    free(v);
+    free(send);
+    return ok == 0 ? 1 : 0;
+}
+
+/* ---- config 4: the uneven-put workload, 64 logical PEs as 8 ranks x 8 ----
+ * Each logical PE's heap (32 MiB sources, PCG32 seed 100 + pe, then 64 MiB
+ * of targets) lives in its rank's part of one shared window.  A step: every
+ * rank issues the requests whose source PE it owns (MPI_Put from its own
+ * window memory), then MPI_Win_flush_all and MPI_Barrier.  Afterwards each
+ * rank compares the targets it owns with the source bytes, read through the
+ * shared window. */
+static int do_batch(int rank, int nranks, int nreq, uint64_t seed, int steps) {
+    const int npes = 64;
+    if (npes % nranks) {
+        if (rank == 0) fprintf(stderr, "batch needs a rank count dividing 64\n");
+        return 1;
+    }
+    const int per = npes / nranks;
+    const size_t SRC = (size_t)32 << 20, DST = (size_t)64 << 20, HEAP = SRC + DST;
+    char *base = NULL;
+    MPI_Win win;
+    MPI_Win_allocate_shared((MPI_Aint)((size_t)per * HEAP), 1, MPI_INFO_NULL, MPI_COMM_WORLD, &base, &win);
+    for (int p = 0; p < per; ++p) {
+        oracle_pcg32_fill((uint32_t *)(base + (size_t)p * HEAP), SRC / 4, 100 + (uint64_t)(rank * per + p), 54);
+        memset(base + (size_t)p * HEAP + SRC, 0, DST);
+    }
+    int *sp = (int *)malloc(sizeof(int) * (size_t)nreq), *dp = (int *)malloc(sizeof(int) * (size_t)nreq);
+    int *dty = (int *)malloc(sizeof(int) * (size_t)nreq);
+    uint64_t *so = (uint64_t *)malloc(8 * (size_t)nreq), *dof = (uint64_t *)malloc(8 * (size_t)nreq);
+    uint64_t *nb = (uint64_t *)malloc(8 * (size_t)nreq);
+    if (oracle_batch_generate(seed, nreq, npes, SRC, DST, 6, 22, sp, dp, so, dof, nb, dty) != 0) {
+        if (rank == 0) fprintf(stderr, "batch generation failed\n");
+        return 1;
+    }
+    uint64_t payload = 0;
+    for (int i = 0; i < nreq; ++i) payload += nb[i];
+    MPI_Win_lock_all(MPI_MODE_NOCHECK, win);
+    MPI_Win_sync(win);
+    MPI_Barrier(MPI_COMM_WORLD);
+    double best = 1e30;
+    for (int t = 0; t < steps; ++t) {
+        MPI_Barrier(MPI_COMM_WORLD);
+        const double t0 = now_s();
+        for (int i = 0; i < nreq; ++i) {
+            if (sp[i] / per != rank) continue;
+            MPI_Put(base + (size_t)(sp[i] % per) * HEAP + so[i], (int)nb[i], MPI_BYTE, dp[i] / per,
+                    (MPI_Aint)((size_t)(dp[i] % per) * HEAP + SRC + dof[i]), (int)nb[i], MPI_BYTE, win);
+        }
+        MPI_Win_flush_all(win);
+        MPI_Barrier(MPI_COMM_WORLD);
+        const double dt = now_s() - t0;
+        if (dt < best) best = dt;
+    }
+    MPI_Win_sync(win);
+    MPI_Barrier(MPI_COMM_WORLD);
+    int ok = 1;
+    for (int i = 0; i < nreq && ok; ++i) {
+        if (dp[i] / per != rank) continue;
+        MPI_Aint size;
+        int disp;
+        char *src_base = NULL;
+        MPI_Win_shared_query(win, sp[i] / per, &size, &disp, &src_base);
+        ok = memcmp(base + (size_t)(dp[i] % per) * HEAP + SRC + dof[i], src_base + (size_t)(sp[i] % per) * HEAP + so[i],
+                    nb[i]) == 0;
+    }
+    int ok_all = 0;
+    MPI_Allreduce(&ok, &ok_all, 1, MPI_INT, MPI_LAND, MPI_COMM_WORLD);
+    if (rank == 0) {
+        printf("{\"mode\": \"batch\", \"requests\": %d, \"pes\": %d, \"ranks\": %d, \"payload_bytes\": %llu, "
+               "\"steps\": %d, \"best_ms\": %.4f, \"gbps\": %.3f, \"bitexact\": %s}\n",
+               nreq, npes, nranks, (unsigned long long)payload, steps, best * 1e3, 2.0 * (double)payload / best / 1e9,
+               ok_all ? "true" : "false");
+        fflush(stdout);
+    }
+    MPI_Win_unlock_all(win);
+    MPI_Win_free(&win);
+    free(sp), free(dp), free(dty), free(so), free(dof), free(nb);
+    return ok_all ? 0 : 1;
+}
+
 int main(int argc, char **argv) {
     MPI_Init(&argc, &argv);
     int rank, size;
     MPI_Comm_rank(MPI_COMM_WORLD, &rank);
     MPI_Comm_size(MPI_COMM_WORLD, &size);
     int rc = 2;
-    if (size != 2) {
-        if (rank == 0) fprintf(stderr, "needs exactly 2 ranks\n");
+    const int two_rank_mode = argc >= 2 && (strcmp(argv[1], "pingpong") == 0 || strcmp(argv[1], "putget") == 0);
+    if (two_rank_mode && size != 2) {
+        if (rank == 0) fprintf(stderr, "%s needs exactly 2 ranks\n", argv[1]);
     } else if (argc >= 3 && strcmp(argv[1], "pingpong") == 0) {
         rc = do_pingpong(rank, atol(argv[2]));
+    } else if (argc >= 5 && strcmp(argv[1], "jacobi") == 0) {
+        rc = do_jacobi(rank, size, atoi(argv[2]), atoi(argv[3]), atoi(argv[4]));
+    } else if (argc >= 5 && strcmp(argv[1], "batch") == 0) {
+        rc = do_batch(rank, size, atoi(argv[2]), (uint64_t)atoll(argv[3]), atoi(argv[4]));
     } else if (argc >= 8 && strcmp(argv[1], "putget") == 0) {
         rc = do_putget(rank, argv[2], argv[3], (size_t)atoll(argv[4]), (size_t)atoll(argv[5]), atol(argv[6]),
                        atof(argv[7]));
     } else if (rank == 0) {
-        fprintf(stderr, "usage: %s pingpong ITERS | putget put|get int|float|double MIN MAX REPS BUDGET_S\n", argv[0]);
+        fprintf(stderr,
+                "usage: %s pingpong ITERS | putget put|get int|float|double MIN MAX REPS BUDGET_S | jacobi N STEPS CHECK "
+                "| batch NREQ SEED STEPS\n",
+                argv[0]);
     }
     MPI_Finalize();
     return rc;

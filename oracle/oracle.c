@@ -32,6 +32,29 @@ void oracle_pcg32_fill(uint32_t *out, size_t n, uint64_t seed, uint64_t seq) {
     }
 }
 
+/* words offset .. offset+n-1 of the same stream, by O'Neill's jump-ahead
+ * (log2(offset) squarings of the affine step) */
+void oracle_pcg32_fill_at(uint32_t *out, size_t n, uint64_t seed, uint64_t seq, uint64_t offset) {
+    uint64_t inc = (seq << 1u) | 1u, st = 0;
+    st = st * PCG_MULT + inc;
+    st += seed;
+    st = st * PCG_MULT + inc;
+    uint64_t acc_mult = 1, acc_plus = 0, cur_mult = PCG_MULT, cur_plus = inc;
+    for (uint64_t d = offset; d; d >>= 1) {
+        if (d & 1) {
+            acc_mult *= cur_mult;
+            acc_plus = acc_plus * cur_mult + cur_plus;
+        }
+        cur_plus = (cur_mult + 1) * cur_plus;
+        cur_mult *= cur_mult;
+    }
+    st = acc_mult * st + acc_plus;
+    for (size_t i = 0; i < n; ++i) {
+        out[i] = pcg_out(st);
+        st = st * PCG_MULT + inc;
+    }
+}
+
 void oracle_pcg32_uniform_pm1(float *out, size_t n, uint64_t seed, uint64_t seq) {
     uint32_t *w = (uint32_t *)out;
     oracle_pcg32_fill(w, n, seed, seq);

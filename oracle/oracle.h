@@ -20,6 +20,8 @@ extern "C" {
 
 /* ---- seeded data (workload generator; PCG32 minimal C reference) ---- */
 void oracle_pcg32_fill(uint32_t *out, size_t n, uint64_t seed, uint64_t seq);
+/* words offset .. offset+n-1 of the stream oracle_pcg32_fill starts at 0 */
+void oracle_pcg32_fill_at(uint32_t *out, size_t n, uint64_t seed, uint64_t seq, uint64_t offset);
 /* u32 -> (u >> 8) * 2^-24 * 2 - 1, exactly representable in f32, in [-1, 1) */
 void oracle_pcg32_uniform_pm1(float *out, size_t n, uint64_t seed, uint64_t seq);
 /* u32 pairs -> (u64 >> 11) * 2^-53, exactly representable in f64, in [0, 1) */
@@ -91,6 +93,11 @@ int oracle_batch_generate(uint64_t seed, int nreq, int npes, uint64_t src_region
                           uint64_t *nbytes, int *dtype);
 void oracle_batch_apply(uint8_t **heaps, int nreq, const int *src_pe, const int *dst_pe, const uint64_t *src_off,
                         const uint64_t *dst_off, const uint64_t *nbytes, uint64_t src_base, uint64_t dst_base);
+
+/* ---- Jacobi host replay (oracle_jacobi.c) ---- */
+void oracle_jacobi_init(double *u, int G, uint64_t seed);
+void oracle_jacobi_step(const double *u, double *v, int G);
+int oracle_jacobi_run(double *u, double *v, int G, int steps);
 
 #ifdef __cplusplus
 }

@@ -332,3 +332,17 @@ def test_jacobi_run_conserves_sum_and_stays_in_range(oracle):
     # steps compose: 5 steps = 2 steps then 3 more
     u2 = oracle.jacobi(G, 2, 42)
     np.testing.assert_array_equal(oracle.jacobi_step(oracle.jacobi_step(oracle.jacobi_step(u2))), u5)
+
+
+def test_pcg32_jump_ahead_matches_the_stream(oracle):
+    """words offset..offset+n-1 by jump-ahead equal the same slice of the
+    sequentially generated stream (offsets 0, 1, odd, powers of two, large)"""
+    import ctypes as C
+    fill_at = oracle.lib.oracle_pcg32_fill_at
+    fill_at.restype, fill_at.argtypes = None, [C.c_void_p, C.c_size_t, C.c_uint64, C.c_uint64, C.c_uint64]
+    full = oracle.pcg32(70000, 42, 54)
+    for off in [0, 1, 2, 3, 1023, 1024, 4097, 65536, 69000]:
+        n = min(500, 70000 - off)
+        got = np.zeros(n, np.uint32)
+        fill_at(got.ctypes.data, n, 42, 54, off)
+        np.testing.assert_array_equal(got, full[off:off + n])
