@@ -138,7 +138,7 @@ def batch_worker(counter_pass=False):
 def learned_worker(n_batches, train_steps):
     """Learned config choice on the uneven-put workload (64 workgroup PEs):
     1. exhaustive autotuner: every one of the 54 candidates measured at every
-       point of a 1188-point feature grid (64,152 samples);
+       point of a 1782-point feature grid (96,228 samples);
     2. the MLP cost model trained on those samples on the GPU;
     3. `n_batches` held-out batches (seeds 1000..) of 4096 puts each, planned
        with the MLP's choices, the table's choices (exhaustive autotuner) and

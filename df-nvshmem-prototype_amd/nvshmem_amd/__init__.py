@@ -197,7 +197,7 @@ def candidates(lib):
     return out
 
 
-def tune_grid(half_octaves=True, widths=(4, 8), concurrency=(1, 8, 64, 512, 4096, 32768), aligns=(0, 1, 2),
+def tune_grid(half_octaves=True, widths=(4, 8), concurrency=(1, 4, 16, 64, 256, 1024, 4096, 16384, 32768), aligns=(0, 1, 2),
               min_log2=6, max_log2=22):
     """feature points of the autotuner's exhaustive table"""
     import numpy as np
