@@ -279,12 +279,16 @@ def jacobi_worker(layout, name=None):
                    for pe in pes_here)
     s0 = total(u)
     lib.nvshmem_barrier_all()
-    ms = timed(hip, lambda: lib.nvshmemh_jacobi_run(u, v, recv, n, JACOBI_STEPS, 64, 0, None), 1, 0) / JACOBI_STEPS
+    # planes per workgroup: one process's launch covers one PE, so it needs
+    # shorter chunks (more workgroups) to fill the device than the launch
+    # that covers all eight PEs
+    chunk = int(os.environ.get("JACOBI_CHUNK", "64" if layout == "local" else "16"))
+    ms = timed(hip, lambda: lib.nvshmemh_jacobi_run(u, v, recv, n, JACOBI_STEPS, chunk, 0, None), 1, 0) / JACOBI_STEPS
     res = u if JACOBI_STEPS % 2 == 0 else v
     lib.nvshmem_barrier_all()
     s1 = total(res)
     err = lib.nvshmemx_get_last_error()
-    out = {"layout": layout, "pes": 8, "cells_per_pe": n ** 3, "steps": JACOBI_STEPS, "ms_per_step": ms,
+    out = {"layout": layout, "pes": 8, "cells_per_pe": n ** 3, "steps": JACOBI_STEPS, "chunk": chunk, "ms_per_step": ms,
            "sum_before": s0, "sum_after": s1, "errors": err}
     lib.nvstop_pes()
     if pe0 == 0:
