@@ -9,7 +9,7 @@
  *    (16 B when target and source agree mod 16), four loads in flight, instead
  *    of one volatile 4/8-byte access per element (nvshmem_device.cu:497-500);
  *  - quiet is an agent-scope fence: all PEs share one agent;
- *  - waits poll with s_sleep back-off and give up after a wall-clock budget,
+ *  - waits spin, then poll with s_sleep back-off, and give up after a wall-clock budget,
  *    recording NVSHMEMI_ERR_WAIT_TIMEOUT, where the reference spins forever
  *    reading %globaltimer (nvshmem_device.cu:551-553).
  */

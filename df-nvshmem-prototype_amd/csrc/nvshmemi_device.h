@@ -93,9 +93,27 @@ __device__ __forceinline__ bool cmp_holds(int v, int op, int ref) {
     }
 }
 
-/* Poll *ivar with agent-scope loads until `v op ref` holds.  Back-off is a
- * short s_sleep ladder (64 clocks per step, at most 8 steps) so a waiting
- * wave yields issue slots without adding more than ~250 ns of latency.  A wait
+/* Polling pace shared by every wait: the first kSpinPolls polls go back to
+ * back (a partner that is already running answers within a microsecond or
+ * two, and sleeping then only adds latency); after that an s_sleep ladder (64
+ * clocks per step, doubling to 8 steps) lets a long wait yield issue slots. */
+#ifndef NVSHMEMI_SPIN_POLLS
+#define NVSHMEMI_SPIN_POLLS 256
+#endif
+struct poll_pace {
+    int polls = 0, steps = 1;
+    __device__ __forceinline__ void pause() {
+        if (polls < NVSHMEMI_SPIN_POLLS) {
+            ++polls;
+            return;
+        }
+        for (int i = 0; i < steps; ++i) __builtin_amdgcn_s_sleep(1);
+        if (steps < 8) steps <<= 1;
+    }
+};
+
+/* Poll *ivar with agent-scope loads until `v op ref` holds, paced by
+ * poll_pace.  A wait
  * that outlives the configured wall-clock budget records
  * NVSHMEMI_ERR_WAIT_TIMEOUT and returns instead of hanging the device.
  * Returns the last value read.  Ends with an agent-scope acquire so payload
@@ -105,10 +123,9 @@ __device__ __forceinline__ int wait_until(int *ivar, int op, int ref) {
     if (!cmp_holds(v, op, ref)) {
         const uint64_t budget = nvshmemi_dstate.wait_timeout_ticks;
         const uint64_t t0 = wall_clock64();
-        int backoff = 1;
+        poll_pace pace;
         for (;;) {
-            for (int i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(1);
-            if (backoff < 8) backoff <<= 1;
+            pace.pause();
             v = load_relaxed(ivar);
             if (cmp_holds(v, op, ref)) break;
             if ((uint64_t)(wall_clock64() - t0) > budget) {
@@ -138,10 +155,9 @@ __device__ __forceinline__ void gen_barrier(int *count, int *gen, int n, int tim
     }
     const uint64_t budget = nvshmemi_dstate.wait_timeout_ticks;
     const uint64_t t0 = wall_clock64();
-    int backoff = 1;
+    poll_pace pace;
     while (load_relaxed(gen) == g0) {
-        for (int i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(1);
-        if (backoff < 8) backoff <<= 1;
+        pace.pause();
         if ((uint64_t)(wall_clock64() - t0) > budget) {
             set_error(timeout_bit);
             break;
