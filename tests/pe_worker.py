@@ -8,6 +8,11 @@ Scenarios (every PE runs the same collective sequence):
   pingpong p + wait round trips between PE 0 and PE 1 (process kernels)
   barrier  rounds of: publish round r to every PE with p, offload barrier,
            check every slot >= r (sequence-counter ordering)
+  batch    every PE plans the puts of a shared uneven-put batch whose source
+           is its own heap (targets in the other processes' IPC-mapped heaps),
+           launches it and replays it from a hipGraph; each PE's targets are
+           checked bit-exact against the oracle's batch
+  jacobi   the 2x2x2 Jacobi application, one PE per process
 """
 import json
 import sys
@@ -91,6 +96,37 @@ def main():
         hip.sync()
         out["violations"] = int(hip.dtoh(viol, 8, np.uint64)[0])
         out["slots"] = hip.dtoh(slots, n * 4, np.int32).tolist()
+        lib.nvshmem_barrier_all()
+    elif scenario == "batch":
+        import ctypes as C
+        import oracle_py
+        orc = oracle_py.Oracle()
+        SRC, DST = 4 << 20, 24 << 20
+        nreq, disc = int(sys.argv[2]), int(sys.argv[3])
+        base = lib.nvshmalloc(SRC + DST)
+        lib.nvshmemh_fill_pcg32(base, SRC // 4, 100 + me, 54, None)
+        hip.memset(base + SRC, 0, DST)
+        hip.sync()
+        reqs = na.gen_requests(lib, 11, nreq, n, SRC, DST, 6, 20)
+        mine = {k: v[reqs["src_pe"] == me] for k, v in reqs.items()}
+        arr = na.make_requests(lib, mine, base, 0, SRC)
+        k = len(arr)
+        lib.nvshmem_barrier_all()
+        plan = C.c_void_p()
+        launch = na.BatchLaunch(0, 0, 2, disc)
+        lib.nvshmemx_batch_plan_create(arr, k, None, C.byref(launch), C.byref(plan))
+        lib.nvshmemx_batch_plan_launch(plan, None)
+        lib.nvshmemx_batch_plan_capture(plan, 3)
+        lib.nvshmemx_batch_plan_graph_launch(plan, None)
+        hip.sync()
+        lib.nvshmemx_batch_plan_destroy(plan)
+        lib.nvshmem_barrier_all()
+        imgs = [np.zeros(SRC + DST, np.uint8) for _ in range(n)]
+        for pe in range(n):
+            imgs[pe][:SRC] = orc.pcg32(SRC // 4, 100 + pe).view(np.uint8)
+        orc.batch_apply(imgs, reqs, 0, SRC)
+        out["batch_ok"] = bool(np.array_equal(hip.dtoh(base + SRC, DST), imgs[me][SRC:]))
+        out["requests_issued"] = k
         lib.nvshmem_barrier_all()
     elif scenario == "jacobi":
         n, steps, outdir = int(sys.argv[2]), int(sys.argv[3]), Path(sys.argv[4])

@@ -63,3 +63,14 @@ def test_multiprocess_offload_barrier_ordering(npes):
     for o in outs:
         assert o["errors"] == 0 and o["violations"] == 0, o
         assert o["slots"] == [rounds] * npes
+
+
+@pytest.mark.parametrize("npes,discipline", [(2, na.BATCH_SOURCE_ORDER), (4, na.BATCH_SOURCE_ORDER),
+                                             (4, na.BATCH_SHARED)])
+def test_multiprocess_batch(npes, discipline):
+    """batched puts between processes: each PE issues its share of one batch
+    into the other PEs' IPC-mapped heaps; every PE's targets bit-exact"""
+    outs = run_job(npes, "batch", 64 * npes, discipline)
+    assert sum(o["requests_issued"] for o in outs) == 64 * npes
+    assert all(o["batch_ok"] for o in outs), outs
+    assert all(o["errors"] == 0 for o in outs)
