@@ -168,6 +168,56 @@ NVSHMEMI_DEF_COOP(int, int)
 NVSHMEMI_DEF_COOP(float, float)
 NVSHMEMI_DEF_COOP(double, double)
 
+/* ------------------------------------------ strided and 2-D transfers -- */
+
+namespace {
+template <typename T>
+__device__ __forceinline__ void strided(T *dst, const T *src, ptrdiff_t dst_st, ptrdiff_t src_st, size_t n,
+                                        size_t lane, size_t lanes) {
+    for (size_t i = lane; i < n; i += lanes) dst[(ptrdiff_t)i * dst_st] = src[(ptrdiff_t)i * src_st];
+}
+/* rows go to the workgroup's wavefronts in turn; each row is a wavefront copy */
+__device__ __forceinline__ void rows_block(char *dst, size_t dpitch, const char *src, size_t spitch, size_t width,
+                                           size_t height) {
+    const unsigned waves = (block_threads() + 63u) / 64u, w = flat_tid() / 64u, lane = flat_tid() & 63u;
+    for (size_t r = w; r < height; r += waves) copy<16, 4>(dst + r * dpitch, src + r * spitch, width, lane, 64);
+    fence_release();
+    __syncthreads();
+}
+} // namespace
+
+#define NVSHMEMI_DEF_STRIDED(Name, Type)                                                                          \
+    extern "C" __device__ void nvshmemx_##Name##_iput_wave(Type *t, const Type *s, ptrdiff_t tst, ptrdiff_t sst,  \
+                                                           size_t n, int pe) {                                   \
+        unsigned r, c;                                                                                            \
+        active_lanes(r, c);                                                                                       \
+        strided((Type *)translate(t, pe), s, tst, sst, n, r, c);                                                  \
+    }                                                                                                             \
+    extern "C" __device__ void nvshmemx_##Name##_iput_block(Type *t, const Type *s, ptrdiff_t tst, ptrdiff_t sst, \
+                                                            size_t n, int pe) {                                  \
+        strided((Type *)translate(t, pe), s, tst, sst, n, flat_tid(), block_threads());                           \
+        fence_release();                                                                                          \
+        __syncthreads();                                                                                          \
+    }                                                                                                             \
+    extern "C" __device__ void nvshmemx_##Name##_iget_block(Type *t, const Type *s, ptrdiff_t tst, ptrdiff_t sst, \
+                                                            size_t n, int pe) {                                  \
+        strided(t, (const Type *)translate(s, pe), tst, sst, n, flat_tid(), block_threads());                     \
+        fence_release();                                                                                          \
+        __syncthreads();                                                                                          \
+    }
+NVSHMEMI_DEF_STRIDED(int, int)
+NVSHMEMI_DEF_STRIDED(float, float)
+NVSHMEMI_DEF_STRIDED(double, double)
+
+extern "C" __device__ void nvshmemx_putmem_2d_block(void *target, size_t tpitch, const void *source, size_t spitch,
+                                                    size_t width, size_t height, int pe) {
+    rows_block(translate(target, pe), tpitch, (const char *)source, spitch, width, height);
+}
+extern "C" __device__ void nvshmemx_getmem_2d_block(void *target, size_t tpitch, const void *source, size_t spitch,
+                                                    size_t width, size_t height, int pe) {
+    rows_block((char *)target, tpitch, translate(source, pe), spitch, width, height);
+}
+
 /* ------------------------------------------------ workgroup barrier -- */
 
 /* Counter + generation barrier over every workgroup of the grid.  The

@@ -207,6 +207,46 @@ __global__ void k_barrier_latency(int iters, long long *samples) {
     }
 }
 
+/* One workgroup calls one strided / 2-D device transfer: op 0 iput_block,
+ * 1 iget_block, 2 iput_wave (wavefront 0 only), 3 putmem_2d_block,
+ * 4 getmem_2d_block; dtype 0 int, 1 float, 2 double (ops 0-2). */
+template <typename T>
+__device__ void strided_op(int op, T *t, const T *s, ptrdiff_t tst, ptrdiff_t sst, size_t n, int pe);
+template <>
+__device__ void strided_op<int>(int op, int *t, const int *s, ptrdiff_t tst, ptrdiff_t sst, size_t n, int pe) {
+    if (op == 0) nvshmemx_int_iput_block(t, s, tst, sst, n, pe);
+    else if (op == 1) nvshmemx_int_iget_block(t, s, tst, sst, n, pe);
+    else if (threadIdx.x < 64) nvshmemx_int_iput_wave(t, s, tst, sst, n, pe);
+}
+template <>
+__device__ void strided_op<float>(int op, float *t, const float *s, ptrdiff_t tst, ptrdiff_t sst, size_t n, int pe) {
+    if (op == 0) nvshmemx_float_iput_block(t, s, tst, sst, n, pe);
+    else if (op == 1) nvshmemx_float_iget_block(t, s, tst, sst, n, pe);
+    else if (threadIdx.x < 64) nvshmemx_float_iput_wave(t, s, tst, sst, n, pe);
+}
+template <>
+__device__ void strided_op<double>(int op, double *t, const double *s, ptrdiff_t tst, ptrdiff_t sst, size_t n,
+                                   int pe) {
+    if (op == 0) nvshmemx_double_iput_block(t, s, tst, sst, n, pe);
+    else if (op == 1) nvshmemx_double_iget_block(t, s, tst, sst, n, pe);
+    else if (threadIdx.x < 64) nvshmemx_double_iput_wave(t, s, tst, sst, n, pe);
+}
+
+__global__ void k_strided_device(int op, int dtype, void *target, const void *source, long long tst, long long sst,
+                                 size_t n, size_t tpitch, size_t spitch, size_t width, size_t height, int pe) {
+    if (op == 3) {
+        nvshmemx_putmem_2d_block(target, tpitch, source, spitch, width, height, pe);
+    } else if (op == 4) {
+        nvshmemx_getmem_2d_block(target, tpitch, source, spitch, width, height, pe);
+    } else if (dtype == 0) {
+        strided_op<int>(op, (int *)target, (const int *)source, tst, sst, n, pe);
+    } else if (dtype == 1) {
+        strided_op<float>(op, (float *)target, (const float *)source, tst, sst, n, pe);
+    } else {
+        strided_op<double>(op, (double *)target, (const double *)source, tst, sst, n, pe);
+    }
+}
+
 /* local mode: workgroup 0 is PE a (initiator), workgroup 1 is PE b */
 __global__ void k_pingpong_local(int *flag_sym, int pe_a, int pe_b, int iters, int *payload_sym, int payload_len,
                                  long long *samples) {
@@ -491,6 +531,14 @@ void nvshmemh_pingpong_process(int *flag_sym, int peer, int initiator, int iters
                                long long *samples, hipStream_t s) {
     hipLaunchKernelGGL(k_pingpong_process, dim3(1), dim3(64), 0, s, flag_sym, nvshmem_my_pe(), peer, initiator, iters,
                        payload_sym, payload_len, samples);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+void nvshmemh_strided_device(int op, int dtype, void *target, const void *source, long long tst, long long sst,
+                             size_t n, size_t tpitch, size_t spitch, size_t width, size_t height, int pe, int block,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_strided_device, dim3(1), dim3(block), 0, s, op, dtype, target, source, tst, sst, n, tpitch,
+                       spitch, width, height, pe);
     NVSHMEMI_HIP(hipGetLastError());
 }
 

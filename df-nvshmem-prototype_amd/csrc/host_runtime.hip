@@ -414,6 +414,61 @@ extern "C" void nvshmemx_putmem_config_on_stream(void *target, const void *sourc
     launch_copy(nvshmem_ptr(target, pe), source, nbytes, c, s);
 }
 
+static void check_2d(size_t pitch_a, size_t pitch_b, size_t width, const char *fn) {
+    if (pitch_a < width || pitch_b < width) NVSHMEMI_FATAL("%s: pitches must be at least the row width", fn);
+}
+static size_t extent_2d(size_t pitch, size_t width, size_t height) { return height ? (height - 1) * pitch + width : 0; }
+
+extern "C" void nvshmemx_putmem_2d_on_stream(void *target, size_t tpitch, const void *source, size_t spitch,
+                                             size_t width, size_t height, int pe, hipStream_t s) {
+    require_init("nvshmemx_putmem_2d_on_stream");
+    check_pe(pe, "nvshmemx_putmem_2d_on_stream");
+    check_2d(tpitch, spitch, width, "nvshmemx_putmem_2d_on_stream");
+    if (!width || !height) return;
+    check_symmetric(target, extent_2d(tpitch, width, height), "nvshmemx_putmem_2d_on_stream");
+    launch_copy_2d(nvshmem_ptr(target, pe), tpitch, source, spitch, width, height, s);
+}
+
+extern "C" void nvshmemx_getmem_2d_on_stream(void *target, size_t tpitch, const void *source, size_t spitch,
+                                             size_t width, size_t height, int pe, hipStream_t s) {
+    require_init("nvshmemx_getmem_2d_on_stream");
+    check_pe(pe, "nvshmemx_getmem_2d_on_stream");
+    check_2d(tpitch, spitch, width, "nvshmemx_getmem_2d_on_stream");
+    if (!width || !height) return;
+    check_symmetric(source, extent_2d(spitch, width, height), "nvshmemx_getmem_2d_on_stream");
+    launch_copy_2d(target, tpitch, nvshmem_ptr((void *)source, pe), spitch, width, height, s);
+}
+
+template <typename T>
+static void strided_on_stream(bool put, T *target, const T *source, ptrdiff_t tst, ptrdiff_t sst, size_t n, int pe,
+                              hipStream_t s, const char *fn) {
+    require_init(fn);
+    check_pe(pe, fn);
+    if (tst < 1 || sst < 1) NVSHMEMI_FATAL("%s: strides must be >= 1 (got %td, %td)", fn, tst, sst);
+    if (!n) return;
+    const size_t ext = ((n - 1) * (size_t)(put ? tst : sst) + 1) * sizeof(T);
+    if (put) {
+        check_symmetric(target, ext, fn);
+        launch_strided((T *)nvshmem_ptr(target, pe), source, tst, sst, n, s);
+    } else {
+        check_symmetric(source, ext, fn);
+        launch_strided(target, (const T *)nvshmem_ptr((void *)source, pe), tst, sst, n, s);
+    }
+}
+
+#define NVSHMEMI_DEF_HOST_STRIDED(Name, Type)                                                                      \
+    extern "C" void nvshmemx_##Name##_iput_on_stream(Type *target, const Type *source, ptrdiff_t tst, ptrdiff_t sst, \
+                                                     size_t n, int pe, hipStream_t s) {                            \
+        strided_on_stream<Type>(true, target, source, tst, sst, n, pe, s, "nvshmemx_" #Name "_iput_on_stream");   \
+    }                                                                                                              \
+    extern "C" void nvshmemx_##Name##_iget_on_stream(Type *target, const Type *source, ptrdiff_t tst, ptrdiff_t sst, \
+                                                     size_t n, int pe, hipStream_t s) {                            \
+        strided_on_stream<Type>(false, target, source, tst, sst, n, pe, s, "nvshmemx_" #Name "_iget_on_stream");  \
+    }
+NVSHMEMI_DEF_HOST_STRIDED(int, int)
+NVSHMEMI_DEF_HOST_STRIDED(float, float)
+NVSHMEMI_DEF_HOST_STRIDED(double, double)
+
 extern "C" void nvshmemx_default_copy_config(size_t nbytes, nvshmemx_copy_config_t *out) {
     if (!out) NVSHMEMI_FATAL("nvshmemx_default_copy_config: null output");
     const CopyConfig c = default_copy_config(nbytes);

@@ -148,6 +148,54 @@ void launch_copy(void *dst, const void *src, size_t nbytes, const CopyConfig &cf
     NVSHMEMI_HIP(hipGetLastError());
 }
 
+/* 2-D copy: rows go to wavefronts (narrow rows) or to workgroups (rows of at
+ * least 4 KiB), grid-stride over the rows. */
+template <bool WG_ROWS>
+__global__ void __launch_bounds__(256) k_copy_2d(char *dst, size_t dpitch, const char *src, size_t spitch, size_t width,
+                                                 size_t height) {
+    if constexpr (WG_ROWS) {
+        for (size_t r = blockIdx.x; r < height; r += gridDim.x)
+            copy<16, 4>(dst + r * dpitch, src + r * spitch, width, threadIdx.x, blockDim.x);
+    } else {
+        const size_t waves = (size_t)gridDim.x * (blockDim.x / 64), w = (size_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+        for (size_t r = w; r < height; r += waves)
+            copy<16, 4>(dst + r * dpitch, src + r * spitch, width, threadIdx.x & 63u, 64);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_strided(T *dst, const T *src, ptrdiff_t dst_st, ptrdiff_t src_st, size_t n) {
+    const size_t lanes = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += lanes)
+        dst[(ptrdiff_t)i * dst_st] = src[(ptrdiff_t)i * src_st];
+}
+
+void launch_copy_2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height,
+                    hipStream_t s) {
+    if (!width || !height) return;
+    const bool wg_rows = width >= 4096;
+    const size_t per_wg = wg_rows ? 1 : 4; /* rows per workgroup per pass */
+    const unsigned grid = (unsigned)std::min<size_t>(8192, (height + per_wg - 1) / per_wg);
+    if (wg_rows)
+        hipLaunchKernelGGL(k_copy_2d<true>, dim3(grid), dim3(256), 0, s, (char *)dst, dpitch, (const char *)src, spitch,
+                           width, height);
+    else
+        hipLaunchKernelGGL(k_copy_2d<false>, dim3(grid), dim3(256), 0, s, (char *)dst, dpitch, (const char *)src,
+                           spitch, width, height);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_strided(T *dst, const T *src, ptrdiff_t dst_st, ptrdiff_t src_st, size_t n, hipStream_t s) {
+    if (!n) return;
+    const unsigned grid = (unsigned)std::min<size_t>(8192, (n + 255) / 256);
+    hipLaunchKernelGGL(k_strided<T>, dim3(grid), dim3(256), 0, s, dst, src, dst_st, src_st, n);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+template void launch_strided<int>(int *, const int *, ptrdiff_t, ptrdiff_t, size_t, hipStream_t);
+template void launch_strided<float>(float *, const float *, ptrdiff_t, ptrdiff_t, size_t, hipStream_t);
+template void launch_strided<double>(double *, const double *, ptrdiff_t, ptrdiff_t, size_t, hipStream_t);
+
 __global__ void k_offload_barrier(int *count, int *gen, int npes) {
     if (threadIdx.x != 0) return;
     gen_barrier(count, gen, npes, NVSHMEMI_ERR_BARRIER_TIMEOUT);

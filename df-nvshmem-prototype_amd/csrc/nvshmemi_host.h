@@ -100,6 +100,10 @@ CopyConfig default_copy_config(size_t nbytes);
 /* host_runtime.hip: fatal unless [addr, addr + nbytes) lies in one PE's heap */
 void check_symmetric(const void *addr, size_t nbytes, const char *fn);
 void launch_copy(void *dst, const void *src, size_t nbytes, const CopyConfig &cfg, hipStream_t s);
+void launch_copy_2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height,
+                    hipStream_t s);
+template <typename T>
+void launch_strided(T *dst, const T *src, ptrdiff_t dst_st, ptrdiff_t src_st, size_t n, hipStream_t s);
 
 /* batch.hip: plans over requests whose targets are already resolved to this
  * process's address of the target heap (also used by the autotuner, which

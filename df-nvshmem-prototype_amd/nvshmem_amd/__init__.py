@@ -78,6 +78,16 @@ SIGNATURES = {
     "nvshmemx_double_get_on_stream": (None, [_P, _P, _SZ, _I, _P]),
     "nvshmemx_putmem_config_on_stream": (None, [_P, _P, _SZ, _I, _P, _P]),
     "nvshmemx_default_copy_config": (None, [_SZ, _P]),
+    "nvshmemx_putmem_2d_on_stream": (None, [_P, _SZ, _P, _SZ, _SZ, _SZ, _I, _P]),
+    "nvshmemx_getmem_2d_on_stream": (None, [_P, _SZ, _P, _SZ, _SZ, _SZ, _I, _P]),
+    "nvshmemx_int_iput_on_stream": (None, [_P, _P, C.c_ssize_t, C.c_ssize_t, _SZ, _I, _P]),
+    "nvshmemx_float_iput_on_stream": (None, [_P, _P, C.c_ssize_t, C.c_ssize_t, _SZ, _I, _P]),
+    "nvshmemx_double_iput_on_stream": (None, [_P, _P, C.c_ssize_t, C.c_ssize_t, _SZ, _I, _P]),
+    "nvshmemx_int_iget_on_stream": (None, [_P, _P, C.c_ssize_t, C.c_ssize_t, _SZ, _I, _P]),
+    "nvshmemx_float_iget_on_stream": (None, [_P, _P, C.c_ssize_t, C.c_ssize_t, _SZ, _I, _P]),
+    "nvshmemx_double_iget_on_stream": (None, [_P, _P, C.c_ssize_t, C.c_ssize_t, _SZ, _I, _P]),
+    "nvshmemh_strided_device": (None, [_I, _I, _P, _P, C.c_longlong, C.c_longlong, _SZ, _SZ, _SZ, _SZ, _SZ, _I, _I,
+                                       _P]),
     "nvshmemx_barrier_all_on_stream": (None, [_P]),
     "nvshmemx_trace_flush": (None, []),
     "nvshmemx_trace_enabled": (_I, []),

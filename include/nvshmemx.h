@@ -88,6 +88,28 @@ void nvshmemx_putmem_config_on_stream(void *target, const void *source, size_t n
 /* The launch shape nvshmemx_{put,get}mem_on_stream use for `nbytes`. */
 void nvshmemx_default_copy_config(size_t nbytes, nvshmemx_copy_config_t *out);
 
+/* Host-launched strided and 2-D transfers (grid-wide).  iput: target[i * tst]
+ * = source[i * sst] on `pe`, i < nelems; iget: the same read from `pe`
+ * (strides in elements, >= 1).  2-D: `height` rows of `width` bytes, row r at
+ * target + r * tpitch and source + r * spitch (pitches >= width); the
+ * symmetric side is the target of a put and the source of a get. */
+void nvshmemx_putmem_2d_on_stream(void *target, size_t tpitch, const void *source, size_t spitch, size_t width,
+                                  size_t height, int pe, hipStream_t stream);
+void nvshmemx_getmem_2d_on_stream(void *target, size_t tpitch, const void *source, size_t spitch, size_t width,
+                                  size_t height, int pe, hipStream_t stream);
+void nvshmemx_int_iput_on_stream(int *target, const int *source, ptrdiff_t tst, ptrdiff_t sst, size_t nelems, int pe,
+                                 hipStream_t stream);
+void nvshmemx_float_iput_on_stream(float *target, const float *source, ptrdiff_t tst, ptrdiff_t sst, size_t nelems,
+                                   int pe, hipStream_t stream);
+void nvshmemx_double_iput_on_stream(double *target, const double *source, ptrdiff_t tst, ptrdiff_t sst, size_t nelems,
+                                    int pe, hipStream_t stream);
+void nvshmemx_int_iget_on_stream(int *target, const int *source, ptrdiff_t tst, ptrdiff_t sst, size_t nelems, int pe,
+                                 hipStream_t stream);
+void nvshmemx_float_iget_on_stream(float *target, const float *source, ptrdiff_t tst, ptrdiff_t sst, size_t nelems,
+                                   int pe, hipStream_t stream);
+void nvshmemx_double_iget_on_stream(double *target, const double *source, ptrdiff_t tst, ptrdiff_t sst, size_t nelems,
+                                    int pe, hipStream_t stream);
+
 /* Stream-ordered device barrier on an explicit stream. */
 void nvshmemx_barrier_all_on_stream(hipStream_t stream);
 

@@ -59,6 +59,38 @@ __device__ void nvshmemx_int_get_block(int *target, const int *source, int len, 
 __device__ void nvshmemx_float_get_block(float *target, const float *source, int len, int pe);
 __device__ void nvshmemx_double_get_block(double *target, const double *source, int len, int pe);
 
+/* Strided and 2-D transfers (the reference API is contiguous only; halo faces
+ * of a 3-D block are strided).  iput: target[i * tst] = source[i * sst] on
+ * `pe` for i < nelems (strides in elements, >= 1), OpenSHMEM iput semantics;
+ * iget: target[i * tst] = source[i * sst] read from `pe`.  putmem_2d:
+ * `height` rows of `width` bytes, row r at target + r * tpitch on `pe` from
+ * source + r * spitch.  _wave: all active lanes of a wavefront call
+ * together; _block: every thread of the workgroup calls, and the block
+ * variants end with a release and a workgroup barrier like the contiguous
+ * ones. */
+__device__ void nvshmemx_int_iput_wave(int *target, const int *source, ptrdiff_t tst, ptrdiff_t sst, size_t nelems,
+                                       int pe);
+__device__ void nvshmemx_float_iput_wave(float *target, const float *source, ptrdiff_t tst, ptrdiff_t sst,
+                                         size_t nelems, int pe);
+__device__ void nvshmemx_double_iput_wave(double *target, const double *source, ptrdiff_t tst, ptrdiff_t sst,
+                                          size_t nelems, int pe);
+__device__ void nvshmemx_int_iput_block(int *target, const int *source, ptrdiff_t tst, ptrdiff_t sst, size_t nelems,
+                                        int pe);
+__device__ void nvshmemx_float_iput_block(float *target, const float *source, ptrdiff_t tst, ptrdiff_t sst,
+                                          size_t nelems, int pe);
+__device__ void nvshmemx_double_iput_block(double *target, const double *source, ptrdiff_t tst, ptrdiff_t sst,
+                                           size_t nelems, int pe);
+__device__ void nvshmemx_int_iget_block(int *target, const int *source, ptrdiff_t tst, ptrdiff_t sst, size_t nelems,
+                                        int pe);
+__device__ void nvshmemx_float_iget_block(float *target, const float *source, ptrdiff_t tst, ptrdiff_t sst,
+                                          size_t nelems, int pe);
+__device__ void nvshmemx_double_iget_block(double *target, const double *source, ptrdiff_t tst, ptrdiff_t sst,
+                                           size_t nelems, int pe);
+__device__ void nvshmemx_putmem_2d_block(void *target, size_t tpitch, const void *source, size_t spitch, size_t width,
+                                         size_t height, int pe);
+__device__ void nvshmemx_getmem_2d_block(void *target, size_t tpitch, const void *source, size_t spitch, size_t width,
+                                         size_t height, int pe);
+
 /* Block until (*ivar cmp value) holds; returns the value observed. */
 __device__ int nvshmemx_int_wait_until_cmp(int *ivar, int cmp, int value);
 

@@ -255,3 +255,27 @@ void oracle_batch_apply(uint8_t **heaps, int nreq, const int *src_pe, const int 
     for (int i = 0; i < nreq; ++i)
         memcpy(heaps[dst_pe[i]] + dst_base + dst_off[i], heaps[src_pe[i]] + src_base + src_off[i], nbytes[i]);
 }
+
+/* ---- strided and 2-D transfers (OpenSHMEM iput/iget semantics; the
+ * reference has contiguous put/get only, nvshmem_device.cu:487-540) ---- */
+void oracle_iput(uint8_t **heaps, size_t target_off, ptrdiff_t tst, const void *source, ptrdiff_t sst, size_t nelems,
+                 size_t elem_bytes, int pe) {
+    const uint8_t *src = (const uint8_t *)source;
+    for (size_t i = 0; i < nelems; ++i)
+        memcpy(heaps[pe] + target_off + (size_t)((ptrdiff_t)i * tst) * elem_bytes,
+               src + (size_t)((ptrdiff_t)i * sst) * elem_bytes, elem_bytes);
+}
+
+void oracle_iget(uint8_t *const *heaps, void *target, ptrdiff_t tst, size_t source_off, ptrdiff_t sst, size_t nelems,
+                 size_t elem_bytes, int pe) {
+    uint8_t *dst = (uint8_t *)target;
+    for (size_t i = 0; i < nelems; ++i)
+        memcpy(dst + (size_t)((ptrdiff_t)i * tst) * elem_bytes,
+               heaps[pe] + source_off + (size_t)((ptrdiff_t)i * sst) * elem_bytes, elem_bytes);
+}
+
+void oracle_put_2d(uint8_t **heaps, size_t target_off, size_t tpitch, const void *source, size_t spitch, size_t width,
+                   size_t height, int pe) {
+    for (size_t r = 0; r < height; ++r)
+        memcpy(heaps[pe] + target_off + r * tpitch, (const uint8_t *)source + r * spitch, width);
+}

@@ -54,6 +54,14 @@ void oracle_p64(uint8_t **heaps, size_t target_off, uint64_t value_bits, int pe)
 uint32_t oracle_g32(uint8_t *const *heaps, size_t source_off, int pe);
 uint64_t oracle_g64(uint8_t *const *heaps, size_t source_off, int pe);
 
+/* ---- strided and 2-D transfers (OpenSHMEM iput/iget; strides in elements) ---- */
+void oracle_iput(uint8_t **heaps, size_t target_off, ptrdiff_t tst, const void *source, ptrdiff_t sst, size_t nelems,
+                 size_t elem_bytes, int pe);
+void oracle_iget(uint8_t *const *heaps, void *target, ptrdiff_t tst, size_t source_off, ptrdiff_t sst, size_t nelems,
+                 size_t elem_bytes, int pe);
+void oracle_put_2d(uint8_t **heaps, size_t target_off, size_t tpitch, const void *source, size_t spitch, size_t width,
+                   size_t height, int pe);
+
 /* ---- waits (reference nvshmem_device.cu:542-584, README:157-174) ----
  * 1 when the wait returns for the observed value */
 int oracle_int_wait_done(int observed, int cmp_value);              /* returns once observed != cmp_value */

@@ -151,6 +151,26 @@ class Oracle:
         which = run(u.ctypes.data, v.ctypes.data, G, steps)
         return (v if which else u).reshape(G, G, G)
 
+    def iput(self, heaps, target_off, tst, source, sst, nelems, elem_bytes, pe):
+        f = self.lib.oracle_iput
+        f.restype, f.argtypes = None, [_P, _SZ, C.c_ssize_t, _P, C.c_ssize_t, _SZ, _SZ, _I]
+        ptrs = (C.c_void_p * len(heaps))(*[h.ctypes.data for h in heaps])
+        src = np.ascontiguousarray(source)
+        f(ptrs, target_off, tst, src.ctypes.data, sst, nelems, elem_bytes, pe)
+
+    def iget(self, heaps, target, tst, source_off, sst, nelems, elem_bytes, pe):
+        f = self.lib.oracle_iget
+        f.restype, f.argtypes = None, [_P, _P, C.c_ssize_t, _SZ, C.c_ssize_t, _SZ, _SZ, _I]
+        ptrs = (C.c_void_p * len(heaps))(*[h.ctypes.data for h in heaps])
+        f(ptrs, target.ctypes.data, tst, source_off, sst, nelems, elem_bytes, pe)
+
+    def put_2d(self, heaps, target_off, tpitch, source, spitch, width, height, pe):
+        f = self.lib.oracle_put_2d
+        f.restype, f.argtypes = None, [_P, _SZ, _SZ, _P, _SZ, _SZ, _SZ, _I]
+        ptrs = (C.c_void_p * len(heaps))(*[h.ctypes.data for h in heaps])
+        src = np.ascontiguousarray(source)
+        f(ptrs, target_off, tpitch, src.ctypes.data, spitch, width, height, pe)
+
     def jacobi_step(self, u):
         """one periodic Jacobi step of a G^3 float64 field"""
         u = np.ascontiguousarray(u, np.float64)

@@ -353,3 +353,105 @@ def test_barrier_latency_episodes(local_pes, hip, npes):
     ns = ticks * 1e6 / lib.nvshmemx_wallclock_khz()
     print(f"{npes} PEs: barrier p50 {np.percentile(ns, 50):.0f} ns, p99 {np.percentile(ns, 99):.0f} ns")
     assert np.percentile(ns, 50) < 100_000
+
+
+
+_STRIDED_DT = {"int": (0, 4), "float": (1, 4), "double": (2, 8)}
+
+
+@pytest.mark.parametrize("path", ["device_block", "device_wave", "host"])
+@pytest.mark.parametrize("dtype", ["int", "double"])
+def test_iput_iget(local_pes, hip, oracle, path, dtype):
+    """strided put into PE 1 and strided get back (OpenSHMEM iput/iget
+    semantics), device-cooperative and host-launched, against the oracle;
+    canary bytes between and around the addressed elements survive"""
+    lib = local_pes.start(2, heap="64M")
+    code, w = _STRIDED_DT[dtype]
+    for tst, sst, n in [(1, 1, 1000), (3, 1, 777), (1, 5, 513), (7, 3, 64), (2, 2, 0), (1, 1, 1), (4, 9, 5000)]:
+        lib.nvshmcleanup()
+        span_t = (tst * max(n - 1, 0) + 1) * w
+        span_s = (sst * max(n - 1, 0) + 1) * w
+        region = span_t + 2 * GUARD
+        sym = lib.nvshmalloc(region)
+        src_sym = lib.nvshmalloc(span_s + 64)
+        dst_on_pe1 = lib.nvshmem_ptr(sym, 1)
+        src_local = lib.nvshmem_ptr(src_sym, 0)
+        data = _src_bytes(oracle, span_s, _seed("iput", path, dtype, tst, sst, n))
+        hip.memset(dst_on_pe1, CANARY, region)
+        hip.htod(src_local, data)
+        target = sym + GUARD
+        if path == "host":
+            getattr(lib, f"nvshmemx_{dtype}_iput_on_stream")(target, src_local, tst, sst, n, 1, None)
+        else:
+            lib.nvshmemh_strided_device(0 if path == "device_block" else 2, code, target, src_local, tst, sst, n,
+                                        0, 0, 0, 0, 1, 256 if path == "device_block" else 64, None)
+        hip.sync()
+        assert lib.nvshmemx_get_last_error() == 0
+        images = [np.full(region, CANARY, np.uint8) for _ in range(2)]
+        if n:
+            oracle.iput(images, GUARD, tst, data, sst, n, w, 1)
+        _check_region(hip, oracle, lib, dst_on_pe1, images[1], region)
+        # strided get of what PE 1 now holds into a local buffer laid out like the source
+        back = hip.malloc(span_s + 2 * GUARD)
+        try:
+            hip.memset(back, CANARY, span_s + 2 * GUARD)
+            if path == "host":
+                getattr(lib, f"nvshmemx_{dtype}_iget_on_stream")(back + GUARD, target, sst, tst, n, 1, None)
+            else:
+                lib.nvshmemh_strided_device(1, code, back + GUARD, target, sst, tst, n, 0, 0, 0, 0, 1, 256, None)
+            hip.sync()
+            assert lib.nvshmemx_get_last_error() == 0
+            want = np.full(span_s + 2 * GUARD, CANARY, np.uint8)
+            if n:
+                oracle.iget(images, want[GUARD:], sst, GUARD, tst, n, w, 1)
+            _check_region(hip, oracle, lib, back, want, span_s + 2 * GUARD)
+        finally:
+            hip.free(back)
+
+
+@pytest.mark.parametrize("path", ["device", "host"])
+def test_put_get_2d(local_pes, hip, oracle, path):
+    """pitched 2-D put into PE 1 and 2-D get back, narrow and wide rows,
+    relatively misaligned pitches, against the oracle"""
+    lib = local_pes.start(2, heap="64M")
+    for width, height, tpitch, spitch in [(1, 1, 1, 1), (5, 7, 13, 9), (4096, 5, 5000, 4100),
+                                          (100000, 3, 100008, 100001), (64, 300, 64, 128), (0, 4, 8, 8)]:
+        lib.nvshmcleanup()
+        span_t = (height - 1) * tpitch + width
+        span_s = (height - 1) * spitch + width
+        region = span_t + 2 * GUARD
+        sym = lib.nvshmalloc(region)
+        src_sym = lib.nvshmalloc(span_s + 64)
+        dst_on_pe1 = lib.nvshmem_ptr(sym, 1)
+        src_local = lib.nvshmem_ptr(src_sym, 0)
+        data = _src_bytes(oracle, span_s, _seed("2d", path, width, height, tpitch, spitch))
+        hip.memset(dst_on_pe1, CANARY, region)
+        if span_s:
+            hip.htod(src_local, data)
+        target = sym + GUARD
+        if path == "host":
+            lib.nvshmemx_putmem_2d_on_stream(target, tpitch, src_local, spitch, width, height, 1, None)
+        else:
+            lib.nvshmemh_strided_device(3, 0, target, src_local, 0, 0, 0, tpitch, spitch, width, height, 1, 256, None)
+        hip.sync()
+        assert lib.nvshmemx_get_last_error() == 0
+        images = [np.full(region, CANARY, np.uint8) for _ in range(2)]
+        oracle.put_2d(images, GUARD, tpitch, data, spitch, width, height, 1)
+        _check_region(hip, oracle, lib, dst_on_pe1, images[1], region)
+        back = hip.malloc(span_s + 2 * GUARD)
+        try:
+            hip.memset(back, CANARY, span_s + 2 * GUARD)
+            if path == "host":
+                lib.nvshmemx_getmem_2d_on_stream(back + GUARD, spitch, target, tpitch, width, height, 1, None)
+            else:
+                lib.nvshmemh_strided_device(4, 0, back + GUARD, target, 0, 0, 0, spitch, tpitch, width, height, 1,
+                                            256, None)
+            hip.sync()
+            assert lib.nvshmemx_get_last_error() == 0
+            want = np.full(span_s + 2 * GUARD, CANARY, np.uint8)
+            for r in range(height):
+                want[GUARD + r * spitch:GUARD + r * spitch + width] = images[1][GUARD + r * tpitch:
+                                                                              GUARD + r * tpitch + width]
+            _check_region(hip, oracle, lib, back, want, span_s + 2 * GUARD)
+        finally:
+            hip.free(back)

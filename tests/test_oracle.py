@@ -346,3 +346,37 @@ def test_pcg32_jump_ahead_matches_the_stream(oracle):
         got = np.zeros(n, np.uint32)
         fill_at(got.ctypes.data, n, 42, 54, off)
         np.testing.assert_array_equal(got, full[off:off + n])
+
+
+def test_strided_and_2d_closed_forms(oracle):
+    """iput/iget/put_2d against numpy strided slicing (an independent
+    restatement): only the addressed elements change, every one of them to
+    its source element; a planted transposition of strides is caught"""
+    rng = np.random.default_rng(5)
+    for w, dt in [(4, np.uint32), (8, np.uint64)]:
+        for tst, sst, n in [(1, 1, 7), (3, 1, 10), (1, 4, 9), (5, 2, 13), (2, 7, 1)]:
+            src = rng.integers(0, 2 ** 31, size=sst * n + 3).astype(dt)
+            heaps = [np.zeros(4096, np.uint8) for _ in range(2)]
+            oracle.iput(heaps, 16, tst, src, sst, n, w, 1)
+            want = np.zeros(4096, np.uint8)
+            view = want[16:16 + (tst * (n - 1) + 1) * w].view(dt)
+            view[::tst] = src[::sst][:n]
+            np.testing.assert_array_equal(heaps[1], want)
+            assert not heaps[0].any()
+            # iget reads back what iput wrote
+            back = np.zeros(sst * n + 3, dt)
+            oracle.iget(heaps, back, sst, 16, tst, n, w, 1)
+            np.testing.assert_array_equal(back[::sst][:n], src[::sst][:n])
+            if tst != sst and n > 1:
+                wrong = np.zeros(4096, np.uint8)
+                wrong[16:16 + (sst * (n - 1) + 1) * w].view(dt)[::sst] = src[::tst][:n] if len(src[::tst]) >= n else 0
+                assert not np.array_equal(heaps[1], wrong)
+    for width, height, tpitch, spitch in [(5, 3, 8, 5), (64, 4, 100, 70), (1, 9, 3, 2)]:
+        src = rng.integers(0, 256, size=spitch * height + 1).astype(np.uint8)
+        heaps = [np.zeros(2048, np.uint8) for _ in range(2)]
+        oracle.put_2d(heaps, 32, tpitch, src, spitch, width, height, 0)
+        want = np.zeros(2048, np.uint8)
+        for r in range(height):
+            want[32 + r * tpitch:32 + r * tpitch + width] = src[r * spitch:r * spitch + width]
+        np.testing.assert_array_equal(heaps[0], want)
+        assert not heaps[1].any()
