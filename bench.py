@@ -50,6 +50,7 @@ def worker(name):
     lib.nvstart_pes()
     dst = lib.nvshmalloc(NBYTES)
     flag = lib.nvshmalloc(256)
+    payload = lib.nvshmalloc(PUTFLAG_SIZES[-1])
     hip.memset(flag, 0, 256)
     hip.memset(dst, 0, NBYTES)
     hip.sync()
@@ -66,6 +67,12 @@ def worker(name):
     lib.nvshmem_barrier_all()                       # 3b: barrier episodes
     lib.nvshmemh_barrier_latency(BARRIER_ITERS, None, None)
     hip.sync()
+    for size in PUTFLAG_SIZES:                      # 3c: put + flag ping-pong per payload size
+        hip.memset(flag, 0, 256)
+        hip.sync()
+        lib.nvshmem_barrier_all()
+        lib.nvshmemh_pingpong_process(flag, 0, 0, PUTFLAG_ITERS, payload, size // 4, None, None)
+        hip.sync()
     err = lib.nvshmemx_get_last_error()
     lib.nvshmem_barrier_all()                       # 4: done
     lib.nvstop_pes()
@@ -73,6 +80,10 @@ def worker(name):
 
 
 BARRIER_ITERS, BARRIER_SKIP = 10000, 10     # barrier episodes timed on the device; the first few absorb launch skew
+# the reference's put-latency microbenchmark: per-thread put of 4 B..64 KiB,
+# quiet, flag p, quiet, wait; 20 warm-up + 200 timed round trips per size
+PUTFLAG_SIZES = [4, 64, 1024, 4096, 16384, 65536]
+PUTFLAG_ITERS, PUTFLAG_SKIP = 220, 20
 BATCH_NPES, BATCH_WGS, BATCH_SRC, BATCH_DST, BATCH_REQS, BATCH_SEED = 64, 4, 32 << 20, 64 << 20, 4096, 7
 
 
@@ -590,6 +601,7 @@ def main():
             lib.nvshmemx_start_local_pes(2, 1)
         dst = lib.nvshmalloc(NBYTES)
         flag = lib.nvshmalloc(256)
+        payload = lib.nvshmalloc(PUTFLAG_SIZES[-1])
         if args.layout == "local":
             hip.memset(lib.nvshmem_ptr(flag, 0), 0, 256)
             hip.memset(lib.nvshmem_ptr(flag, 1), 0, 256)
@@ -629,6 +641,22 @@ def main():
         hip.sync()
         bs = hip.dtoh(bsamples, BARRIER_ITERS * 8, np.int64)[BARRIER_SKIP:]
         khz = lib.nvshmemx_wallclock_khz()
+        putflag = {}
+        psamples = hip.malloc(PUTFLAG_ITERS * 8)
+        for size in PUTFLAG_SIZES:                                       # 3c
+            hip.memset(lib.nvshmem_ptr(flag, 0), 0, 256)
+            if args.layout == "local":
+                hip.memset(lib.nvshmem_ptr(flag, 1), 0, 256)
+            hip.sync()
+            lib.nvshmem_barrier_all()
+            if args.layout == "process":
+                lib.nvshmemh_pingpong_process(flag, 1, 1, PUTFLAG_ITERS, payload, size // 4, psamples, None)
+            else:
+                lib.nvshmemh_pingpong_local(flag, 0, 1, PUTFLAG_ITERS, payload, size // 4, psamples, None)
+            hip.sync()
+            ps = hip.dtoh(psamples, PUTFLAG_ITERS * 8, np.int64)[PUTFLAG_SKIP:].astype(np.float64) * 1e6 / khz
+            putflag[str(size)] = {"p50": round(float(np.percentile(ps, 50)), 1),
+                                  "p99": round(float(np.percentile(ps, 99)), 1)}
         err = lib.nvshmemx_get_last_error()
         lib.nvshmem_barrier_all()                                        # 4
         lib.nvstop_pes()
@@ -648,7 +676,7 @@ def main():
             memcpy_gbps=2 * NBYTES / (memcpy_ms * 1e-3) / 1e9, put_ok=put_ok, get_ok=get_ok, errors=err,
             put_kernel=put_kernel,
             rtt_p50=float(np.percentile(rtt_ns, 50)), rtt_p99=float(np.percentile(rtt_ns, 99)),
-            bar_p50=float(np.percentile(bar_ns, 50)), bar_p99=float(np.percentile(bar_ns, 99)))
+            bar_p50=float(np.percentile(bar_ns, 50)), bar_p99=float(np.percentile(bar_ns, 99)), putflag=putflag)
     finally:
         if child is not None and child.poll() is None:
             child.kill()
@@ -691,6 +719,8 @@ def main():
         "barrier_ns": {"p50": round(r["bar_p50"], 1), "p99": round(r["bar_p99"], 1),
                        "episodes": BARRIER_ITERS - BARRIER_SKIP,
                        "between": "2 processes" if args.layout == "process" else "2 workgroup PEs"},
+        "put_flag_rtt_ns": {"sizes": r["putflag"], "iters": PUTFLAG_ITERS - PUTFLAG_SKIP,
+                            "pattern": "per-thread put, quiet, p(flag), quiet, wait; both directions"},
         "bitexact": {"put": r["put_ok"], "get": r["get_ok"]},
         "device_errors": r["errors"],
         "batched": batched,

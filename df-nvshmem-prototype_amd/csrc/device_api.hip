@@ -67,12 +67,46 @@ NVSHMEMI_DEF_G(double, double, NVSHMEMI_OP_GET_DOUBLE)
 
 /* --------------------------------------------------- per-thread bulk -- */
 
+/* A single lane's copy is bound by round trips, not bandwidth.  When target
+ * and source agree modulo 16, the 16-byte vectors go in blocks of 16, then 8,
+ * then 4 loads in flight, and the last few one by one, so neither long nor
+ * short puts wait on more round trips than they need; other alignments take
+ * the general copy. */
+namespace {
+template <int K>
+__device__ __forceinline__ void lane_block(u32x4 *d, const u32x4 *s) {
+    u32x4 r[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) r[k] = s[k];
+#pragma unroll
+    for (int k = 0; k < K; ++k) d[k] = r[k];
+}
+__device__ __forceinline__ void lane_copy(void *dstv, const void *srcv, size_t nbytes) {
+    char *dst = (char *)dstv;
+    const char *src = (const char *)srcv;
+    if (((uintptr_t)dst ^ (uintptr_t)src) & 15) {
+        copy<16, 4>(dst, src, nbytes, 0, 1);
+        return;
+    }
+    const split<16> sp(dst, nbytes);
+    for (size_t i = 0; i < sp.head; ++i) dst[i] = src[i];
+    u32x4 *d = (u32x4 *)(dst + sp.head);
+    const u32x4 *s = (const u32x4 *)(src + sp.head);
+    size_t i = 0;
+    for (; i + 16 <= sp.nvec; i += 16) lane_block<16>(d + i, s + i);
+    if (i + 8 <= sp.nvec) lane_block<8>(d + i, s + i), i += 8;
+    if (i + 4 <= sp.nvec) lane_block<4>(d + i, s + i), i += 4;
+    for (; i < sp.nvec; ++i) d[i] = s[i];
+    for (size_t t = 0; t < sp.tail; ++t) dst[sp.tail_start + t] = src[sp.tail_start + t];
+}
+} // namespace
+
 #define NVSHMEMI_DEF_PUT(Name, Type, OpType)                                                \
     extern "C" __device__ void nvshmem_##Name##_put(Type *target, Type *source, int len, int pe) { \
         NVSHMEMI_IDEAL();                                                                   \
         NVSHMEMI_TRACE(OpType, source, target, len, pe, 0);                                \
         if (len <= 0) return;                                                               \
-        copy<16, 4>(translate(target, pe), source, (size_t)len * sizeof(Type), 0, 1);   \
+        lane_copy(translate(target, pe), source, (size_t)len * sizeof(Type));             \
     }
 NVSHMEMI_DEF_PUT(int, int, NVSHMEMI_OP_PUT_INT)
 NVSHMEMI_DEF_PUT(float, float, NVSHMEMI_OP_PUT_FLOAT)
@@ -83,7 +117,7 @@ NVSHMEMI_DEF_PUT(double, double, NVSHMEMI_OP_PUT_DOUBLE)
         NVSHMEMI_IDEAL();                                                                   \
         NVSHMEMI_TRACE(OpType, target, source, len, pe, 0);                                \
         if (len <= 0) return;                                                               \
-        copy<16, 4>(target, translate(source, pe), (size_t)len * sizeof(Type), 0, 1);   \
+        lane_copy(target, translate(source, pe), (size_t)len * sizeof(Type));             \
     }
 NVSHMEMI_DEF_GET(int, int, NVSHMEMI_OP_GET_INT)
 NVSHMEMI_DEF_GET(float, float, NVSHMEMI_OP_GET_FLOAT)
