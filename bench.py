@@ -259,7 +259,7 @@ def learned_worker(n_batches, train_steps):
 JACOBI_N, JACOBI_STEPS = 256, 100
 
 
-def jacobi_worker(layout, name=None):
+def jacobi_worker(layout, name=None, counter_pass=False):
     """3-D 7-point Jacobi, 8 PEs (2x2x2, periodic), 256^3 float64 cells per
     PE, 100 steps; each step: 6 face puts into the neighbours' receive slots,
     the barrier (kernel boundary in local mode, nvshmem_barrier_all_offload
@@ -294,6 +294,13 @@ def jacobi_worker(layout, name=None):
     # shorter chunks (more workgroups) to fill the device than the launch
     # that covers all eight PEs
     chunk = int(os.environ.get("JACOBI_CHUNK", "64" if layout == "local" else "16"))
+    if counter_pass:  # five steps and one 1 GiB device copy (the counters' scale)
+        lib.nvshmemh_jacobi_run(u, v, recv, n, 5, chunk, 0, None)
+        ref = hip.malloc(2 << 30)
+        hip.check(hip.lib.hipMemcpyAsync(ref + (1 << 30), ref, 1 << 30, hip.D2D, None))
+        hip.sync()
+        lib.nvstop_pes()
+        return
     ms = timed(hip, lambda: lib.nvshmemh_jacobi_run(u, v, recv, n, JACOBI_STEPS, chunk, 0, None), 1, 0) / JACOBI_STEPS
     res = u if JACOBI_STEPS % 2 == 0 else v
     lib.nvshmem_barrier_all()
@@ -571,7 +578,7 @@ def main():
         sweep_worker()
         return
     if args.jacobi_worker:
-        jacobi_worker(args.jacobi_worker)
+        jacobi_worker(args.jacobi_worker, counter_pass=args.counter_pass)
         return
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: all PEs share one GPU by design; --gpus {args.gpus} is not supported",

@@ -52,6 +52,7 @@ constexpr int kRowPad = 32;
  * thread owning one column of kTJ / 4 rows */
 constexpr int kTJ = 16;
 constexpr int kTK = 64;
+constexpr unsigned kXcds = 8; /* MI355X: eight accelerator complex dies */
 
 __device__ __forceinline__ uint32_t out32(uint64_t old) {
     const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
@@ -149,11 +150,19 @@ __global__ void k_unpack(double *u_sym, const double *recv_sym, int n, int pe0, 
 template <bool FUSED>
 __global__ void __launch_bounds__(256) k_stencil_t(const double *u_sym, double *v_sym, double *recv_sym, int n, int pe0,
                                                    int chunk, int pin, int pout) {
-    const geom g(n, pe0 + (int)blockIdx.z);
+    /* XCD-aware tile order: the dispatcher deals workgroup ids round-robin
+     * over the eight XCDs, each with its own L2.  Renumbering so that each XCD
+     * gets a contiguous run of tiles (k fastest, then j) lets tiles that share
+     * halo cache lines run on the same L2 instead of fetching them again. */
+    const unsigned gx = gridDim.x, gy = gridDim.y, nwg = gx * gy * gridDim.z;
+    unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if (nwg % kXcds == 0) lin = (lin % kXcds) * (nwg / kXcds) + lin / kXcds;
+    const unsigned bx = lin % gx, by = (lin / gx) % gy, bz = lin / (gx * gy);
+    const geom g(n, pe0 + (int)bz);
     const int jtiles = (n + kTJ - 1) / kTJ;
-    const int k0 = (int)blockIdx.x * kTK + 1;
-    const int j0 = (int)(blockIdx.y % jtiles) * kTJ + 1;
-    const int i0 = (int)(blockIdx.y / jtiles) * chunk + 1;
+    const int k0 = (int)bx * kTK + 1;
+    const int j0 = (int)(by % (unsigned)jtiles) * kTJ + 1;
+    const int i0 = (int)(by / (unsigned)jtiles) * chunk + 1;
     const int i1 = i0 + chunk - 1 < n ? i0 + chunk - 1 : n;
     const double *__restrict__ u = (const double *)nvshmemx_ptr_d(u_sym, g.pe);
     double *__restrict__ v = (double *)nvshmemx_ptr_d(v_sym, g.pe);
