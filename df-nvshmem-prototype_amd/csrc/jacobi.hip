@@ -48,10 +48,6 @@ constexpr uint64_t kMult = 6364136223846793005ULL;
 constexpr int kKOff = 15;
 constexpr int kRowPad = 32;
 
-/* stencil tile: kTJ rows x 64 columns per workgroup of 256 threads, each
- * thread owning one column of kTJ / 4 rows */
-constexpr int kTJ = 16;
-constexpr int kTK = 64;
 constexpr unsigned kXcds = 8; /* MI355X: eight accelerator complex dies */
 
 __device__ __forceinline__ uint32_t out32(uint64_t old) {
@@ -136,20 +132,24 @@ __global__ void k_unpack(double *u_sym, const double *recv_sym, int n, int pe0, 
 }
 
 /* Workgroup (k tile, j tile x i chunk, PE): walks planes i0..i1 of a
- * kTJ x kTK tile.  Each plane (tile plus a one-cell ring in j and k) is read
- * from memory once into a three-plane LDS ring; plane i+2 is fetched into
- * registers while plane i is consumed.  Thread (wave w, lane l) owns column
- * k0 + l of rows j0 + 4w .. j0 + 4w + 3 and keeps their x-1 values in
- * registers.
+ * TJ x TK tile.  Each plane (tile plus a one-cell ring in j and k) is read
+ * from memory once into one of two LDS buffers; plane i+2 is fetched into
+ * registers while plane i is consumed.  Thread (wave w, lane l) owns columns
+ * k0 + l + 64c (c < TK / 64) of rows j0 + w TJ/4 .. j0 + (w+1) TJ/4 - 1 and
+ * loads exactly those cells of every plane (waves 0 and 3 also load the j
+ * halo rows), so the x-1 and x+1 values it needs stay in its registers and
+ * LDS only serves the in-plane neighbours: two buffers and one barrier per
+ * plane suffice.
  *
  * FUSED: the halo is not in u but in this PE's receive slots of parity
  * `pin` (a halo cell is read straight from the slot of its face), and every
  * boundary value of v is also written into the receive slot of parity `pout`
  * of the neighbour across that face -- the next step's halo exchange rides on
  * this step's stores, so a step is one kernel plus the barrier. */
-template <bool FUSED>
+template <bool FUSED, int TJ, int TK>
 __global__ void __launch_bounds__(256) k_stencil_t(const double *u_sym, double *v_sym, double *recv_sym, int n, int pe0,
                                                    int chunk, int pin, int pout) {
+    constexpr int KC = TK / 64; /* columns per lane */
     /* XCD-aware tile order: the dispatcher deals workgroup ids round-robin
      * over the eight XCDs, each with its own L2.  Renumbering so that each XCD
      * gets a contiguous run of tiles (k fastest, then j) lets tiles that share
@@ -159,25 +159,23 @@ __global__ void __launch_bounds__(256) k_stencil_t(const double *u_sym, double *
     if (nwg % kXcds == 0) lin = (lin % kXcds) * (nwg / kXcds) + lin / kXcds;
     const unsigned bx = lin % gx, by = (lin / gx) % gy, bz = lin / (gx * gy);
     const geom g(n, pe0 + (int)bz);
-    const int jtiles = (n + kTJ - 1) / kTJ;
-    const int k0 = (int)bx * kTK + 1;
-    const int j0 = (int)(by % (unsigned)jtiles) * kTJ + 1;
+    const int jtiles = (n + TJ - 1) / TJ;
+    const int k0 = (int)bx * TK + 1;
+    const int j0 = (int)(by % (unsigned)jtiles) * TJ + 1;
     const int i0 = (int)(by / (unsigned)jtiles) * chunk + 1;
     const int i1 = i0 + chunk - 1 < n ? i0 + chunk - 1 : n;
     const double *__restrict__ u = (const double *)nvshmemx_ptr_d(u_sym, g.pe);
     double *__restrict__ v = (double *)nvshmemx_ptr_d(v_sym, g.pe);
     const double *__restrict__ rin = FUSED ? (const double *)nvshmemx_ptr_d(recv_sym, g.pe) : nullptr;
     const size_t sy = g.row_stride(), sx = (size_t)(n + 2) * sy, nn = (size_t)n * n;
-    __shared__ double tile[3][kTJ + 2][kTK + 2];
+    __shared__ double tile[2][TJ + 2][TK + 2];
     const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
-    const int kcol = k0 + lane;
-    const int jlim = j0 + kTJ < n + 1 ? j0 + kTJ : n + 1; /* last row to load (halo included) */
-    const int klim = k0 + kTK < n + 1 ? k0 + kTK : n + 1; /* last column to load */
-    constexpr int RPW = kTJ / 4;                          /* rows per wavefront */
-    constexpr int NR = (kTJ + 2 + 3) / 4;                 /* tile rows a wavefront loads */
-    const bool has_halo = threadIdx.x < 2 * (kTJ + 2);
+    const int jlim = j0 + TJ < n + 1 ? j0 + TJ : n + 1; /* last row to load (halo included) */
+    const int klim = k0 + TK < n + 1 ? k0 + TK : n + 1; /* last column to load */
+    constexpr int RPW = TJ / 4;                         /* rows per wavefront */
+    const bool has_halo = threadIdx.x < 2 * (TJ + 2);
     const int hr = (int)(threadIdx.x >> 1), hhi = (int)(threadIdx.x & 1u);
-    const int hk = hhi ? k0 + kTK : k0 - 1;
+    const int hk = hhi ? k0 + TK : k0 - 1;
 
     /* cell (i, j, k) of the (n+2)^3 block, 0 <= i, j, k <= n+1 */
     auto at = [&](int i, int j, int k) -> double {
@@ -198,13 +196,20 @@ __global__ void __launch_bounds__(256) k_stencil_t(const double *u_sym, double *
         dst[((size_t)pout * 6 + (size_t)(d ^ 1)) * nn + idx] = val;
     };
 
-    /* plane i of the tile into registers (issued early), then into LDS */
-    double pre[NR], preh = 0.0;
+    /* plane i of the tile into registers (issued early), then into LDS.
+     * Entry q < RPW is own row j0 + wv RPW + q; entry RPW the j halo row of
+     * waves 0 (row j0 - 1) and 3 (row j0 + TJ). */
+    const int hrow = wv == 0 ? 0 : (wv == 3 ? TJ + 1 : -1); /* tile row of the halo entry */
+    double pre[RPW + 1][KC], preh = 0.0;
     auto issue = [&](int i) {
 #pragma unroll
-        for (int rr = 0; rr < NR; ++rr) {
-            const int r = wv + 4 * rr, j = j0 - 1 + r;
-            pre[rr] = (r < kTJ + 2 && j <= jlim && kcol <= klim) ? at(i, j, kcol) : 0.0;
+        for (int q = 0; q <= RPW; ++q) {
+            const int r = q < RPW ? wv * RPW + q + 1 : hrow, j = j0 - 1 + r;
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+                const int kc = k0 + lane + 64 * c;
+                pre[q][c] = (r >= 0 && j <= jlim && kc <= klim) ? at(i, j, kc) : 0.0;
+            }
         }
         if (has_halo) {
             const int j = j0 - 1 + hr;
@@ -213,60 +218,78 @@ __global__ void __launch_bounds__(256) k_stencil_t(const double *u_sym, double *
     };
     auto commit = [&](int buf) {
 #pragma unroll
-        for (int rr = 0; rr < NR; ++rr) {
-            const int r = wv + 4 * rr;
-            if (r < kTJ + 2) tile[buf][r][lane + 1] = pre[rr];
+        for (int q = 0; q <= RPW; ++q) {
+            const int r = q < RPW ? wv * RPW + q + 1 : hrow;
+            if (r >= 0) {
+#pragma unroll
+                for (int c = 0; c < KC; ++c) tile[buf][r][lane + 1 + 64 * c] = pre[q][c];
+            }
         }
-        if (has_halo) tile[buf][hr][hhi ? kTK + 1 : 0] = preh;
+        if (has_halo) tile[buf][hr][hhi ? TK + 1 : 0] = preh;
     };
 
-    double xm[RPW];
-    bool ok[RPW];
+    double xm[RPW][KC], xp[RPW][KC]; /* own cells of planes i - 1 and i + 1 */
+    bool ok[RPW][KC];
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
         const int j = j0 + wv * RPW + q;
-        ok[q] = kcol <= n && j <= n;
-        xm[q] = ok[q] ? at(i0 - 1, j, kcol) : 0.0;
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            const int kc = k0 + lane + 64 * c;
+            ok[q][c] = kc <= n && j <= n;
+            xm[q][c] = ok[q][c] ? at(i0 - 1, j, kc) : 0.0;
+        }
     }
     issue(i0);
     commit(0);
     issue(i0 + 1);
     commit(1);
+#pragma unroll
+    for (int q = 0; q < RPW; ++q)
+#pragma unroll
+        for (int c = 0; c < KC; ++c) xp[q][c] = pre[q][c];
     __syncthreads();
-    int bc = 0, bn = 1, bf = 2; /* planes i, i + 1, and the one being fetched */
+    int bc = 0; /* buffer of plane i; plane i + 1 is in the other */
     for (int i = i0; i <= i1; ++i) {
         const bool fetch = i + 2 <= i1 + 1;
         if (fetch) issue(i + 2);
-        const size_t vb = (size_t)i * sx + kKOff + kcol;
 #pragma unroll
         for (int q = 0; q < RPW; ++q) {
             const int r = wv * RPW + q + 1;
-            const double x0 = tile[bc][r][lane + 1];
-            const double xp = tile[bn][r][lane + 1];
-            const double s = ((((xm[q] + xp) + tile[bc][r - 1][lane + 1]) + tile[bc][r + 1][lane + 1]) +
-                              tile[bc][r][lane]) +
-                             tile[bc][r][lane + 2];
-            const double val = 0.25 * x0 + 0.125 * s;
             const int j = j0 + r - 1;
-            if (ok[q]) {
-                __builtin_nontemporal_store(val, &v[vb + (size_t)j * sy]);
-                if constexpr (FUSED) {
-                    if (i == 1) send(0, (size_t)(j - 1) * n + (kcol - 1), val);
-                    if (i == n) send(1, (size_t)(j - 1) * n + (kcol - 1), val);
-                    if (j == 1) send(2, (size_t)(i - 1) * n + (kcol - 1), val);
-                    if (j == n) send(3, (size_t)(i - 1) * n + (kcol - 1), val);
-                    if (kcol == 1) send(4, (size_t)(i - 1) * n + (j - 1), val);
-                    if (kcol == n) send(5, (size_t)(i - 1) * n + (j - 1), val);
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+                const int cc = lane + 1 + 64 * c, kc = k0 + lane + 64 * c;
+                const double x0 = tile[bc][r][cc];
+                const double s = ((((xm[q][c] + xp[q][c]) + tile[bc][r - 1][cc]) + tile[bc][r + 1][cc]) +
+                                  tile[bc][r][cc - 1]) +
+                                 tile[bc][r][cc + 1];
+                const double val = 0.25 * x0 + 0.125 * s;
+                if (ok[q][c]) {
+                    __builtin_nontemporal_store(val, &v[(size_t)i * sx + (size_t)j * sy + (size_t)(kc + kKOff)]);
+                    if constexpr (FUSED) {
+                        if (i == 1) send(0, (size_t)(j - 1) * n + (kc - 1), val);
+                        if (i == n) send(1, (size_t)(j - 1) * n + (kc - 1), val);
+                        if (j == 1) send(2, (size_t)(i - 1) * n + (kc - 1), val);
+                        if (j == n) send(3, (size_t)(i - 1) * n + (kc - 1), val);
+                        if (kc == 1) send(4, (size_t)(i - 1) * n + (j - 1), val);
+                        if (kc == n) send(5, (size_t)(i - 1) * n + (j - 1), val);
+                    }
                 }
+                xm[q][c] = x0;
             }
-            xm[q] = x0;
         }
-        if (fetch) commit(bf);
+        /* every wave is done with buffer bc (plane i): refill it with plane
+         * i + 2, which is read two iterations on, past the next barrier */
         __syncthreads();
-        const int t = bc;
-        bc = bn;
-        bn = bf;
-        bf = t;
+        if (fetch) {
+            commit(bc);
+#pragma unroll
+            for (int q = 0; q < RPW; ++q)
+#pragma unroll
+                for (int c = 0; c < KC; ++c) xp[q][c] = pre[q][c];
+        }
+        bc ^= 1;
     }
 }
 
@@ -289,6 +312,23 @@ __global__ void k_init(double *u_sym, int n, int pe0, uint64_t s0, uint64_t inc)
 
 void check_n(int n) {
     if (n < 2 || n > 1024) NVSHMEMI_FATAL("jacobi: N = %d outside [2, 1024]", n);
+}
+
+/* Stencil tile: 8 rows x 128 columns per workgroup of 256 threads (two
+ * columns and two rows per lane).  Against 16 x 64 it halves the j halo rows
+ * a plane re-reads per output row; 4 x 256 re-reads more of them again. */
+constexpr int kTJ = 8;
+constexpr int kTK = 128;
+
+template <bool FUSED>
+static void launch_stencil(double *u_sym, double *v_sym, double *recv_sym, int n, int pe0, int npes_here, int chunk,
+                           int pin, int pout, hipStream_t s) {
+    if (chunk <= 0) chunk = n;
+    const int jt = (n + kTJ - 1) / kTJ, ic = (n + chunk - 1) / chunk;
+    const dim3 grid((unsigned)((n + kTK - 1) / kTK), (unsigned)(jt * ic), (unsigned)npes_here);
+    hipLaunchKernelGGL((k_stencil_t<FUSED, kTJ, kTK>), grid, dim3(256), 0, s, u_sym, v_sym, recv_sym, n, pe0, chunk,
+                       pin, pout);
+    NVSHMEMI_HIP(hipGetLastError());
 }
 
 } // namespace
@@ -334,22 +374,14 @@ void nvshmemh_jacobi_unpack(double *u_sym, double *recv_sym, int n, int pe0, int
 }
 
 void nvshmemh_jacobi_stencil(double *u_sym, double *v_sym, int n, int pe0, int npes_here, int chunk, hipStream_t s) {
-    if (chunk <= 0) chunk = n;
-    const int jt = (n + kTJ - 1) / kTJ, ic = (n + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_stencil_t<false>, dim3((unsigned)((n + kTK - 1) / kTK), (unsigned)(jt * ic), npes_here),
-                       dim3(256), 0, s, u_sym, v_sym, (double *)nullptr, n, pe0, chunk, 0, 0);
-    NVSHMEMI_HIP(hipGetLastError());
+    launch_stencil<false>(u_sym, v_sym, nullptr, n, pe0, npes_here, chunk, 0, 0, s);
 }
 
 /* One fused step: halo from this PE's receive slots of parity `pin`, v's
  * boundary into the neighbours' slots of parity `pout`. */
 void nvshmemh_jacobi_stencil_fused(double *u_sym, double *v_sym, double *recv_sym, int n, int pe0, int npes_here,
                                    int chunk, int pin, int pout, hipStream_t s) {
-    if (chunk <= 0) chunk = n;
-    const int jt = (n + kTJ - 1) / kTJ, ic = (n + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_stencil_t<true>, dim3((unsigned)((n + kTK - 1) / kTK), (unsigned)(jt * ic), npes_here),
-                       dim3(256), 0, s, u_sym, v_sym, recv_sym, n, pe0, chunk, pin & 1, pout & 1);
-    NVSHMEMI_HIP(hipGetLastError());
+    launch_stencil<true>(u_sym, v_sym, recv_sym, n, pe0, npes_here, chunk, pin & 1, pout & 1, s);
 }
 
 /* `steps` full steps; returns the buffer holding the result (u or v).  The
