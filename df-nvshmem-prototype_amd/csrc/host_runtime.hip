@@ -371,8 +371,10 @@ void check_symmetric(const void *addr, size_t nbytes, const char *fn) {
     const size_t span = g.mode == NVSHMEMI_MODE_LOCAL ? g.heap_stride * (size_t)g.npes : g.heap_bytes;
     if (a < lo || (size_t)(a - lo) >= span)
         NVSHMEMI_FATAL("%s: %p is not a symmetric address (heap %p + %zu bytes)", fn, addr, (const void *)lo, span);
+    /* in local mode a PE's slot is heap_stride bytes, of which the first
+     * heap_bytes are its heap: an offset in the padding is not symmetric */
     const size_t off = g.mode == NVSHMEMI_MODE_LOCAL ? (size_t)(a - lo) & (g.heap_stride - 1) : (size_t)(a - lo);
-    if (nbytes > g.heap_bytes - off)
+    if (off >= g.heap_bytes || nbytes > g.heap_bytes - off)
         NVSHMEMI_FATAL("%s: %zu bytes at symmetric offset %zu run past the %zu-byte heap", fn, nbytes, off,
                        g.heap_bytes);
 }

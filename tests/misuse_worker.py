@@ -31,6 +31,10 @@ def main(case):
         lib.nvshmemx_putmem_on_stream(private, sym, 4096, 1, None)
     elif case == "put_past_heap_end":
         lib.nvshmemx_putmem_on_stream(sym, private, int(lib.nvshmemx_heap_size()) + (4 << 20), 1, None)
+    elif case == "put_into_slot_padding":
+        # local PEs share one allocation in power-of-two slots; past the heap's
+        # end, inside the slot's padding, is not symmetric memory
+        lib.nvshmemx_putmem_on_stream(sym + int(lib.nvshmemx_heap_size()) + (8 << 20), private, 4096, 1, None)
     elif case == "symmetric_ok":
         lib.nvshmemx_putmem_on_stream(sym, private, 4096, 1, None)
     hip.sync()

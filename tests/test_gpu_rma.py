@@ -318,8 +318,8 @@ def test_wait_timeout_is_reported(local_pes, hip, until):
         lib.nvshmemx_set_wait_timeout_ms(5000)
 
 
-@pytest.mark.parametrize("case", ["get_private_source", "put_private_target", "put_past_heap_end", "heap_exhausted",
-                                  "too_many_pes", "symmetric_ok"])
+@pytest.mark.parametrize("case", ["get_private_source", "put_private_target", "put_past_heap_end",
+                                  "put_into_slot_padding", "heap_exhausted", "too_many_pes", "symmetric_ok"])
 def test_non_symmetric_ranges_are_refused(case):
     """host launchers check that the symmetric side of a copy lies in a PE's
     heap before launching: a private or out-of-heap range ends the process with
