@@ -83,6 +83,7 @@ BARRIER_ITERS, BARRIER_SKIP = 10000, 10     # barrier episodes timed on the devi
 # the reference's put-latency microbenchmark: per-thread put of 4 B..64 KiB,
 # quiet, flag p, quiet, wait; 20 warm-up + 200 timed round trips per size
 PUTFLAG_SIZES = [4, 64, 1024, 4096, 16384, 65536]
+STREAM_SIZES_LOG2 = [26, 27, 28, 29]  # 64..512 MiB bulk streams beside the 1 GiB one
 PUTFLAG_ITERS, PUTFLAG_SKIP = 220, 20
 BATCH_NPES, BATCH_WGS, BATCH_SRC, BATCH_DST, BATCH_REQS, BATCH_SEED = 64, 4, 32 << 20, 64 << 20, 4096, 7
 
@@ -629,6 +630,29 @@ def main():
                        2)
         memcpy_ms = timed(hip, lambda: hip.check(hip.lib.hipMemcpyAsync(back, src, NBYTES, hip.D2D, None)),
                           max(args.steps // 4, 5), 2)
+        # smaller bulk sizes as streams of back-to-back launches whose buffers
+        # rotate over the whole 1 GiB source and target, so the 256 MB MALL
+        # cannot serve them (same offsets on both sides: the copies stay exact)
+        streams_by_size = {}
+        for lg in STREAM_SIZES_LOG2:
+            size = 1 << lg
+            k = NBYTES // size
+            reps = max(16, 2 * k)
+            rot = [0]
+
+            def put_rot():
+                i = rot[0] % k
+                rot[0] += 1
+                lib.nvshmemx_float_put_on_stream(dst + i * size, src + i * size, size // 4, 1, None)
+
+            def get_rot():
+                i = rot[0] % k
+                rot[0] += 1
+                lib.nvshmemx_float_get_on_stream(back + i * size, dst + i * size, size // 4, 1, None)
+            pm = timed(hip, put_rot, reps, k)
+            gm = timed(hip, get_rot, reps, k)
+            streams_by_size[str(size)] = {"put_gbps": round(2 * size / (pm * 1e-3) / 1e9, 1),
+                                          "get_gbps": round(2 * size / (gm * 1e-3) / 1e9, 1), "launches": reps}
         hip.sync()
         get_ok = bool(np.array_equal(hip.dtoh(back, NBYTES, np.uint32), hip.dtoh(src, NBYTES, np.uint32)))
         if args.layout == "local":
@@ -685,7 +709,8 @@ def main():
             memcpy_gbps=2 * NBYTES / (memcpy_ms * 1e-3) / 1e9, put_ok=put_ok, get_ok=get_ok, errors=err,
             put_kernel=put_kernel,
             rtt_p50=float(np.percentile(rtt_ns, 50)), rtt_p99=float(np.percentile(rtt_ns, 99)),
-            bar_p50=float(np.percentile(bar_ns, 50)), bar_p99=float(np.percentile(bar_ns, 99)), putflag=putflag)
+            bar_p50=float(np.percentile(bar_ns, 50)), bar_p99=float(np.percentile(bar_ns, 99)), putflag=putflag,
+            streams_by_size=streams_by_size)
     finally:
         if child is not None and child.poll() is None:
             child.kill()
@@ -722,6 +747,9 @@ def main():
         "cpu_baseline": cpu,
         "get_gbps": round(r["get_gbps"], 1),
         "hip_memcpy_d2d_gbps": round(r["memcpy_gbps"], 1),
+        "bulk_streams_gbps": {"sizes": r["streams_by_size"], "frac_of_peak_min": round(
+            min(min(v["put_gbps"], v["get_gbps"]) for v in r["streams_by_size"].values()) / PEAK_HBM_GBPS, 4),
+            "pattern": "back-to-back float put (get) launches, PE 0 -> PE 1, buffers rotated over 1 GiB"},
         "p_wait_rtt_ns": {"p50": round(r["rtt_p50"], 1), "p99": round(r["rtt_p99"], 1),
                           "iters": PINGPONG_ITERS, "between": "2 processes" if args.layout == "process" else
                           "2 workgroup PEs"},
