@@ -37,9 +37,12 @@ def _run_local(lib, hip, n, steps, chunk):
     return out
 
 
-@pytest.mark.parametrize("n,steps,chunk", [(16, 100, None), (16, 7, 5), (48, 5, 7), (256, 3, 64)])
+@pytest.mark.parametrize("n,steps,chunk", [(16, 100, None), (16, 7, 5), (48, 5, 7), (13, 6, 5), (130, 3, 64),
+                                           (256, 3, 64)])
 def test_jacobi_local_bitexact(local_pes, hip, oracle, n, steps, chunk):
-    heap = "16M" if n == 16 else "320M"
+    """n = 13 and 130 leave partial tiles in both j (8-row tiles) and k
+    (128-column tiles) and odd row lengths"""
+    heap = "16M" if n <= 16 else "320M"
     lib = local_pes.start(8, heap=heap)
     out = _run_local(lib, hip, n, steps, chunk)
     want = oracle.jacobi(2 * n, steps, SEED)
@@ -52,12 +55,12 @@ def test_jacobi_local_bitexact(local_pes, hip, oracle, n, steps, chunk):
                                  f"{blk[tuple(bad[0])]!r}")
 
 
-@pytest.mark.parametrize("n", [16, 48])
+@pytest.mark.parametrize("n", [16, 48, 13, 130])
 def test_jacobi_separate_kernels_bitexact(local_pes, hip, oracle, n):
     """the unfused step (pack+put of u's faces, unpack into u's halo, stencil
     reading the halo from u) gives the same bits as the fused step; n = 48 has
-    partial stencil tiles in k"""
-    lib = local_pes.start(8, heap="16M")
+    partial stencil tiles in k, n = 13 and 130 in j and k"""
+    lib = local_pes.start(8, heap="16M" if n <= 48 else "96M")
     fb = lib.nvshmemh_jacobi_field_bytes(n)
     u, v = lib.nvshmalloc(fb), lib.nvshmalloc(fb)
     recv = lib.nvshmalloc(lib.nvshmemh_jacobi_recv_bytes(n))
