@@ -254,29 +254,16 @@ extern "C" __device__ void nvshmemx_getmem_2d_block(void *target, size_t tpitch,
 
 /* ------------------------------------------------ workgroup barrier -- */
 
-/* Counter + generation barrier over every workgroup of the grid.  The
- * generation word is the barrier's sense: a workgroup reads it before
- * arriving (it cannot advance before this workgroup arrives), the last
- * arriver resets the counter and advances the generation with release, and
- * the others wait for the generation to change.  No per-workgroup state
- * survives between launches, so grids of different sizes may reuse it. */
+/* Count-and-generation barrier over every workgroup of the grid (the
+ * protocol of the offload barrier, one arrival per workgroup, on a word of
+ * its own).  No per-workgroup state survives between launches, so grids of
+ * different sizes may reuse it. */
 extern "C" __device__ void nvshmemx_barrier_all_wg(void) {
     fence_release();
     __syncthreads();
     if (flat_tid() == 0) {
-        char *internal = nvshmemi_dstate.heap_base[0];
-        int *count = (int *)(internal + NVSHMEMI_OFF_WGBAR_COUNT);
-        int *gen = (int *)(internal + NVSHMEMI_OFF_WGBAR_SENSE);
-        const int nwg = (int)(gridDim.x * gridDim.y * gridDim.z);
-        const int g0 = load_acquire(gen);
-        const int old = __hip_atomic_fetch_add(count, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == nwg - 1) {
-            store_relaxed(count, 0);
-            store_release(gen, g0 + 1);
-        } else {
-            wait_until(gen, CMP_NE, g0);
-        }
-        fence_acquire();
+        unsigned int *word = (unsigned int *)(nvshmemi_dstate.heap_base[0] + NVSHMEMI_OFF_WGBAR_WORD);
+        gen_barrier(word, (int)(gridDim.x * gridDim.y * gridDim.z), NVSHMEMI_ERR_WAIT_TIMEOUT);
     }
     __syncthreads();
 }

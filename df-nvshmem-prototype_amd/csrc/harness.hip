@@ -190,19 +190,18 @@ __global__ void k_pingpong_process(int *flag_sym, int me, int peer, int initiato
 }
 
 /* Barrier latency: `iters` episodes of the offload barrier's protocol (the
- * counter and generation words in PE 0's internal region), one lane per
+ * count-and-generation word in PE 0's internal region), one lane per
  * workgroup, each workgroup one arrival: launch one workgroup per process in
  * process mode, one per PE in local mode.  samples[i] = wall-clock ticks of
  * episode i as seen by workgroup 0 of this launch. */
 __global__ void k_barrier_latency(int iters, long long *samples) {
     if (threadIdx.x != 0) return;
     char *root = nvshmemi_dstate.heap_base[0];
-    int *count = (int *)(root + NVSHMEMI_OFF_BAR_COUNT);
-    int *gen = (int *)(root + NVSHMEMI_OFF_BAR_SENSE);
+    unsigned int *word = (unsigned int *)(root + NVSHMEMI_OFF_BAR_WORD);
     for (int i = 0; i < iters; ++i) {
         if (failed()) break;
         const long long t0 = (long long)wall_clock64();
-        nvshmemi::gen_barrier(count, gen, nvshmemi_dstate.npes, NVSHMEMI_ERR_BARRIER_TIMEOUT);
+        nvshmemi::gen_barrier(word, nvshmemi_dstate.npes, NVSHMEMI_ERR_BARRIER_TIMEOUT);
         if (blockIdx.x == 0 && samples) samples[i] = (long long)wall_clock64() - t0;
     }
 }

@@ -11,11 +11,12 @@
  *
  * Barrier (reference nvshmem_kernels.cu:56-102 gathers arrivals on PE 0 with
  * one spinning thread per peer, epoch values mod 65536, double-buffered by
- * parity, and fails open after 10^6 cycles): here one lane per PE does an
- * agent-scope fetch-add on a counter in PE 0's internal heap region and waits
- * for the generation word (the barrier's sense) to advance; the last arriver
- * resets the counter and advances the generation with release.  A wait that
- * outlives the wall-clock budget records NVSHMEMI_ERR_BARRIER_TIMEOUT.
+ * parity, and fails open after 10^6 cycles): here one lane per PE does one
+ * agent-scope fetch-add on a word in PE 0's internal heap region that holds
+ * the arrival count and the generation (the barrier's sense), and waits for
+ * the generation to advance; the last arriver stores the next generation with
+ * a zero count, with release.  A wait that outlives the wall-clock budget
+ * records NVSHMEMI_ERR_BARRIER_TIMEOUT.
  */
 #include <algorithm>
 
@@ -196,15 +197,15 @@ template void launch_strided<int>(int *, const int *, ptrdiff_t, ptrdiff_t, size
 template void launch_strided<float>(float *, const float *, ptrdiff_t, ptrdiff_t, size_t, hipStream_t);
 template void launch_strided<double>(double *, const double *, ptrdiff_t, ptrdiff_t, size_t, hipStream_t);
 
-__global__ void k_offload_barrier(int *count, int *gen, int npes) {
+__global__ void k_offload_barrier(unsigned int *word, int npes) {
     if (threadIdx.x != 0) return;
-    gen_barrier(count, gen, npes, NVSHMEMI_ERR_BARRIER_TIMEOUT);
+    gen_barrier(word, npes, NVSHMEMI_ERR_BARRIER_TIMEOUT);
 }
 
 void launch_offload_barrier(hipStream_t s) {
     char *root = g.heap_base[0];
-    hipLaunchKernelGGL(k_offload_barrier, dim3(1), dim3(64), 0, s, (int *)(root + NVSHMEMI_OFF_BAR_COUNT),
-                       (int *)(root + NVSHMEMI_OFF_BAR_SENSE), g.npes);
+    hipLaunchKernelGGL(k_offload_barrier, dim3(1), dim3(64), 0, s, (unsigned int *)(root + NVSHMEMI_OFF_BAR_WORD),
+                       g.npes);
     NVSHMEMI_HIP(hipGetLastError());
 }
 

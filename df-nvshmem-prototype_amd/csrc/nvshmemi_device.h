@@ -138,25 +138,28 @@ __device__ __forceinline__ int wait_until(int *ivar, int op, int ref) {
     return v;
 }
 
-/* One episode of the generation barrier on (count, gen) among `n` arrivals,
- * called by one lane per arrival: read the generation (it cannot advance
- * before this arrival), arrive with an agent-scope fetch-add; the last arriver
- * resets the count and advances the generation with release, the others poll
- * the generation with back-off until it moves or the wait budget runs out
- * (then `timeout_bit` is set and the episode is abandoned).  Ends with an
- * acquire, so what every arrival published before arriving is visible. */
-__device__ __forceinline__ void gen_barrier(int *count, int *gen, int n, int timeout_bit) {
-    const int g0 = load_acquire(gen);
-    const int old = __hip_atomic_fetch_add(count, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == n - 1) {
-        store_relaxed(count, 0);
-        store_release(gen, g0 + 1);
+/* One episode of the generation barrier on `word` among `n` arrivals (n <
+ * 65536), called by one lane per arrival.  The word holds the arrival count
+ * in its low 16 bits and the generation in its high 16 bits, so one
+ * agent-scope fetch-add both arrives and reports the generation of this
+ * episode (it cannot advance before this arrival).  The last arriver stores
+ * the next generation with a zero count, with release; the others poll the
+ * generation with back-off until it moves or the wait budget runs out (then
+ * `timeout_bit` is set and the episode is abandoned).  No arrival of the next
+ * episode can happen before that store: it needs the new generation.  Ends
+ * with an acquire, so what every arrival published before arriving is
+ * visible. */
+__device__ __forceinline__ void gen_barrier(unsigned int *word, int n, int timeout_bit) {
+    const unsigned int old = __hip_atomic_fetch_add(word, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int g0 = old >> 16;
+    if ((old & 0xFFFFu) == (unsigned int)(n - 1)) {
+        store_release(word, ((g0 + 1u) & 0xFFFFu) << 16);
         return;
     }
     const uint64_t budget = nvshmemi_dstate.wait_timeout_ticks;
     const uint64_t t0 = wall_clock64();
     poll_pace pace;
-    while (load_relaxed(gen) == g0) {
+    while ((load_relaxed(word) >> 16) == g0) {
         pace.pause();
         if ((uint64_t)(wall_clock64() - t0) > budget) {
             set_error(timeout_bit);

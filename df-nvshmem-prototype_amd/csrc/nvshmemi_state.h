@@ -27,15 +27,13 @@
 #define NVSHMEMI_MAX_PES 128
 
 /* bytes at the start of every PE heap reserved for the library (barrier
- * words, per-workgroup barrier senses); user allocations start after it */
+ * words); user allocations start after it */
 #define NVSHMEMI_INTERNAL_BYTES (64u * 1024u)
-/* offsets inside the internal region; 128-byte separation keeps the words on
- * distinct cache lines */
-#define NVSHMEMI_OFF_BAR_COUNT 0u
-#define NVSHMEMI_OFF_BAR_SENSE 128u
-#define NVSHMEMI_OFF_WGBAR_COUNT 256u
-#define NVSHMEMI_OFF_WGBAR_SENSE 384u
-#define NVSHMEMI_OFF_WG_SENSES 4096u /* int per workgroup, up to 8192 workgroups */
+/* offsets inside the internal region, on distinct cache lines.  A barrier
+ * word holds the arrival count in its low 16 bits and the generation (the
+ * barrier's sense) in its high 16 bits. */
+#define NVSHMEMI_OFF_BAR_WORD 0u     /* offload barrier among PEs */
+#define NVSHMEMI_OFF_WGBAR_WORD 256u /* barrier among the workgroups of a grid */
 #define NVSHMEMI_MAX_BARRIER_WGS 8192u
 
 #define NVSHMEMI_ALLOC_ALIGN 256u

@@ -169,22 +169,20 @@ int oracle_ref_barrier_half(long long k) {
     return oracle_ref_barrier_value(k) % 2; /* nvshmem.cu:873 */
 }
 
-void oracle_gen_barrier_init(oracle_gen_barrier_t *b, int npes) {
-    b->count = 0;
-    b->gen = 0;
+void oracle_gen_barrier_init(oracle_gen_barrier_t *b, int npes, uint32_t generation) {
+    b->word = (generation & 0xFFFFu) << 16;
     b->npes = npes;
 }
-int oracle_gen_barrier_read_gen(const oracle_gen_barrier_t *b) { return b->gen; }
 int oracle_gen_barrier_arrive(oracle_gen_barrier_t *b) {
-    int old = b->count++;
-    if (old == b->npes - 1) {
-        b->count = 0;
-        b->gen++;
-        return 1;
-    }
-    return 0;
+    const uint32_t old = b->word;
+    b->word = old + 1u; /* the fetch-add */
+    const uint32_t g0 = old >> 16;
+    if ((old & 0xFFFFu) == (uint32_t)(b->npes - 1)) b->word = ((g0 + 1u) & 0xFFFFu) << 16;
+    return (int)g0;
 }
-int oracle_gen_barrier_released(const oracle_gen_barrier_t *b, int gen_seen) { return b->gen != gen_seen; }
+int oracle_gen_barrier_released(const oracle_gen_barrier_t *b, int gen_seen) {
+    return (int)(b->word >> 16) != gen_seen;
+}
 
 /* ------------------------------------------------------------- trace -- */
 

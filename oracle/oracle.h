@@ -74,17 +74,18 @@ int oracle_ref_barrier_value(long long k);
 /* sync-array half used by the k-th call: value % 2 */
 int oracle_ref_barrier_half(long long k);
 
-/* Counter/generation barrier of this build, as a state machine: arrive() for
- * PE `pe` returns 1 if that arrival released the episode.  Used to check
- * exhaustively over arrival orders that no PE is released early. */
+/* Count-and-generation barrier of this build, as a state machine over its one
+ * 32-bit word (arrival count in the low 16 bits, generation in the high 16):
+ * arrive() is the fetch-add, returns the generation it observed, and on the
+ * last arrival stores the next generation with a zero count; released() is the
+ * waiters' poll.  Used to check exhaustively over arrival orders that no PE is
+ * released early. */
 typedef struct oracle_gen_barrier {
-    int count;
-    int gen;
+    uint32_t word;
     int npes;
 } oracle_gen_barrier_t;
-void oracle_gen_barrier_init(oracle_gen_barrier_t *b, int npes);
-int oracle_gen_barrier_read_gen(const oracle_gen_barrier_t *b);
-int oracle_gen_barrier_arrive(oracle_gen_barrier_t *b);           /* fetch-add; releases on the last */
+void oracle_gen_barrier_init(oracle_gen_barrier_t *b, int npes, uint32_t generation);
+int oracle_gen_barrier_arrive(oracle_gen_barrier_t *b);
 int oracle_gen_barrier_released(const oracle_gen_barrier_t *b, int gen_seen);
 
 /* ---- trace CSV (reference nvshmem.cu:117, 390, 411) ---- */

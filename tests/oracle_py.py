@@ -39,8 +39,7 @@ class Oracle:
             "oracle_nvshmalloc": (C.c_longlong, [_P, _SZ]),
             "oracle_heap_init": (None, [_P, _SZ, _SZ]),
             "oracle_nvshmcleanup": (None, [_P]),
-            "oracle_gen_barrier_init": (None, [_P, _I]),
-            "oracle_gen_barrier_read_gen": (_I, [_P]),
+            "oracle_gen_barrier_init": (None, [_P, _I, C.c_uint32]),
             "oracle_gen_barrier_arrive": (_I, [_P]),
             "oracle_gen_barrier_released": (_I, [_P, _I]),
             "oracle_trace_header": (C.c_char_p, []),
@@ -93,9 +92,9 @@ class Oracle:
         arr = (C.c_uint64 * len(bases))(*bases)
         return self.lib.oracle_translate(arr, anchor, mask, addr, pe)
 
-    def gen_barrier(self, npes):
-        buf = (C.c_int * 3)()
-        self.lib.oracle_gen_barrier_init(buf, npes)
+    def gen_barrier(self, npes, generation=0):
+        buf = (C.c_uint32 * 2)()
+        self.lib.oracle_gen_barrier_init(buf, npes, generation)
         return buf
 
     def batch_generate(self, seed, nreq, npes, src_region, dst_region, min_log2=6, max_log2=22):

@@ -143,10 +143,11 @@ def test_reference_barrier_value_sequence(oracle):
     assert all(L.oracle_ref_barrier_half(k) == vals[k] % 2 for k in range(0, 65540, 997))
 
 
-def _explore_barrier(npes, episodes, arrive, read_gen, released, init):
-    """Enumerate every interleaving of the per-PE steps [read gen, arrive,
-    wait-until-released] for `episodes` barrier episodes and check that no PE
-    leaves episode e before all PEs arrived in e.  Returns #states visited."""
+def _explore_barrier(npes, episodes, arrive, released, init):
+    """Enumerate every interleaving of the per-PE steps [arrive (one fetch-add,
+    which also returns the generation it observed), wait-until-released] for
+    `episodes` barrier episodes and check that no PE leaves episode e before
+    all PEs arrived in e.  Returns #states visited."""
     seen = set()
     stack = [((0,) * npes, (None,) * npes, (0,) * npes, init())]
     visited = 0
@@ -163,73 +164,85 @@ def _explore_barrier(npes, episodes, arrive, read_gen, released, init):
             st = list(state)
             ph, gs, e = list(phase), list(gen_seen), list(ep)
             if ph[p] == 0:
-                gs[p] = read_gen(st)
+                gs[p] = arrive(st)
                 ph[p] = 1
-            elif ph[p] == 1:
-                arrive(st)
-                ph[p] = 2
             else:
                 if not released(st, gs[p]):
                     continue
                 # leaving episode e: every PE must have arrived in e
-                arrived = [(ep[q] > e[p]) or (ep[q] == e[p] and phase[q] == 2) for q in range(npes)]
+                arrived = [(ep[q] > e[p]) or (ep[q] == e[p] and phase[q] == 1) for q in range(npes)]
                 assert all(arrived), f"PE {p} released early in episode {e[p]}"
                 ph[p], e[p] = 0, e[p] + 1
             stack.append((tuple(ph), tuple(gs), tuple(e), tuple(st)))
     return visited
 
 
-def _gen_barrier_model(npes, release_at):
+def _word_barrier_model(npes, release_at):
+    """the count-and-generation word in Python, releasing after `release_at`
+    + 1 arrivals (npes - 1 is the correct protocol)"""
     def init():
-        return [0, 0]  # count, gen
-
-    def read_gen(st):
-        return st[1]
+        return [0]
 
     def arrive(st):
         old = st[0]
-        st[0] += 1
-        if old == release_at:
-            st[0] = 0
-            st[1] += 1
+        st[0] = old + 1
+        if (old & 0xFFFF) == release_at:
+            st[0] = (((old >> 16) + 1) & 0xFFFF) << 16
+        return old >> 16
 
     def released(st, g):
-        return st[1] != g
-    return init, read_gen, arrive, released
+        return (st[0] >> 16) != g
+    return init, arrive, released
 
 
-@pytest.mark.parametrize("npes", [1, 2, 3, 4])
-def test_generation_barrier_exhaustive(oracle, npes):
-    """the counter/generation barrier (oracle state machine) never releases a
-    PE early, over every interleaving of 3 episodes"""
-    b = oracle.gen_barrier(npes)
+@pytest.mark.parametrize("npes,generation", [(1, 0), (2, 0), (3, 0), (4, 0), (3, 65535)])
+def test_generation_barrier_exhaustive(oracle, npes, generation):
+    """the count-and-generation barrier (oracle state machine) never releases
+    a PE early, over every interleaving of 3 episodes; generation 65535 wraps
+    to 0 inside the run"""
+    b = oracle.gen_barrier(npes, generation)
     L = oracle.lib
+    w0 = b[0]
+    assert w0 == (generation & 0xFFFF) << 16
 
     def init():
-        L.oracle_gen_barrier_init(b, npes)
-        return [0, 0]
-
-    def read_gen(st):
-        b[0], b[1] = st
-        return L.oracle_gen_barrier_read_gen(b)
+        return [w0]
 
     def arrive(st):
-        b[0], b[1] = st
-        L.oracle_gen_barrier_arrive(b)
-        st[0], st[1] = b[0], b[1]
+        b[0] = st[0]
+        g = L.oracle_gen_barrier_arrive(b)
+        st[0] = b[0]
+        return g
 
     def released(st, g):
-        b[0], b[1] = st
+        b[0] = st[0]
         return bool(L.oracle_gen_barrier_released(b, g))
-    assert _explore_barrier(npes, 3, arrive, read_gen, released, init) > 0
+    assert _explore_barrier(npes, 3, arrive, released, init) > 0
+
+
+def test_generation_barrier_word_known_answer(oracle):
+    """three arrivals of a 3-PE episode at generation 7: counts 1, 2, then the
+    next generation with a zero count; every arrival saw generation 7"""
+    b = oracle.gen_barrier(3, 7)
+    seen = [oracle.lib.oracle_gen_barrier_arrive(b) for _ in range(2)]
+    assert seen == [7, 7] and b[0] == (7 << 16) + 2
+    assert not oracle.lib.oracle_gen_barrier_released(b, 7)
+    assert oracle.lib.oracle_gen_barrier_arrive(b) == 7 and b[0] == 8 << 16
+    assert oracle.lib.oracle_gen_barrier_released(b, 7)
+    b = oracle.gen_barrier(2, 65535)
+    oracle.lib.oracle_gen_barrier_arrive(b)
+    oracle.lib.oracle_gen_barrier_arrive(b)
+    assert b[0] == 0  # generation wraps to 0
 
 
 def test_barrier_checker_catches_planted_mistake():
     """the exhaustive checker fails a barrier that releases one arrival early
     (mistake planted in a CPU model, never on the GPU)"""
-    init, read_gen, arrive, released = _gen_barrier_model(3, release_at=1)
+    init, arrive, released = _word_barrier_model(3, release_at=1)
     with pytest.raises(AssertionError, match="released early"):
-        _explore_barrier(3, 2, arrive, read_gen, released, init)
+        _explore_barrier(3, 2, arrive, released, init)
+    init, arrive, released = _word_barrier_model(3, release_at=2)
+    assert _explore_barrier(3, 2, arrive, released, init) > 0
 
 
 # ---------------------------------------------------------------- trace CSV
