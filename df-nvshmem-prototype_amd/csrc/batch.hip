@@ -389,6 +389,48 @@ extern "C" void nvshmemx_batch_plan_graph_launch(nvshmemx_batch_plan_t p, hipStr
     NVSHMEMI_HIP(hipGraphLaunch(p->exec, s));
 }
 
+/* Judge between plans built for the same exchange: each plan is launched
+ * once to warm up, then `reps` rounds launch every plan once in turn, each
+ * launch bracketed by HIP events on `s`; the plan with the smallest median
+ * launch time wins.  Every launch executes the plan's puts (idempotent: the
+ * same sources to the same targets), so the caller judges when the sources
+ * hold what the exchange sends.  ms_out (optional, nplans entries) receives
+ * each plan's median. */
+extern "C" int nvshmemx_batch_plan_judge(const nvshmemx_batch_plan_t *plans, int nplans, int reps, hipStream_t s,
+                                         float *ms_out) {
+    require_init("nvshmemx_batch_plan_judge");
+    if (!plans || nplans < 1 || reps < 1) NVSHMEMI_FATAL("nvshmemx_batch_plan_judge: bad arguments");
+    for (int i = 0; i < nplans; ++i)
+        if (!plans[i]) NVSHMEMI_FATAL("nvshmemx_batch_plan_judge: plan %d is null", i);
+    for (int i = 0; i < nplans; ++i) plan_launch(plans[i], s);
+    const size_t nev = (size_t)nplans * (size_t)reps;
+    std::vector<hipEvent_t> ev(2 * nev);
+    for (hipEvent_t &e : ev) NVSHMEMI_HIP(hipEventCreate(&e));
+    for (int r = 0; r < reps; ++r)
+        for (int i = 0; i < nplans; ++i) {
+            const size_t k = (size_t)r * (size_t)nplans + (size_t)i;
+            NVSHMEMI_HIP(hipEventRecord(ev[2 * k], s));
+            plan_launch(plans[i], s);
+            NVSHMEMI_HIP(hipEventRecord(ev[2 * k + 1], s));
+        }
+    NVSHMEMI_HIP(hipEventSynchronize(ev.back()));
+    int best = 0;
+    float best_ms = 0.0f;
+    std::vector<float> t((size_t)reps);
+    for (int i = 0; i < nplans; ++i) {
+        for (int r = 0; r < reps; ++r) {
+            const size_t k = (size_t)r * (size_t)nplans + (size_t)i;
+            NVSHMEMI_HIP(hipEventElapsedTime(&t[(size_t)r], ev[2 * k], ev[2 * k + 1]));
+        }
+        std::sort(t.begin(), t.end());
+        const float med = t[(size_t)reps / 2];
+        if (ms_out) ms_out[i] = med;
+        if (i == 0 || med < best_ms) best = i, best_ms = med;
+    }
+    for (hipEvent_t &e : ev) NVSHMEMI_HIP(hipEventDestroy(e));
+    return best;
+}
+
 extern "C" size_t nvshmemx_batch_plan_pieces(nvshmemx_batch_plan_t p) { return p ? p->npieces : 0; }
 extern "C" size_t nvshmemx_batch_plan_bytes(nvshmemx_batch_plan_t p) { return p ? p->total_bytes : 0; }
 

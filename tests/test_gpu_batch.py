@@ -148,3 +148,33 @@ def test_batch_unaligned_ragged(setup, hip, oracle):
     plan = _run_plan(lib, hip, arr, n, choices, None)
     lib.nvshmemx_batch_plan_destroy(plan)
     _check(lib, hip, base, want)
+
+
+def test_batch_plan_judge_picks_the_faster_plan(setup, hip, oracle):
+    """the judge times every candidate plan of one exchange on the device and
+    returns the fastest: a plan confined to one workgroup must lose to the
+    default plan, in either position; its medians are positive and ordered
+    like its choice; the destinations hold the exchange afterwards"""
+    lib, base, images = setup
+    reqs = na.gen_requests(lib, 21, 512, NPES, SRC, DST)
+    arr = na.make_requests(lib, reqs, base, 0, SRC)
+    want = _expected(oracle, images, reqs)
+    _reset_dst(lib, hip, base)
+    n = len(arr)
+    fast, slow = C.c_void_p(), C.c_void_p()
+    lib.nvshmemx_batch_plan_create(arr, n, None, None, C.byref(fast))
+    one_wg = na.BatchLaunch(1, 256, 2, na.BATCH_SHARED)
+    lib.nvshmemx_batch_plan_create(arr, n, None, C.byref(one_wg), C.byref(slow))
+    try:
+        for order, fast_at in (((slow, fast), 1), ((fast, slow), 0)):
+            plans = (C.c_void_p * 2)(*order)
+            ms = (C.c_float * 2)()
+            best = lib.nvshmemx_batch_plan_judge(plans, 2, 3, None, ms)
+            assert best == fast_at, list(ms)
+            assert 0 < ms[fast_at] < ms[1 - fast_at]
+        hip.sync()
+        assert lib.nvshmemx_get_last_error() == 0
+        _check(lib, hip, base, want)
+    finally:
+        lib.nvshmemx_batch_plan_destroy(fast)
+        lib.nvshmemx_batch_plan_destroy(slow)
