@@ -70,9 +70,40 @@ NVSHMEMI_DEF_G(double, double, NVSHMEMI_OP_GET_DOUBLE)
 /* A single lane's copy is bound by round trips, not bandwidth.  When target
  * and source agree modulo 16, the 16-byte vectors go in blocks of 16, then 8,
  * then 4 loads in flight, and the last few one by one, so neither long nor
- * short puts wait on more round trips than they need; other alignments take
- * the general copy. */
+ * short puts wait on more round trips than they need; the unaligned head and
+ * the tail (under 16 bytes each) are each one round trip: all their loads are
+ * issued before the first store.  Other alignments take the general copy. */
 namespace {
+/* n < 16 bytes, any alignment: byte accesses */
+__device__ __forceinline__ void lane_bytes(char *d, const char *s, size_t n) {
+    unsigned char r[15];
+#pragma unroll
+    for (int i = 0; i < 15; ++i)
+        if ((size_t)i < n) r[i] = (unsigned char)s[i];
+#pragma unroll
+    for (int i = 0; i < 15; ++i)
+        if ((size_t)i < n) d[i] = (char)r[i];
+}
+/* n < 16 bytes that end (`up`) or start (!up) on a 16-byte boundary of both
+ * sides: pieces of 1, 2, 4, 8 bytes (ascending) or 8, 4, 2, 1 (descending),
+ * each naturally aligned, so a 4-byte element moves as one dword */
+template <bool UP>
+__device__ __forceinline__ void lane_edge(char *d, const char *s, size_t n) {
+    const size_t o1 = UP ? 0 : (n & 14), o2 = UP ? (n & 1) : (n & 12), o4 = UP ? (n & 3) : (n & 8),
+                 o8 = UP ? (n & 7) : 0;
+    unsigned long long a8 = 0;
+    unsigned int a4 = 0;
+    unsigned short a2 = 0;
+    unsigned char a1 = 0;
+    if (n & 8) a8 = *(const unsigned long long *)(s + o8);
+    if (n & 4) a4 = *(const unsigned int *)(s + o4);
+    if (n & 2) a2 = *(const unsigned short *)(s + o2);
+    if (n & 1) a1 = *(const unsigned char *)(s + o1);
+    if (n & 8) *(unsigned long long *)(d + o8) = a8;
+    if (n & 4) *(unsigned int *)(d + o4) = a4;
+    if (n & 2) *(unsigned short *)(d + o2) = a2;
+    if (n & 1) *(unsigned char *)(d + o1) = a1;
+}
 template <int K>
 __device__ __forceinline__ void lane_block(u32x4 *d, const u32x4 *s) {
     u32x4 r[K];
@@ -89,7 +120,10 @@ __device__ __forceinline__ void lane_copy(void *dstv, const void *srcv, size_t n
         return;
     }
     const split<16> sp(dst, nbytes);
-    for (size_t i = 0; i < sp.head; ++i) dst[i] = src[i];
+    if (sp.head) {
+        if (((uintptr_t)(dst + sp.head) & 15) == 0) lane_edge<true>(dst, src, sp.head);
+        else lane_bytes(dst, src, sp.head); /* the whole copy ends short of a boundary */
+    }
     u32x4 *d = (u32x4 *)(dst + sp.head);
     const u32x4 *s = (const u32x4 *)(src + sp.head);
     size_t i = 0;
@@ -97,7 +131,7 @@ __device__ __forceinline__ void lane_copy(void *dstv, const void *srcv, size_t n
     if (i + 8 <= sp.nvec) lane_block<8>(d + i, s + i), i += 8;
     if (i + 4 <= sp.nvec) lane_block<4>(d + i, s + i), i += 4;
     for (; i < sp.nvec; ++i) d[i] = s[i];
-    for (size_t t = 0; t < sp.tail; ++t) dst[sp.tail_start + t] = src[sp.tail_start + t];
+    if (sp.tail) lane_edge<false>(dst + sp.tail_start, src + sp.tail_start, sp.tail);
 }
 } // namespace
 
