@@ -325,6 +325,7 @@ extern "C" int nvshmemx_initialized(void) { return g.initialized ? 1 : 0; }
 extern "C" int nvshmemx_mode(void) { return g.mode; }
 extern "C" int nvshmemx_wgs_per_pe(void) { return g.wgs_per_pe; }
 extern "C" size_t nvshmemx_heap_size(void) { return g.heap_size; }
+extern "C" int nvshmemx_heap_kind(void) { return g.fine_grained ? NVSHMEMX_HEAP_FINE : NVSHMEMX_HEAP_COARSE; }
 extern "C" const char *nvshmemx_bootstrap_name(void) { return g.boot ? g.boot->name() : "none"; }
 
 extern "C" int nvshmemx_get_last_error(void) {

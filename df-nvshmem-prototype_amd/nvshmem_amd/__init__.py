@@ -61,6 +61,7 @@ SIGNATURES = {
     "nvshmemx_mode": (_I, []),
     "nvshmemx_wgs_per_pe": (_I, []),
     "nvshmemx_heap_size": (_SZ, []),
+    "nvshmemx_heap_kind": (_I, []),
     "nvshmemx_bootstrap_name": (C.c_char_p, []),
     "nvshmemx_wallclock_khz": (_I, []),
     "nvshmemx_get_last_error": (_I, []),

@@ -43,6 +43,11 @@ int nvshmemx_initialized(void);
 int nvshmemx_mode(void);
 int nvshmemx_wgs_per_pe(void);
 size_t nvshmemx_heap_size(void);
+/* How the symmetric heap was allocated (NVSHMEM_HEAP_KIND): coarse-grained
+ * hipMalloc memory (the default) or fine-grained hipExtMallocWithFlags memory. */
+#define NVSHMEMX_HEAP_COARSE 0
+#define NVSHMEMX_HEAP_FINE 1
+int nvshmemx_heap_kind(void);
 const char *nvshmemx_bootstrap_name(void);
 int nvshmemx_wallclock_khz(void);
 

@@ -32,7 +32,8 @@ def main():
     lib, hip = na.load(), na.Hip()
     lib.nvstart_pes()
     me, n = lib.nvshmem_my_pe(), lib.nvshmem_n_pes()
-    out = {"pe": me, "npes": n, "mode": lib.nvshmemx_mode(), "bootstrap": lib.nvshmemx_bootstrap_name().decode()}
+    out = {"pe": me, "npes": n, "mode": lib.nvshmemx_mode(), "bootstrap": lib.nvshmemx_bootstrap_name().decode(),
+           "heap_kind": lib.nvshmemx_heap_kind()}
     if scenario == "putget":
         import oracle_py
         orc = oracle_py.Oracle()

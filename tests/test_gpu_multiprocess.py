@@ -74,3 +74,18 @@ def test_multiprocess_batch(npes, discipline):
     assert sum(o["requests_issued"] for o in outs) == 64 * npes
     assert all(o["batch_ok"] for o in outs), outs
     assert all(o["errors"] == 0 for o in outs)
+
+
+@pytest.mark.parametrize("scenario,npes,args", [("putget", 2, ()), ("pingpong", 2, (2000,)), ("barrier", 4, (100,))])
+def test_multiprocess_fine_grained_heap(scenario, npes, args):
+    """the same exchanges with every PE's heap in fine-grained device memory
+    (NVSHMEM_HEAP_KIND=fine), exported and opened through IPC"""
+    outs = run_job(npes, scenario, *args, extra={"NVSHMEM_HEAP_KIND": "fine"})
+    for o in outs:
+        assert o["errors"] == 0 and o["heap_kind"] == 1, o
+        if scenario == "putget":
+            assert o["put_ok"] and o["get_ok"], o
+        if scenario == "pingpong":
+            assert o["final_flag"] == args[0], o
+        if scenario == "barrier":
+            assert o["violations"] == 0 and o["slots"] == [args[0]] * npes, o

@@ -455,3 +455,16 @@ def test_put_get_2d(local_pes, hip, oracle, path):
             _check_region(hip, oracle, lib, back, want, span_s + 2 * GUARD)
         finally:
             hip.free(back)
+
+
+def test_fine_grained_heap_local():
+    """NVSHMEM_HEAP_KIND=fine: the heap comes from fine-grained device memory;
+    grid and per-thread puts and gets stay bit-exact (aligned and misaligned),
+    p + wait round trips and barrier episodes complete without a time-out.
+    Runs in a process of its own (tests/fine_heap_worker.py), so the
+    fine-grained heap is allocated and freed away from the other tests."""
+    env = {k: v for k, v in os.environ.items() if not k.startswith("NVSHMEM_")}
+    p = subprocess.run([sys.executable, str(Path(__file__).with_name("fine_heap_worker.py"))], env=env,
+                       capture_output=True, text=True, timeout=180)
+    assert p.returncode == 0 and "fine heap ok" in p.stdout, (p.stdout[-1000:], p.stderr[-2000:])
+    print([line for line in p.stdout.splitlines() if line.startswith("fine heap:")][-1])
