@@ -134,6 +134,15 @@ def test_grid_putget_sizes(local_pes, hip, oracle, dtype, op):
         nb *= 2
 
 
+@pytest.mark.parametrize("op", ["put", "get"])
+def test_grid_putget_1GiB(local_pes, hip, oracle, op):
+    """the largest size of the put/get workload, 1 GiB of float32, on the
+    library's default launch shape for it, against the oracle byte for byte"""
+    lib = local_pes.start(2, heap="2100M")
+    fn = _run_put if op == "put" else _run_get
+    fn(lib, hip, oracle, path="grid", dtype="float", nelem=(1 << 30) // 4, toff=0, soff=0, seed=_seed("1GiB", op))
+
+
 @pytest.mark.parametrize("mode,block,unroll,streams,workgroups", [
     (0, 256, 4, 0, 7), (0, 1024, 8, 0, 0), (1, 512, 8, 0, 13), (1, 64, 1, 0, 0),
     (2, 512, 1, 2, 0), (2, 256, 2, 1, 0), (2, 1024, 1, 4, 0), (2, 64, 8, 3, 0), (2, 256, 4, 5, 3)])
