@@ -158,8 +158,8 @@ def learned_worker(n_batches, train_steps):
        path is timed on the first 10;
     4. 100 batches each of 16 and 128 puts from the same size distribution,
        where how many puts share the launch decides the best split.
-    Every batch is also judged: the plans of the three choices timed on the
-    device by nvshmemx_batch_plan_judge, the fastest kept ("judged").
+    Every batch is also selected on the device: the plans of the three choices timed on the
+    device by nvshmemx_batch_plan_select, the fastest kept ("selected").
     Prints one JSON line."""
     import ctypes as C
     import numpy as np
@@ -198,22 +198,22 @@ def learned_worker(n_batches, train_steps):
         lib.nvshmemx_batch_plan_destroy(plan)
         return ms
 
-    def judged_plan(arr, n, choice_sets):
-        """plans for every choice set, judged on the device (3 rounds); the
+    def selected_plan(arr, n, choice_sets):
+        """plans for every choice set, timed on the device (3 rounds); the
         winner timed like the others; returns (ms, index of the winner)"""
         plans = [C.c_void_p() for _ in choice_sets]
         for p, ch in zip(plans, choice_sets):
             lib.nvshmemx_batch_plan_create(arr, n, ch, None, C.byref(p))
-        best = lib.nvshmemx_batch_plan_judge((C.c_void_p * len(plans))(*[p.value for p in plans]), len(plans), 3,
+        best = lib.nvshmemx_batch_plan_select((C.c_void_p * len(plans))(*[p.value for p in plans]), len(plans), 3,
                                              None, None)
         ms = timed(hip, lambda: lib.nvshmemx_batch_plan_launch(plans[best], None), 2, 1)
         for p in plans:
             lib.nvshmemx_batch_plan_destroy(p)
         return ms, best
 
-    judge_names = ("mlp", "table", "rule")
-    rec = {"mlp": [], "table": [], "rule": [], "judged": [], "one_piece": [], "per_call": []}
-    picks = {k: 0 for k in judge_names}
+    select_names = ("mlp", "table", "rule")
+    rec = {"mlp": [], "table": [], "rule": [], "selected": [], "one_piece": [], "per_call": []}
+    picks = {k: 0 for k in select_names}
     choose_ms = []
     bad = 0
     t0 = time.time()
@@ -232,9 +232,9 @@ def learned_worker(n_batches, train_steps):
         rec["mlp"].append(timed_plan(arr, n, ch_mlp))
         rec["table"].append(timed_plan(arr, n, ch_tab))
         rec["rule"].append(timed_plan(arr, n, None))
-        jm, jb = judged_plan(arr, n, (ch_mlp, ch_tab, None))
-        rec["judged"].append(jm)
-        picks[judge_names[jb]] += 1
+        jm, jb = selected_plan(arr, n, (ch_mlp, ch_tab, None))
+        rec["selected"].append(jm)
+        picks[select_names[jb]] += 1
         rec["one_piece"].append(timed_plan(arr, n, one))
         if b < 10:
             rec["per_call"].append(timed(hip, lambda: lib.nvshmemx_put_requests_on_stream(arr, n, None), 1, 0))
@@ -244,8 +244,8 @@ def learned_worker(n_batches, train_steps):
     # input is what the untuned rule cannot see
     small = {}
     for nreq in (16, 128):
-        rs = {"mlp": [], "table": [], "rule": [], "judged": []}
-        spicks = {k: 0 for k in judge_names}
+        rs = {"mlp": [], "table": [], "rule": [], "selected": []}
+        spicks = {k: 0 for k in select_names}
         for b in range(100):
             reqs = na.gen_requests(lib, 5000 + 1000 * nreq + b, nreq, BATCH_NPES, BATCH_SRC, BATCH_DST)
             arr = na.make_requests(lib, reqs, base, 0, BATCH_SRC)
@@ -255,14 +255,14 @@ def learned_worker(n_batches, train_steps):
             rs["mlp"].append(timed_plan(arr, nreq, ch_mlp))
             rs["table"].append(timed_plan(arr, nreq, ch_tab))
             rs["rule"].append(timed_plan(arr, nreq, None))
-            jm, jb = judged_plan(arr, nreq, (ch_mlp, ch_tab, None))
-            rs["judged"].append(jm)
-            spicks[judge_names[jb]] += 1
+            jm, jb = selected_plan(arr, nreq, (ch_mlp, ch_tab, None))
+            rs["selected"].append(jm)
+            spicks[select_names[jb]] += 1
         small[str(nreq)] = {"mean_ms": {k: round(float(np.mean(v)), 4) for k, v in rs.items()},
                             "mlp_over_table": round(float(np.mean(rs["mlp"]) / np.mean(rs["table"])), 4),
                             "mlp_gain_over_rule": round(float(np.mean(rs["rule"]) / np.mean(rs["mlp"])), 4),
-                            "judged_gain_over_rule": round(float(np.mean(rs["rule"]) / np.mean(rs["judged"])), 4),
-                            "judge_picks": spicks}
+                            "selected_gain_over_rule": round(float(np.mean(rs["rule"]) / np.mean(rs["selected"])), 4),
+                            "selection_picks": spicks}
     out["fewer_puts"] = small
     out["eval_seconds"] = round(time.time() - t0, 1)
     m, t, r = (np.array(rec[k]) for k in ("mlp", "table", "rule"))
@@ -275,9 +275,9 @@ def learned_worker(n_batches, train_steps):
         "mlp_gain_over_one_piece": round(float(np.mean(rec["one_piece"]) / np.mean(m)), 4),
         "mlp_speedup_vs_per_call": round(float(np.mean(pc) / np.mean(m[:len(pc)])), 2),
         "mlp_choose_ms_per_batch": round(float(np.median(choose_ms)), 3),
-        "judged_gain_over_rule": round(float(np.mean(r) / np.mean(rec["judged"])), 4),
-        "judged_over_table": round(float(np.mean(rec["judged"]) / np.mean(t)), 4),
-        "judge_picks": picks,
+        "selected_gain_over_rule": round(float(np.mean(r) / np.mean(rec["selected"])), 4),
+        "selected_over_table": round(float(np.mean(rec["selected"]) / np.mean(t)), 4),
+        "selection_picks": picks,
         "bitexact": bad == 0,
         "errors": lib.nvshmemx_get_last_error(),
     })

@@ -389,19 +389,19 @@ extern "C" void nvshmemx_batch_plan_graph_launch(nvshmemx_batch_plan_t p, hipStr
     NVSHMEMI_HIP(hipGraphLaunch(p->exec, s));
 }
 
-/* Judge between plans built for the same exchange: each plan is launched
+/* Select the fastest of plans built for the same exchange: each plan is launched
  * once to warm up, then `reps` rounds launch every plan once in turn, each
  * launch bracketed by HIP events on `s`; the plan with the smallest median
  * launch time wins.  Every launch executes the plan's puts (idempotent: the
- * same sources to the same targets), so the caller judges when the sources
+ * same sources to the same targets), so the caller selects when the sources
  * hold what the exchange sends.  ms_out (optional, nplans entries) receives
  * each plan's median. */
-extern "C" int nvshmemx_batch_plan_judge(const nvshmemx_batch_plan_t *plans, int nplans, int reps, hipStream_t s,
+extern "C" int nvshmemx_batch_plan_select(const nvshmemx_batch_plan_t *plans, int nplans, int reps, hipStream_t s,
                                          float *ms_out) {
-    require_init("nvshmemx_batch_plan_judge");
-    if (!plans || nplans < 1 || reps < 1) NVSHMEMI_FATAL("nvshmemx_batch_plan_judge: bad arguments");
+    require_init("nvshmemx_batch_plan_select");
+    if (!plans || nplans < 1 || reps < 1) NVSHMEMI_FATAL("nvshmemx_batch_plan_select: bad arguments");
     for (int i = 0; i < nplans; ++i)
-        if (!plans[i]) NVSHMEMI_FATAL("nvshmemx_batch_plan_judge: plan %d is null", i);
+        if (!plans[i]) NVSHMEMI_FATAL("nvshmemx_batch_plan_select: plan %d is null", i);
     for (int i = 0; i < nplans; ++i) plan_launch(plans[i], s);
     const size_t nev = (size_t)nplans * (size_t)reps;
     std::vector<hipEvent_t> ev(2 * nev);

@@ -132,7 +132,7 @@ SIGNATURES.update({
     "nvshmemx_batch_plan_launch": (None, [_P, _P]),
     "nvshmemx_batch_plan_capture": (_I, [_P, _I]),
     "nvshmemx_batch_plan_graph_launch": (None, [_P, _P]),
-    "nvshmemx_batch_plan_judge": (_I, [_P, _I, _I, _P, _P]),
+    "nvshmemx_batch_plan_select": (_I, [_P, _I, _I, _P, _P]),
     "nvshmemx_batch_plan_pieces": (_SZ, [_P]),
     "nvshmemx_batch_plan_bytes": (_SZ, [_P]),
     "nvshmemx_batch_plan_destroy": (None, [_P]),

@@ -2,7 +2,7 @@
 oracle, so the oracle test compares two implementations.
 
 pcg32_kat.json holds the published pcg32-demo vector of the PCG minimal C
-reference (srandom(42, 54), round 1); this script re-derives it and refuses to
+reference (srandom(42, 54), the first block the demo prints); this script re-derives it and refuses to
 write anything if it disagrees.
 """
 import hashlib
@@ -33,7 +33,7 @@ def pcg32(n, seed, seq):
 def main():
     assert [hex(x) for x in pcg32(6, 42, 54)] == PUBLISHED
     (HERE / "pcg32_kat.json").write_text(json.dumps(
-        {"source": "PCG minimal C reference, pcg32-demo output, round 1", "seed": 42, "seq": 54,
+        {"source": "PCG minimal C reference, pcg32-demo output, first block", "seed": 42, "seq": 54,
          "outputs": PUBLISHED}, indent=1) + "\n")
     cases = []
     for n, seed, seq in [(1000, 42, 54), (4097, 7, 54), (65536, 0x5EED, 7), (3, 0, 0), (10000, 2**63 + 5, 2**40)]:

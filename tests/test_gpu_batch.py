@@ -150,8 +150,8 @@ def test_batch_unaligned_ragged(setup, hip, oracle):
     _check(lib, hip, base, want)
 
 
-def test_batch_plan_judge_picks_the_faster_plan(setup, hip, oracle):
-    """the judge times every candidate plan of one exchange on the device and
+def test_batch_plan_select_picks_the_faster_plan(setup, hip, oracle):
+    """the selection times every candidate plan of one exchange on the device and
     returns the fastest: a plan confined to one workgroup must lose to the
     default plan, in either position; its medians are positive and ordered
     like its choice; the destinations hold the exchange afterwards"""
@@ -169,7 +169,7 @@ def test_batch_plan_judge_picks_the_faster_plan(setup, hip, oracle):
         for order, fast_at in (((slow, fast), 1), ((fast, slow), 0)):
             plans = (C.c_void_p * 2)(*order)
             ms = (C.c_float * 2)()
-            best = lib.nvshmemx_batch_plan_judge(plans, 2, 3, None, ms)
+            best = lib.nvshmemx_batch_plan_select(plans, 2, 3, None, ms)
             assert best == fast_at, list(ms)
             assert 0 < ms[fast_at] < ms[1 - fast_at]
         hip.sync()

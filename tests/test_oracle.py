@@ -19,7 +19,7 @@ GOLDEN = Path(__file__).resolve().parent / "golden"
 
 def test_pcg32_published_known_answer(oracle):
     """pcg32-demo from the PCG minimal C reference: srandom(42, 54), first six
-    outputs of round 1 (published with the library)."""
+    outputs of the first block the demo prints (published with the library)."""
     kat = json.loads((GOLDEN / "pcg32_kat.json").read_text())
     got = oracle.pcg32(len(kat["outputs"]), kat["seed"], kat["seq"])
     assert [int(x) for x in got] == [int(x, 16) for x in kat["outputs"]]
