@@ -590,7 +590,7 @@ def main():
     ap.add_argument("--counter-pass", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--learned-worker", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--learned-batches", type=int, default=1000)
-    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-steps", type=int, default=12000)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--jacobi-worker", choices=["local", "process"], default=None, help=argparse.SUPPRESS)
