@@ -581,10 +581,12 @@ void nvshmemh_wait_once(int *ivar, int cmp_value, int until, int *out, hipStream
 }
 
 void nvshmemh_traced_ops(int *ints_sym, float *floats_sym, double *doubles_sym, int *flag_sym, int me, int pe,
-                         int reps, int block, hipStream_t s) {
-    /* one workgroup: in local mode it is PE 0, so callers pass me = 0 there */
-    hipLaunchKernelGGL(k_traced_ops, dim3(1), dim3(block), 0, s, ints_sym, floats_sym, doubles_sym, flag_sym, me, pe,
-                       reps);
+                         int reps, int block, int grid, hipStream_t s) {
+    /* `grid` workgroups each issue the sequence from PE me's buffers (in local
+     * mode workgroups 0 .. wgs_per_pe - 1 are PE 0, so callers pass me = 0) */
+    if (grid < 1) NVSHMEMI_FATAL("nvshmemh_traced_ops: grid %d", grid);
+    hipLaunchKernelGGL(k_traced_ops, dim3((unsigned)grid), dim3(block), 0, s, ints_sym, floats_sym, doubles_sym,
+                       flag_sym, me, pe, reps);
     NVSHMEMI_HIP(hipGetLastError());
 }
 

@@ -19,11 +19,11 @@ SEQ = [("put_int", 16), ("put_float", 8), ("get_double", 4), ("put_int", 1), ("g
        ("wait_until_int", 1)]
 
 
-def run(tmp_path, reps, extra=None):
+def run(tmp_path, reps, extra=None, grid=1):
     env = dict(os.environ, NVSHMEM_TRACE_DIR=str(tmp_path), NVSHMEM_HEAP_SIZE="16M")
     env.update(extra or {})
-    p = subprocess.run([sys.executable, str(HERE / "trace_worker.py"), str(reps)], env=env, capture_output=True,
-                       text=True, timeout=120)
+    p = subprocess.run([sys.executable, str(HERE / "trace_worker.py"), str(reps), str(grid)], env=env,
+                       capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stderr[-3000:]
     files = sorted(tmp_path.glob("nvshmem_processed_tracefile_*_0"))
     assert len(files) == 1, (list(tmp_path.iterdir()), p.stdout)
@@ -62,3 +62,23 @@ def test_trace_overflow_is_reported(tmp_path):
     rows = f.read_text().splitlines()[1:]
     assert len(rows) == 8
     assert [r.split(", ")[4] for r in rows] == [s[0] for s in SEQ] + [SEQ[0][0]]
+
+
+def test_trace_many_workgroups(tmp_path):
+    """16 workgroups trace at once: every record is kept (claims on the ring
+    head race, none is lost or doubled), each workgroup's records come out in
+    its issue order with non-decreasing timestamps"""
+    reps, grid = 5, 16
+    stdout, info, f = run(tmp_path, reps, grid=grid)
+    assert info["enabled"] == 1 and info["errors"] == 0
+    rows = [ROW.match(l) for l in f.read_text().splitlines()[1:]]
+    assert all(rows) and len(rows) == grid * len(SEQ) * reps
+    by_block = {}
+    for m in rows:
+        by_block.setdefault(int(m.group(3)), []).append(m)
+    assert sorted(by_block) == list(range(grid))
+    for b, ms in by_block.items():
+        assert [(m.group(5), int(m.group(8))) for m in ms] == SEQ * reps, b
+        ts = [int(m.group(1)) for m in ms]
+        assert ts == sorted(ts), b
+    assert "overflow: 0 conflict: 0" in stdout

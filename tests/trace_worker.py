@@ -1,6 +1,6 @@
 """Runs the traced harness sequence with the tracing build of the library
-(build/trace/libnvshmem_amd.so) in local mode (2 PEs) and prints where the
-processed trace went.  Launched by test_gpu_trace.py in its own process so the
+(build/trace/libnvshmem_amd.so) in local mode (2 PEs, argv[2] workgroups
+issuing it at once, default 1) and prints where the processed trace went.  Launched by test_gpu_trace.py in its own process so the
 tracing and non-tracing builds never share a process."""
 import json
 import sys
@@ -16,8 +16,9 @@ import nvshmem_amd as na  # noqa: E402
 
 def main():
     reps = int(sys.argv[1])
+    grid = int(sys.argv[2]) if len(sys.argv) > 2 else 1
     lib, hip = na.load(trace=True), na.Hip()
-    assert lib.nvshmemx_start_local_pes(2, 1) == 0
+    assert lib.nvshmemx_start_local_pes(2, max(1, grid // 2)) == 0
     ints = lib.nvshmalloc(64 * 4)
     floats = lib.nvshmalloc(64 * 4)
     doubles = lib.nvshmalloc(64 * 8)
@@ -27,7 +28,7 @@ def main():
             hip.memset(lib.nvshmem_ptr(p, pe), 0, n)
         hip.htod(lib.nvshmem_ptr(flag, pe), np.array([7], np.int32))
     hip.sync()
-    lib.nvshmemh_traced_ops(ints, floats, doubles, flag, 0, 1, reps, 64, None)
+    lib.nvshmemh_traced_ops(ints, floats, doubles, flag, 0, 1, reps, 64, grid, None)
     hip.sync()
     err = lib.nvshmemx_get_last_error()
     enabled = lib.nvshmemx_trace_enabled()
