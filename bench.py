@@ -687,6 +687,12 @@ def main():
         if args.layout == "local":
             put_ok = bool(np.array_equal(hip.dtoh(lib.nvshmem_ptr(dst, 1), NBYTES, np.uint32),
                                          hip.dtoh(src, NBYTES, np.uint32)))
+        # the copy's own HBM ceiling on this part: read-only and write-only
+        # streams over the same 1 GiB (after the bit-exact checks: the write
+        # stream overwrites `back`)
+        sink = hip.malloc(64)
+        read_ms = timed(hip, lambda: lib.nvshmemh_stream_read(src, NBYTES, sink, None), 20, 3)
+        write_ms = timed(hip, lambda: lib.nvshmemh_stream_write(back, NBYTES, None), 20, 3)
         lib.nvshmem_barrier_all()                                        # 2
         _log("puts/gets done; ping-pong")
         lib.nvshmem_barrier_all()                                        # 3
@@ -736,7 +742,8 @@ def main():
         result = dict(
             put_ms=put_ms, put_gbps=put_gbps, get_gbps=2 * NBYTES / (get_ms * 1e-3) / 1e9,
             memcpy_gbps=2 * NBYTES / (memcpy_ms * 1e-3) / 1e9, put_ok=put_ok, get_ok=get_ok, errors=err,
-            put_kernel=put_kernel,
+            put_kernel=put_kernel, read_gbps=NBYTES / (read_ms * 1e-3) / 1e9,
+            write_gbps=NBYTES / (write_ms * 1e-3) / 1e9,
             rtt_p50=float(np.percentile(rtt_ns, 50)), rtt_p99=float(np.percentile(rtt_ns, 99)),
             bar_p50=float(np.percentile(bar_ns, 50)), bar_p99=float(np.percentile(bar_ns, 99)), putflag=putflag,
             streams_by_size=streams_by_size)
@@ -776,6 +783,12 @@ def main():
         "cpu_baseline": cpu,
         "get_gbps": round(r["get_gbps"], 1),
         "hip_memcpy_d2d_gbps": round(r["memcpy_gbps"], 1),
+        "hbm_ceilings_gbps": {
+            "read": round(r["read_gbps"], 1), "write": round(r["write_gbps"], 1),
+            "copy_bound": round(2 / (1 / r["read_gbps"] + 1 / r["write_gbps"]), 1),
+            "put_frac_of_copy_bound": round(r["put_gbps"] * (1 / r["read_gbps"] + 1 / r["write_gbps"]) / 2, 4),
+            "pattern": "1 GiB read-only and write-only streams of 16-byte non-temporal accesses; a copy moves every "
+                       "byte once each way, so its bound is the two streams back to back"},
         "bulk_streams_gbps": {"sizes": r["streams_by_size"], "frac_of_peak_min": round(
             min(min(v["put_gbps"], v["get_gbps"]) for v in r["streams_by_size"].values()) / PEAK_HBM_GBPS, 4),
             "pattern": "back-to-back float put (get) launches, PE 0 -> PE 1, buffers rotated over 1 GiB"},
