@@ -172,13 +172,16 @@ __device__ void pingpong_body(int *flag_sym, int me, int peer, int initiator, in
             if (failed()) break;
         } else {
             nvshmem_int_wait(myflag, i - 1);
-            if (failed()) break;
             if (payload_len > 0) {
                 nvshmem_int_put(payload_sym, mypayload, payload_len, peer);
                 nvshmem_quiet();
             }
             nvshmem_int_p(flag_sym, i, peer);
             nvshmem_quiet();
+            /* checked after the reply: the check is a load that misses L2
+             * (the wait ended with an acquire), and before the reply it would
+             * sit inside the initiator's timed round trip */
+            if (failed()) break;
         }
     }
 }
