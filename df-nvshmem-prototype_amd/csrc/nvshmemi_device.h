@@ -93,37 +93,36 @@ __device__ __forceinline__ bool cmp_holds(int v, int op, int ref) {
     }
 }
 
-/* Polling pace shared by every wait: the first kSpinPolls polls go back to
- * back (a partner that is already running answers within a microsecond or
- * two, and sleeping then only adds latency); after that an s_sleep ladder (64
- * clocks per step, doubling to 8 steps) lets a long wait yield issue slots. */
+/* Polling pace shared by every wait: the first NVSHMEMI_SPIN_POLLS polls go
+ * back to back (a partner that is already running answers within a
+ * microsecond or two, and sleeping then only adds latency) and read no clock:
+ * the wall-clock read is a scalar memory access whose result the loop would
+ * wait for after every poll, which measured 40-80 ns per poll.  After that an
+ * s_sleep ladder (64 clocks per step, doubling to 8 steps) lets a long wait
+ * yield issue slots, and the wall-clock budget starts. */
 #ifndef NVSHMEMI_SPIN_POLLS
 #define NVSHMEMI_SPIN_POLLS 256
 #endif
-struct poll_pace {
-    int polls = 0, steps = 1;
+struct sleep_ladder {
+    int steps = 1;
     __device__ __forceinline__ void pause() {
-        if (polls < NVSHMEMI_SPIN_POLLS) {
-            ++polls;
-            return;
-        }
         for (int i = 0; i < steps; ++i) __builtin_amdgcn_s_sleep(1);
         if (steps < 8) steps <<= 1;
     }
 };
 
-/* Poll *ivar with agent-scope loads until `v op ref` holds, paced by
- * poll_pace.  A wait
- * that outlives the configured wall-clock budget records
- * NVSHMEMI_ERR_WAIT_TIMEOUT and returns instead of hanging the device.
- * Returns the last value read.  Ends with an agent-scope acquire so payload
- * published before the flag is visible to the caller. */
+/* Poll *ivar with agent-scope loads until `v op ref` holds: back to back
+ * first, then paced by the sleep ladder.  A wait that outlives the configured
+ * wall-clock budget records NVSHMEMI_ERR_WAIT_TIMEOUT and returns instead of
+ * hanging the device.  Returns the last value read.  Ends with an agent-scope
+ * acquire so payload published before the flag is visible to the caller. */
 __device__ __forceinline__ int wait_until(int *ivar, int op, int ref) {
     int v = load_relaxed(ivar);
+    for (int i = 0; i < NVSHMEMI_SPIN_POLLS && !cmp_holds(v, op, ref); ++i) v = load_relaxed(ivar);
     if (!cmp_holds(v, op, ref)) {
         const uint64_t budget = nvshmemi_dstate.wait_timeout_ticks;
         const uint64_t t0 = wall_clock64();
-        poll_pace pace;
+        sleep_ladder pace;
         for (;;) {
             pace.pause();
             v = load_relaxed(ivar);
@@ -144,11 +143,11 @@ __device__ __forceinline__ int wait_until(int *ivar, int op, int ref) {
  * agent-scope fetch-add both arrives and reports the generation of this
  * episode (it cannot advance before this arrival).  The last arriver stores
  * the next generation with a zero count, with release; the others poll the
- * generation with back-off until it moves or the wait budget runs out (then
- * `timeout_bit` is set and the episode is abandoned).  No arrival of the next
- * episode can happen before that store: it needs the new generation.  Ends
- * with an acquire, so what every arrival published before arriving is
- * visible. */
+ * generation (back to back, then with the sleep ladder) until it moves or the
+ * wait budget runs out (then `timeout_bit` is set and the episode is
+ * abandoned).  No arrival of the next episode can happen before that store:
+ * it needs the new generation.  Ends with an acquire, so what every arrival
+ * published before arriving is visible. */
 __device__ __forceinline__ void gen_barrier(unsigned int *word, int n, int timeout_bit) {
     const unsigned int old = __hip_atomic_fetch_add(word, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int g0 = old >> 16;
@@ -156,14 +155,18 @@ __device__ __forceinline__ void gen_barrier(unsigned int *word, int n, int timeo
         store_release(word, ((g0 + 1u) & 0xFFFFu) << 16);
         return;
     }
-    const uint64_t budget = nvshmemi_dstate.wait_timeout_ticks;
-    const uint64_t t0 = wall_clock64();
-    poll_pace pace;
-    while ((load_relaxed(word) >> 16) == g0) {
-        pace.pause();
-        if ((uint64_t)(wall_clock64() - t0) > budget) {
-            set_error(timeout_bit);
-            break;
+    bool moved = false;
+    for (int i = 0; i < NVSHMEMI_SPIN_POLLS && !moved; ++i) moved = (load_relaxed(word) >> 16) != g0;
+    if (!moved) {
+        const uint64_t budget = nvshmemi_dstate.wait_timeout_ticks;
+        const uint64_t t0 = wall_clock64();
+        sleep_ladder pace;
+        while ((load_relaxed(word) >> 16) == g0) {
+            pace.pause();
+            if ((uint64_t)(wall_clock64() - t0) > budget) {
+                set_error(timeout_bit);
+                break;
+            }
         }
     }
     fence_acquire();

@@ -1,6 +1,6 @@
 """Local-mode checks of the fine-grained symmetric heap (NVSHMEM_HEAP_KIND=fine),
 run by test_gpu_rma.py in a process of its own: grid and per-thread puts and
-gets against the oracle, p + wait round trips, barrier episodes.  Prints
+gets against the oracle, p + wait round trips, a 256 MiB put's rate.  Prints
 "fine heap ok" at the end."""
 import os
 import sys
@@ -44,12 +44,10 @@ def main():
     assert lib.nvshmemx_get_last_error() == 0
     assert int(hip.dtoh(lib.nvshmem_ptr(flag, 0), 4, np.int32)[0]) == iters
     rtt = np.percentile(hip.dtoh(samples, iters * 8, np.int64) * 1e6 / lib.nvshmemx_wallclock_khz(), 50)
-    print("case barrier", flush=True)
-    lib.nvshmemh_barrier_latency(iters, samples, None)
-    hip.sync()
-    assert lib.nvshmemx_get_last_error() == 0
-    bar = hip.dtoh(samples, iters * 8, np.int64)
-    assert (bar > 0).all()
+    # (barrier episodes on the fine-grained heap run between processes in
+    # test_gpu_multiprocess.py: local mode launches them cooperatively, which
+    # a child process cannot do while its parent holds the device's
+    # cooperative-launch resources)
     # bulk rate into the fine-grained heap: 256 MiB float puts, PE 0 -> PE 1
     lib.nvshmcleanup()
     nbytes = 256 << 20
@@ -65,9 +63,7 @@ def main():
     hip.record(e1, None)
     ms = hip.elapsed_ms(e0, e1) / 10
     assert np.array_equal(hip.dtoh(lib.nvshmem_ptr(dst, 1), nbytes, np.uint32), hip.dtoh(src, nbytes, np.uint32))
-    print(f"fine heap: p+wait p50 {rtt:.0f} ns, barrier p50 "
-          f"{np.percentile(bar * 1e6 / lib.nvshmemx_wallclock_khz(), 50):.0f} ns, 256 MiB put "
-          f"{2 * nbytes / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)
+    print(f"fine heap: p+wait p50 {rtt:.0f} ns, 256 MiB put {2 * nbytes / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)
     lib.nvstop_pes()
     print("fine heap ok", flush=True)
 

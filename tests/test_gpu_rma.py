@@ -469,7 +469,8 @@ def test_put_get_2d(local_pes, hip, oracle, path):
 def test_fine_grained_heap_local():
     """NVSHMEM_HEAP_KIND=fine: the heap comes from fine-grained device memory;
     grid and per-thread puts and gets stay bit-exact (aligned and misaligned),
-    p + wait round trips and barrier episodes complete without a time-out.
+    p + wait round trips complete without a time-out (barrier episodes on
+    this heap: test_gpu_multiprocess.py).
     Runs in a process of its own (tests/fine_heap_worker.py), so the
     fine-grained heap is allocated and freed away from the other tests."""
     env = {k: v for k, v in os.environ.items() if not k.startswith("NVSHMEM_")}
