@@ -4,23 +4,27 @@
  * 2x2x2 grid, N^3 float64 cells per PE).  A caller of the library, exported
  * with C entry points (nvshmemh_jacobi_*) for the test and benchmark drivers.
  *
- * Per PE (symmetric allocations): u and v, each (N+2)^3 doubles with a
- * one-cell halo, index ((i * (N+2)) + j) * (N+2) + k, k fastest; and a receive
- * area recv[2][6][N*N] (parity x face slot).
+ * Per PE (symmetric allocations): u and v, each (N+2)^2 rows of N+32 doubles
+ * holding a one-cell halo (cell (i, j, k), 0 <= i, j, k <= N+1, at
+ * ((i * (N+2)) + j) * (N+32) + k + 15, so interior cell k = 1 starts a
+ * 128-byte line when N is a multiple of 16); and a receive area
+ * recv[2][6][N*N] (parity x face slot).
  *
- * One step:
- *   1. pack+put: every boundary plane of u is written straight into the
- *      neighbour's receive slot for that side (translated symmetric address),
- *      so packing and the put are one pass; faces are strided in x and y, so
- *      the reference's contiguous put would need a separate pack;
- *   2. barrier: nvshmem_barrier_all_offload (separate processes) or the
- *      kernel boundary (all PEs in one launch, local mode);
- *   3. unpack: receive slots into u's halo;
- *   4. stencil: v = 0.25 * u + 0.125 * ((((( u[x-1] + u[x+1]) + u[y-1]) +
- *      u[y+1]) + u[z-1]) + u[z+1]) in this exact order (the host replay uses
- *      the same order; multiplications by powers of two are exact, so the
- *      result is bit-exact with or without FMA contraction);
- *   5. swap u, v.
+ * The update, in this exact order (the host replay uses the same order;
+ * multiplications by powers of two are exact, so the result is bit-exact with
+ * or without FMA contraction):
+ *     v = 0.25 * u + 0.125 * ((((( u[x-1] + u[x+1]) + u[y-1]) + u[y+1]) + u[z-1]) + u[z+1])
+ *
+ * A step of nvshmemh_jacobi_run is one fused launch for every PE of the
+ * process: the halo is read straight from this PE's receive slots of the
+ * step's parity, and every boundary value of v is also written into the
+ * receive slot of the next parity on the neighbour across that face (a
+ * translated symmetric address), so the next step's halo exchange rides on
+ * this step's stores; then the barrier (nvshmem_barrier_all_offload between
+ * processes, the kernel boundary in local mode).  A run starts with one
+ * pack+put of u's faces.  The separate kernels (pack+put, unpack into u's
+ * halo, stencil reading the halo from u) stay exported.  Faces are strided in
+ * x and y, so the reference's contiguous put would need a separate pack.
  * Receive slots alternate by step parity, so one barrier per step suffices: a
  * PE can run at most one step ahead of a neighbour and then writes the other
  * parity.
