@@ -35,6 +35,23 @@ using namespace nvshmemi;
     } while (0)
 #endif
 
+/* A PE outside [0, npes): the call does nothing and records
+ * NVSHMEMI_ERR_BAD_PE (read by nvshmemx_get_last_error) instead of touching
+ * the address an empty slot of the base table would give (the reference does
+ * not check, nvshmem_device.cu:445-446). */
+namespace {
+__device__ __forceinline__ bool bad_pe(int pe) {
+    if ((unsigned)pe < (unsigned)nvshmemi_dstate.npes) return false;
+    set_error(NVSHMEMI_ERR_BAD_PE);
+    return true;
+}
+/* translated address for a valid pe, the untranslated one otherwise (the
+ * caller then moves nothing) */
+__device__ __forceinline__ char *bad_pe_target(const void *addr, int pe) {
+    return (unsigned)pe < (unsigned)nvshmemi_dstate.npes ? translate(addr, pe) : (char *)addr;
+}
+} // namespace
+
 /* ------------------------------------------------------------ quiet -- */
 
 extern "C" __device__ void nvshmem_quiet(void) {
@@ -48,6 +65,7 @@ extern "C" __device__ void nvshmem_quiet(void) {
 #define NVSHMEMI_DEF_P(Name, Type, OpType)                                            \
     extern "C" __device__ void nvshmem_##Name##_p(Type *target, const Type value, int pe) { \
         NVSHMEMI_IDEAL();                                                             \
+        if (bad_pe(pe)) return;                                                       \
         NVSHMEMI_TRACE(OpType, (const void *)0, target, 1, pe, 0);                   \
         store_relaxed((Type *)translate(target, pe), value);                          \
     }
@@ -58,6 +76,7 @@ NVSHMEMI_DEF_P(double, double, NVSHMEMI_OP_PUT_DOUBLE)
 #define NVSHMEMI_DEF_G(Name, Type, OpType)                                            \
     extern "C" __device__ Type nvshmem_##Name##_g(Type *source, int pe) {               \
         NVSHMEMI_IDEAL((Type)0);                                                      \
+        if (bad_pe(pe)) return (Type)0;                                               \
         NVSHMEMI_TRACE(OpType, (const void *)0, source, 1, pe, 0);                   \
         return load_relaxed((const Type *)translate(source, pe));                     \
     }
@@ -138,6 +157,7 @@ __device__ __forceinline__ void lane_copy(void *dstv, const void *srcv, size_t n
 #define NVSHMEMI_DEF_PUT(Name, Type, OpType)                                                \
     extern "C" __device__ void nvshmem_##Name##_put(Type *target, Type *source, int len, int pe) { \
         NVSHMEMI_IDEAL();                                                                   \
+        if (bad_pe(pe)) return;                                                             \
         NVSHMEMI_TRACE(OpType, source, target, len, pe, 0);                                \
         if (len <= 0) return;                                                               \
         lane_copy(translate(target, pe), source, (size_t)len * sizeof(Type));             \
@@ -149,6 +169,7 @@ NVSHMEMI_DEF_PUT(double, double, NVSHMEMI_OP_PUT_DOUBLE)
 #define NVSHMEMI_DEF_GET(Name, Type, OpType)                                                \
     extern "C" __device__ void nvshmem_##Name##_get(Type *target, Type *source, int len, int pe) { \
         NVSHMEMI_IDEAL();                                                                   \
+        if (bad_pe(pe)) return;                                                             \
         NVSHMEMI_TRACE(OpType, target, source, len, pe, 0);                                \
         if (len <= 0) return;                                                               \
         lane_copy(target, translate(source, pe), (size_t)len * sizeof(Type));             \
@@ -185,7 +206,9 @@ extern "C" __device__ int nvshmemx_int_wait_until_cmp(int *ivar, int cmp, int va
 
 extern "C" __device__ int nvshmemx_my_pe_d(void) { return my_pe(); }
 extern "C" __device__ int nvshmemx_n_pes_d(void) { return nvshmemi_dstate.npes; }
-extern "C" __device__ void *nvshmemx_ptr_d(const void *addr, int pe) { return translate(addr, pe); }
+extern "C" __device__ void *nvshmemx_ptr_d(const void *addr, int pe) {
+    return (unsigned)pe < (unsigned)nvshmemi_dstate.npes ? (void *)translate(addr, pe) : nullptr;
+}
 
 namespace {
 /* rank of the calling lane among the wave's active lanes, and their count */
@@ -207,30 +230,30 @@ __device__ __forceinline__ void block_copy(void *dst, const void *src, size_t nb
 } // namespace
 
 extern "C" __device__ void nvshmemx_putmem_wave(void *target, const void *source, size_t nbytes, int pe) {
-    wave_copy(translate(target, pe), source, nbytes);
+    if (!bad_pe(pe)) wave_copy(translate(target, pe), source, nbytes);
 }
 extern "C" __device__ void nvshmemx_getmem_wave(void *target, const void *source, size_t nbytes, int pe) {
-    wave_copy(target, translate(source, pe), nbytes);
+    if (!bad_pe(pe)) wave_copy(target, translate(source, pe), nbytes);
 }
 extern "C" __device__ void nvshmemx_putmem_block(void *target, const void *source, size_t nbytes, int pe) {
-    block_copy(translate(target, pe), source, nbytes);
+    if (!bad_pe(pe)) block_copy(translate(target, pe), source, nbytes);
 }
 extern "C" __device__ void nvshmemx_getmem_block(void *target, const void *source, size_t nbytes, int pe) {
-    block_copy(target, translate(source, pe), nbytes);
+    if (!bad_pe(pe)) block_copy(target, translate(source, pe), nbytes);
 }
 
 #define NVSHMEMI_DEF_COOP(Name, Type)                                                              \
     extern "C" __device__ void nvshmemx_##Name##_put_wave(Type *t, const Type *s, int len, int pe) { \
-        if (len > 0) wave_copy(translate(t, pe), s, (size_t)len * sizeof(Type));                   \
+        if (len > 0 && !bad_pe(pe)) wave_copy(translate(t, pe), s, (size_t)len * sizeof(Type));    \
     }                                                                                              \
     extern "C" __device__ void nvshmemx_##Name##_get_wave(Type *t, const Type *s, int len, int pe) { \
-        if (len > 0) wave_copy(t, translate(s, pe), (size_t)len * sizeof(Type));                   \
+        if (len > 0 && !bad_pe(pe)) wave_copy(t, translate(s, pe), (size_t)len * sizeof(Type));    \
     }                                                                                              \
     extern "C" __device__ void nvshmemx_##Name##_put_block(Type *t, const Type *s, int len, int pe) { \
-        block_copy(translate(t, pe), s, len > 0 ? (size_t)len * sizeof(Type) : 0);                 \
+        if (!bad_pe(pe)) block_copy(translate(t, pe), s, len > 0 ? (size_t)len * sizeof(Type) : 0); \
     }                                                                                              \
     extern "C" __device__ void nvshmemx_##Name##_get_block(Type *t, const Type *s, int len, int pe) { \
-        block_copy(t, translate(s, pe), len > 0 ? (size_t)len * sizeof(Type) : 0);                 \
+        if (!bad_pe(pe)) block_copy(t, translate(s, pe), len > 0 ? (size_t)len * sizeof(Type) : 0); \
     }
 NVSHMEMI_DEF_COOP(int, int)
 NVSHMEMI_DEF_COOP(float, float)
@@ -259,17 +282,17 @@ __device__ __forceinline__ void rows_block(char *dst, size_t dpitch, const char 
                                                            size_t n, int pe) {                                   \
         unsigned r, c;                                                                                            \
         active_lanes(r, c);                                                                                       \
-        strided((Type *)translate(t, pe), s, tst, sst, n, r, c);                                                  \
+        if (!bad_pe(pe)) strided((Type *)translate(t, pe), s, tst, sst, n, r, c);                                 \
     }                                                                                                             \
     extern "C" __device__ void nvshmemx_##Name##_iput_block(Type *t, const Type *s, ptrdiff_t tst, ptrdiff_t sst, \
                                                             size_t n, int pe) {                                  \
-        strided((Type *)translate(t, pe), s, tst, sst, n, flat_tid(), block_threads());                           \
+        if (!bad_pe(pe)) strided((Type *)translate(t, pe), s, tst, sst, n, flat_tid(), block_threads());          \
         fence_release();                                                                                          \
         __syncthreads();                                                                                          \
     }                                                                                                             \
     extern "C" __device__ void nvshmemx_##Name##_iget_block(Type *t, const Type *s, ptrdiff_t tst, ptrdiff_t sst, \
                                                             size_t n, int pe) {                                  \
-        strided(t, (const Type *)translate(s, pe), tst, sst, n, flat_tid(), block_threads());                     \
+        if (!bad_pe(pe)) strided(t, (const Type *)translate(s, pe), tst, sst, n, flat_tid(), block_threads());    \
         fence_release();                                                                                          \
         __syncthreads();                                                                                          \
     }
@@ -279,11 +302,13 @@ NVSHMEMI_DEF_STRIDED(double, double)
 
 extern "C" __device__ void nvshmemx_putmem_2d_block(void *target, size_t tpitch, const void *source, size_t spitch,
                                                     size_t width, size_t height, int pe) {
-    rows_block(translate(target, pe), tpitch, (const char *)source, spitch, width, height);
+    if (bad_pe(pe)) height = 0; /* every thread still meets the workgroup barrier */
+    rows_block(bad_pe_target(target, pe), tpitch, (const char *)source, spitch, width, height);
 }
 extern "C" __device__ void nvshmemx_getmem_2d_block(void *target, size_t tpitch, const void *source, size_t spitch,
                                                     size_t width, size_t height, int pe) {
-    rows_block((char *)target, tpitch, translate(source, pe), spitch, width, height);
+    if (bad_pe(pe)) height = 0;
+    rows_block((char *)target, tpitch, bad_pe_target(source, pe), spitch, width, height);
 }
 
 /* ------------------------------------------------ workgroup barrier -- */

@@ -25,7 +25,7 @@ extern "C" {
 /* sticky device error bits returned by nvshmemx_get_last_error() */
 #define NVSHMEMX_ERR_WAIT_TIMEOUT 0x1    /* a device wait outlived NVSHMEM_WAIT_TIMEOUT_MS */
 #define NVSHMEMX_ERR_BARRIER_TIMEOUT 0x2 /* an offload barrier outlived the same budget */
-#define NVSHMEMX_ERR_BAD_PE 0x4
+#define NVSHMEMX_ERR_BAD_PE 0x4          /* a device call named a PE outside [0, npes); it did nothing */
 #define NVSHMEMX_ERR_TRACE_OVERFLOW 0x8
 #define NVSHMEMX_ERR_TRACE_CONFLICT 0x10
 

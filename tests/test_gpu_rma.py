@@ -478,3 +478,44 @@ def test_fine_grained_heap_local():
                        capture_output=True, text=True, timeout=180)
     assert p.returncode == 0 and "fine heap ok" in p.stdout, (p.stdout[-1000:], p.stderr[-2000:])
     print([line for line in p.stdout.splitlines() if line.startswith("fine heap:")][-1])
+
+
+@pytest.mark.parametrize("pe", [2, -1, 127])
+def test_out_of_range_pe_is_refused_on_device(local_pes, hip, pe):
+    """device calls that name a PE outside [0, npes) (2 PEs here; 127 is a
+    slot of the base table that no PE fills) move nothing and record
+    NVSHMEMX_ERR_BAD_PE: per-thread put and get, p and g (g returns 0), and the
+    wavefront- and workgroup-cooperative puts"""
+    lib = local_pes.start(2)
+    n = 64
+    sym = lib.nvshmalloc(n * 4)
+    src = lib.nvshmalloc(n * 4)
+    vals = hip.malloc(n * 4)
+    out = hip.malloc(n * 4)
+    try:
+        for q in (0, 1):
+            hip.memset(lib.nvshmem_ptr(sym, q), 0x5A, n * 4)
+            hip.memset(lib.nvshmem_ptr(src, q), 0x33, n * 4)
+        hip.memset(out, 0x77, n * 4)
+        hip.sync()
+        calls = [
+            lambda: lib.nvshmemh_thread_putget(0, 1, sym, lib.nvshmem_ptr(src, 0), n, 16, pe, None),
+            lambda: lib.nvshmemh_thread_putget(1, 1, lib.nvshmem_ptr(src, 0), sym, n, 16, pe, None),
+            lambda: lib.nvshmemh_pg(0, 0, sym, vals, out, n, pe, None),
+            lambda: lib.nvshmemh_pg(1, 0, sym, vals, out, n, pe, None),
+            lambda: lib.nvshmemh_coop_putget(0, 0, sym, lib.nvshmem_ptr(src, 0), n * 4, 64, pe, 256, None),
+            lambda: lib.nvshmemh_coop_putget(0, 1, sym, lib.nvshmem_ptr(src, 0), n * 4, 128, pe, 256, None),
+        ]
+        for call in calls:
+            lib.nvshmemx_clear_errors()
+            call()
+            hip.sync()
+            assert lib.nvshmemx_get_last_error() == na.ERR_BAD_PE
+        for q in (0, 1):
+            assert (hip.dtoh(lib.nvshmem_ptr(sym, q), n * 4) == 0x5A).all()
+            assert (hip.dtoh(lib.nvshmem_ptr(src, q), n * 4) == 0x33).all()
+        assert (hip.dtoh(out, n * 4, np.int32) == 0).all()      # g of a bad PE returns 0
+    finally:
+        lib.nvshmemx_clear_errors()
+        hip.free(vals)
+        hip.free(out)
