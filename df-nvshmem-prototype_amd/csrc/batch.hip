@@ -39,20 +39,9 @@
 #include "nvshmemx.h"
 #include "nvshmemi_device.h"
 #include "nvshmemi_host.h"
+#include "batch_plan.h"
 
 namespace nvshmemi {
-
-struct piece_t {
-    const char *src;
-    char *dst;
-    unsigned long long nbytes;
-    unsigned int vec;
-    unsigned int pad;
-};
-
-/* per-piece charge in the LPT cost, in bytes: about one access latency of a
- * wavefront streaming 4 x 1 KiB (0 to 64 KiB measured the same) */
-constexpr double kPieceCharge = 4096.0;
 
 template <int NT>
 __device__ __forceinline__ void copy_piece(const piece_t &p, unsigned lane, unsigned n) {
@@ -75,74 +64,6 @@ __global__ void __launch_bounds__(1024) k_batch(const piece_t *__restrict__ piec
     for (unsigned int j = wave_off[w]; j < wave_end; ++j) copy_piece<NT>(pieces[j], threadIdx.x & 63u, 64);
 }
 
-/* one request's pieces under a choice; boundaries on kPieceQuantum offsets */
-static void split_request(const char *src, char *dst, size_t nbytes, const nvshmemx_batch_choice_t &ch,
-                          std::vector<piece_t> &out) {
-    if (nbytes == 0) return;
-    const size_t quantum = kPieceQuantum;
-    size_t want = (size_t)std::max(1, std::min(ch.pieces, 256));
-    if (ch.mode == 0) want *= 4; /* a workgroup's worth of wavefronts per piece */
-    const size_t max_parts = std::max<size_t>(1, nbytes / quantum);
-    const size_t parts = std::min(want, max_parts);
-    size_t per = (nbytes + parts - 1) / parts;
-    per = (per + quantum - 1) / quantum * quantum;
-    const unsigned vec = ch.vec_bytes >= 16 ? 16 : (ch.vec_bytes >= 8 ? 8 : 4);
-    for (size_t off = 0; off < nbytes; off += per) {
-        const size_t n = std::min(per, nbytes - off);
-        out.push_back(piece_t{src + off, dst + off, (unsigned long long)n, vec, 0});
-    }
-}
-
-/* LPT: pieces (any order) onto `workers` bins; returns per-worker lists.
- * `base_load[w]` is load a worker already carries; `share` divides a piece's
- * bytes among the lanes of a workgroup worker's wavefronts. */
-static std::vector<std::vector<piece_t>> lpt(std::vector<piece_t> pieces, int workers, size_t share,
-                                            std::vector<double> &load) {
-    std::vector<std::vector<piece_t>> lists((size_t)workers);
-    std::stable_sort(pieces.begin(), pieces.end(),
-                     [](const piece_t &a, const piece_t &b) { return a.nbytes > b.nbytes; });
-    typedef std::pair<double, int> item;
-    std::priority_queue<item, std::vector<item>, std::greater<item>> heap;
-    for (int w = 0; w < workers; ++w) heap.push(item(load[(size_t)w], w));
-    const double charge = kPieceCharge;
-    for (const piece_t &p : pieces) {
-        item top = heap.top();
-        heap.pop();
-        lists[(size_t)top.second].push_back(p);
-        top.first += (double)(p.nbytes / share) + charge;
-        load[(size_t)top.second] = top.first;
-        heap.push(top);
-    }
-    return lists;
-}
-
-/* Source order: pieces sorted by source address and cut into `workers`
- * consecutive runs of (nearly) equal cost, run w to worker w.  The hardware
- * dispatches workgroups in index order, so the workgroups running at any
- * moment read neighbouring sources: puts whose sources overlap (or sit in the
- * same heap) then share cache lines instead of each streaming its own.  The
- * last run ends within one piece of the mean (runs close on the first piece
- * that reaches their share). */
-static std::vector<std::vector<piece_t>> source_runs(std::vector<piece_t> pieces, int workers, size_t share,
-                                                    std::vector<double> &load) {
-    std::vector<std::vector<piece_t>> lists((size_t)workers);
-    std::stable_sort(pieces.begin(), pieces.end(),
-                     [](const piece_t &a, const piece_t &b) { return a.src < b.src; });
-    double total = 0;
-    for (const piece_t &p : pieces) total += (double)(p.nbytes / share) + kPieceCharge;
-    const double target = total / (double)workers;
-    double acc = 0;
-    size_t w = 0;
-    for (const piece_t &p : pieces) {
-        const double c = (double)(p.nbytes / share) + kPieceCharge;
-        lists[w].push_back(p);
-        load[w] += c;
-        acc += c;
-        if (acc >= target * (double)(w + 1) && w + 1 < (size_t)workers) ++w;
-    }
-    return lists;
-}
-
 } // namespace nvshmemi
 
 using namespace nvshmemi;
@@ -163,8 +84,7 @@ struct nvshmemx_batch_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t capture_stream = nullptr;
-    std::vector<piece_t> host_pieces; /* staging, reused across rebuilds */
-    std::vector<unsigned int> host_wg_off, host_wave_off;
+    plan_layout layout; /* host arrays, reused across rebuilds */
 };
 
 namespace nvshmemi {
@@ -174,15 +94,6 @@ static void ensure_capacity(void **d, size_t &cap, size_t need, size_t elem) {
     if (*d) NVSHMEMI_HIP(hipFree(*d));
     cap = std::max<size_t>(need, 1024);
     NVSHMEMI_HIP(hipMalloc(d, cap * elem));
-}
-
-/* Fisher-Yates with a fixed xorshift64 stream: a reproducible order */
-static void scatter(std::vector<piece_t> &v) {
-    unsigned long long x = 0x9E3779B97F4A7C15ull;
-    for (size_t i = v.size(); i > 1; --i) {
-        x ^= x << 13, x ^= x >> 7, x ^= x << 17;
-        std::swap(v[i - 1], v[(size_t)(x % i)]);
-    }
 }
 
 /* (Re)build plan `p` from resolved requests; device buffers are reused when
@@ -196,81 +107,13 @@ static void scatter(std::vector<piece_t> &v) {
 void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, const nvshmemx_batch_choice_t *choices,
                const nvshmemx_batch_choice_t *one_choice, int nsets, int wgs_per_set, int block, int nontemporal,
                int assign) {
-    if (block < 64 || block > 1024 || block % 64) NVSHMEMI_FATAL("batch block size %d invalid", block);
-    const int waves_per_wg = block / 64;
-    std::vector<std::vector<piece_t>> wg(nsets), wave(nsets);
-    size_t total = 0;
-    for (size_t i = 0; i < reqs.size(); ++i) {
-        const resolved_req &r = reqs[i];
-        nvshmemx_batch_choice_t ch;
-        if (one_choice) ch = *one_choice;
-        else if (choices) ch = choices[i];
-        else nvshmemx_batch_default_choice(r.nbytes, 4, &ch);
-        split_request(r.src, r.dst, r.nbytes, ch, ch.mode ? wg[r.set] : wave[r.set]);
-        total += r.nbytes;
-    }
-    if (wgs_per_set <= 0) {
-        /* grid sized to the work: one workgroup per workgroup piece or per
-         * workgroup's worth of wavefront pieces, at most the default grid */
-        size_t most = 1;
-        for (int s = 0; s < nsets; ++s)
-            most = std::max(most, wg[s].size() + (wave[s].size() + waves_per_wg - 1) / waves_per_wg);
-        wgs_per_set = (int)std::min<size_t>(std::max(1, kBatchWorkgroups / nsets), most);
-    }
-    const size_t G = (size_t)nsets * wgs_per_set, W = G * waves_per_wg;
-    const size_t wps = (size_t)wgs_per_set * waves_per_wg; /* wavefront workers per set */
-    std::vector<piece_t> &all = p->host_pieces;
-    all.clear();
-    std::vector<unsigned int> &wo = p->host_wg_off, &vo = p->host_wave_off;
-    wo.assign(G + 1, 0);
-    vo.assign(W + 1, 0);
-    if (assign == ASSIGN_LPT || assign == ASSIGN_SOURCE_ORDER) {
-        std::vector<std::vector<piece_t>> wg_lists(G), wave_lists(W);
-        for (int s = 0; s < nsets; ++s) {
-            const size_t g0 = (size_t)s * wgs_per_set, w0 = g0 * waves_per_wg;
-            std::vector<double> wg_load((size_t)wgs_per_set, 0.0);
-            std::vector<std::vector<piece_t>> a = assign == ASSIGN_LPT
-                                                      ? lpt(wg[s], wgs_per_set, (size_t)waves_per_wg, wg_load)
-                                                      : source_runs(wg[s], wgs_per_set, (size_t)waves_per_wg, wg_load);
-            /* a wavefront starts its own pieces after its workgroup's */
-            std::vector<double> wave_load(wps);
-            for (size_t w = 0; w < wps; ++w) wave_load[w] = wg_load[w / waves_per_wg];
-            std::vector<std::vector<piece_t>> b = lpt(wave[s], (int)wps, 1, wave_load);
-            for (size_t i = 0; i < a.size(); ++i) wg_lists[g0 + i] = std::move(a[i]);
-            for (size_t i = 0; i < b.size(); ++i) wave_lists[w0 + i] = std::move(b[i]);
-        }
-        for (size_t i = 0; i < G; ++i) {
-            wo[i] = (unsigned)all.size();
-            all.insert(all.end(), wg_lists[i].begin(), wg_lists[i].end());
-        }
-        wo[G] = (unsigned)all.size();
-        for (size_t i = 0; i < W; ++i) {
-            vo[i] = (unsigned)all.size();
-            all.insert(all.end(), wave_lists[i].begin(), wave_lists[i].end());
-        }
-        vo[W] = (unsigned)all.size();
-    } else {
-        /* worker i of a set takes pieces i, i + workers, i + 2 * workers, ...
-         * of the set's list, written straight into the flat layout */
-        size_t npieces = 0;
-        for (int s = 0; s < nsets; ++s) {
-            if (assign == ASSIGN_SCATTERED) scatter(wg[s]), scatter(wave[s]);
-            npieces += wg[s].size() + wave[s].size();
-        }
-        all.reserve(npieces);
-        for (int s = 0; s < nsets; ++s)
-            for (size_t i = 0; i < (size_t)wgs_per_set; ++i) {
-                wo[(size_t)s * wgs_per_set + i] = (unsigned)all.size();
-                for (size_t j = i; j < wg[s].size(); j += (size_t)wgs_per_set) all.push_back(wg[s][j]);
-            }
-        wo[G] = (unsigned)all.size();
-        for (int s = 0; s < nsets; ++s)
-            for (size_t i = 0; i < wps; ++i) {
-                vo[(size_t)s * wps + i] = (unsigned)all.size();
-                for (size_t j = i; j < wave[s].size(); j += wps) all.push_back(wave[s][j]);
-            }
-        vo[W] = (unsigned)all.size();
-    }
+    plan_layout &L = p->layout;
+    build_layout(reqs, choices, one_choice, nsets, wgs_per_set, block, assign, L);
+    wgs_per_set = L.wgs_per_set;
+    const size_t G = (size_t)nsets * wgs_per_set, W = G * (size_t)(block / 64);
+    const std::vector<piece_t> &all = L.pieces;
+    const std::vector<unsigned int> &wo = L.wg_off, &vo = L.wave_off;
+    const size_t total = L.total_bytes;
     if (all.size() >= 0xFFFFFFF0u) NVSHMEMI_FATAL("batch of %zu pieces is too large", all.size());
     ensure_capacity((void **)&p->d_pieces, p->piece_capacity, all.size(), sizeof(piece_t));
     ensure_capacity((void **)&p->d_wg_off, p->wg_capacity, G + 1, sizeof(unsigned int));
@@ -311,16 +154,8 @@ void plan_launch(nvshmemx_batch_plan *p, hipStream_t s) {
 } // namespace nvshmemi
 
 extern "C" void nvshmemx_batch_default_choice(size_t nbytes, int elem_bytes, nvshmemx_batch_choice_t *out) {
-    /* the rule used when no tuned choice is given: ~256 KiB per workgroup
-     * piece, wavefront pieces below 64 KiB, widest access the element allows */
-    nvshmemx_batch_choice_t c;
-    c.mode = nbytes >= (64u << 10) ? 1 : 0;
-    const size_t per = c.mode ? (256u << 10) : (16u << 10);
-    size_t p = (nbytes + per - 1) / per;
-    c.pieces = (int)std::max<size_t>(1, std::min<size_t>(p, 256));
-    c.vec_bytes = 16;
     (void)elem_bytes;
-    *out = c;
+    *out = default_choice(nbytes);
 }
 
 static int source_pe(const void *src) {
