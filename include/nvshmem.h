@@ -53,7 +53,9 @@ void nvshmcleanup(void);
 void nvshmem_barrier_all(void);
 
 /* Stream-ordered device barrier; the host returns immediately
- * (reference nvshmem.cu:869-878, kernel nvshmem_kernels.cu:56-102). */
+ * (reference nvshmem.cu:869-878, kernel nvshmem_kernels.cu:56-102).  With
+ * local (workgroup-partitioned) PEs it does nothing: every PE lives in the
+ * same kernels, which the stream already orders. */
 void nvshmem_barrier_all_offload(void);
 
 /* Address of the symmetric object `local` on PE `pe`, for direct loads and
