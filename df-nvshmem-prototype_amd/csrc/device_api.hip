@@ -35,15 +35,28 @@ using namespace nvshmemi;
     } while (0)
 #endif
 
-/* A PE outside [0, npes): the call does nothing and records
- * NVSHMEMI_ERR_BAD_PE (read by nvshmemx_get_last_error) instead of touching
- * the address an empty slot of the base table would give (the reference does
- * not check, nvshmem_device.cu:445-446). */
+/* A PE outside [0, npes), or a symmetric range that leaves the heap, makes
+ * the call do nothing and records NVSHMEMI_ERR_BAD_PE or
+ * NVSHMEMI_ERR_BAD_ADDRESS (read by nvshmemx_get_last_error) instead of
+ * touching an address outside every PE's mapping (the reference does not
+ * check, nvshmem_device.cu:445-446). */
 namespace {
 __device__ __forceinline__ bool bad_pe(int pe) {
     if ((unsigned)pe < (unsigned)nvshmemi_dstate.npes) return false;
     set_error(NVSHMEMI_ERR_BAD_PE);
     return true;
+}
+__device__ __forceinline__ bool bad_target(const void *sym, size_t nbytes, int pe) {
+    if (bad_pe(pe)) return true;
+    const nvshmemi_device_state_t &s = nvshmemi_dstate;
+    const uint64_t off = ((uint64_t)((const char *)sym - s.anchor)) & s.offset_mask;
+    if (off < s.heap_bytes && nbytes <= s.heap_bytes - off) return false;
+    set_error(NVSHMEMI_ERR_BAD_ADDRESS);
+    return true;
+}
+/* bytes a strided transfer of n elements of `elem` bytes spans */
+__device__ __forceinline__ size_t strided_extent(size_t n, ptrdiff_t stride, size_t elem) {
+    return n ? ((n - 1) * (size_t)stride + 1) * elem : 0;
 }
 /* translated address for a valid pe, the untranslated one otherwise (the
  * caller then moves nothing) */
@@ -65,7 +78,7 @@ extern "C" __device__ void nvshmem_quiet(void) {
 #define NVSHMEMI_DEF_P(Name, Type, OpType)                                            \
     extern "C" __device__ void nvshmem_##Name##_p(Type *target, const Type value, int pe) { \
         NVSHMEMI_IDEAL();                                                             \
-        if (bad_pe(pe)) return;                                                       \
+        if (bad_target(target, sizeof(Type), pe)) return;                             \
         NVSHMEMI_TRACE(OpType, (const void *)0, target, 1, pe, 0);                   \
         store_relaxed((Type *)translate(target, pe), value);                          \
     }
@@ -76,7 +89,7 @@ NVSHMEMI_DEF_P(double, double, NVSHMEMI_OP_PUT_DOUBLE)
 #define NVSHMEMI_DEF_G(Name, Type, OpType)                                            \
     extern "C" __device__ Type nvshmem_##Name##_g(Type *source, int pe) {               \
         NVSHMEMI_IDEAL((Type)0);                                                      \
-        if (bad_pe(pe)) return (Type)0;                                               \
+        if (bad_target(source, sizeof(Type), pe)) return (Type)0;                     \
         NVSHMEMI_TRACE(OpType, (const void *)0, source, 1, pe, 0);                   \
         return load_relaxed((const Type *)translate(source, pe));                     \
     }
@@ -157,7 +170,7 @@ __device__ __forceinline__ void lane_copy(void *dstv, const void *srcv, size_t n
 #define NVSHMEMI_DEF_PUT(Name, Type, OpType)                                                \
     extern "C" __device__ void nvshmem_##Name##_put(Type *target, Type *source, int len, int pe) { \
         NVSHMEMI_IDEAL();                                                                   \
-        if (bad_pe(pe)) return;                                                             \
+        if (bad_target(target, len > 0 ? (size_t)len * sizeof(Type) : 0, pe)) return;      \
         NVSHMEMI_TRACE(OpType, source, target, len, pe, 0);                                \
         if (len <= 0) return;                                                               \
         lane_copy(translate(target, pe), source, (size_t)len * sizeof(Type));             \
@@ -169,7 +182,7 @@ NVSHMEMI_DEF_PUT(double, double, NVSHMEMI_OP_PUT_DOUBLE)
 #define NVSHMEMI_DEF_GET(Name, Type, OpType)                                                \
     extern "C" __device__ void nvshmem_##Name##_get(Type *target, Type *source, int len, int pe) { \
         NVSHMEMI_IDEAL();                                                                   \
-        if (bad_pe(pe)) return;                                                             \
+        if (bad_target(source, len > 0 ? (size_t)len * sizeof(Type) : 0, pe)) return;      \
         NVSHMEMI_TRACE(OpType, target, source, len, pe, 0);                                \
         if (len <= 0) return;                                                               \
         lane_copy(target, translate(source, pe), (size_t)len * sizeof(Type));             \
@@ -230,30 +243,34 @@ __device__ __forceinline__ void block_copy(void *dst, const void *src, size_t nb
 } // namespace
 
 extern "C" __device__ void nvshmemx_putmem_wave(void *target, const void *source, size_t nbytes, int pe) {
-    if (!bad_pe(pe)) wave_copy(translate(target, pe), source, nbytes);
+    if (!bad_target(target, nbytes, pe)) wave_copy(translate(target, pe), source, nbytes);
 }
 extern "C" __device__ void nvshmemx_getmem_wave(void *target, const void *source, size_t nbytes, int pe) {
-    if (!bad_pe(pe)) wave_copy(target, translate(source, pe), nbytes);
+    if (!bad_target(source, nbytes, pe)) wave_copy(target, translate(source, pe), nbytes);
 }
 extern "C" __device__ void nvshmemx_putmem_block(void *target, const void *source, size_t nbytes, int pe) {
-    if (!bad_pe(pe)) block_copy(translate(target, pe), source, nbytes);
+    if (!bad_target(target, nbytes, pe)) block_copy(translate(target, pe), source, nbytes);
 }
 extern "C" __device__ void nvshmemx_getmem_block(void *target, const void *source, size_t nbytes, int pe) {
-    if (!bad_pe(pe)) block_copy(target, translate(source, pe), nbytes);
+    if (!bad_target(source, nbytes, pe)) block_copy(target, translate(source, pe), nbytes);
 }
 
 #define NVSHMEMI_DEF_COOP(Name, Type)                                                              \
     extern "C" __device__ void nvshmemx_##Name##_put_wave(Type *t, const Type *s, int len, int pe) { \
-        if (len > 0 && !bad_pe(pe)) wave_copy(translate(t, pe), s, (size_t)len * sizeof(Type));    \
+        if (len > 0 && !bad_target(t, (size_t)len * sizeof(Type), pe))                             \
+            wave_copy(translate(t, pe), s, (size_t)len * sizeof(Type));                            \
     }                                                                                              \
     extern "C" __device__ void nvshmemx_##Name##_get_wave(Type *t, const Type *s, int len, int pe) { \
-        if (len > 0 && !bad_pe(pe)) wave_copy(t, translate(s, pe), (size_t)len * sizeof(Type));    \
+        if (len > 0 && !bad_target(s, (size_t)len * sizeof(Type), pe))                             \
+            wave_copy(t, translate(s, pe), (size_t)len * sizeof(Type));                            \
     }                                                                                              \
     extern "C" __device__ void nvshmemx_##Name##_put_block(Type *t, const Type *s, int len, int pe) { \
-        if (!bad_pe(pe)) block_copy(translate(t, pe), s, len > 0 ? (size_t)len * sizeof(Type) : 0); \
+        if (!bad_target(t, len > 0 ? (size_t)len * sizeof(Type) : 0, pe))                          \
+            block_copy(translate(t, pe), s, len > 0 ? (size_t)len * sizeof(Type) : 0);              \
     }                                                                                              \
     extern "C" __device__ void nvshmemx_##Name##_get_block(Type *t, const Type *s, int len, int pe) { \
-        if (!bad_pe(pe)) block_copy(t, translate(s, pe), len > 0 ? (size_t)len * sizeof(Type) : 0); \
+        if (!bad_target(s, len > 0 ? (size_t)len * sizeof(Type) : 0, pe))                          \
+            block_copy(t, translate(s, pe), len > 0 ? (size_t)len * sizeof(Type) : 0);              \
     }
 NVSHMEMI_DEF_COOP(int, int)
 NVSHMEMI_DEF_COOP(float, float)
@@ -282,17 +299,20 @@ __device__ __forceinline__ void rows_block(char *dst, size_t dpitch, const char 
                                                            size_t n, int pe) {                                   \
         unsigned r, c;                                                                                            \
         active_lanes(r, c);                                                                                       \
-        if (!bad_pe(pe)) strided((Type *)translate(t, pe), s, tst, sst, n, r, c);                                 \
+        if (!bad_target(t, strided_extent(n, tst, sizeof(Type)), pe))                                             \
+            strided((Type *)translate(t, pe), s, tst, sst, n, r, c);                                              \
     }                                                                                                             \
     extern "C" __device__ void nvshmemx_##Name##_iput_block(Type *t, const Type *s, ptrdiff_t tst, ptrdiff_t sst, \
                                                             size_t n, int pe) {                                  \
-        if (!bad_pe(pe)) strided((Type *)translate(t, pe), s, tst, sst, n, flat_tid(), block_threads());          \
+        if (!bad_target(t, strided_extent(n, tst, sizeof(Type)), pe))                                             \
+            strided((Type *)translate(t, pe), s, tst, sst, n, flat_tid(), block_threads());                       \
         fence_release();                                                                                          \
         __syncthreads();                                                                                          \
     }                                                                                                             \
     extern "C" __device__ void nvshmemx_##Name##_iget_block(Type *t, const Type *s, ptrdiff_t tst, ptrdiff_t sst, \
                                                             size_t n, int pe) {                                  \
-        if (!bad_pe(pe)) strided(t, (const Type *)translate(s, pe), tst, sst, n, flat_tid(), block_threads());    \
+        if (!bad_target(s, strided_extent(n, sst, sizeof(Type)), pe))                                             \
+            strided(t, (const Type *)translate(s, pe), tst, sst, n, flat_tid(), block_threads());                 \
         fence_release();                                                                                          \
         __syncthreads();                                                                                          \
     }
@@ -302,12 +322,13 @@ NVSHMEMI_DEF_STRIDED(double, double)
 
 extern "C" __device__ void nvshmemx_putmem_2d_block(void *target, size_t tpitch, const void *source, size_t spitch,
                                                     size_t width, size_t height, int pe) {
-    if (bad_pe(pe)) height = 0; /* every thread still meets the workgroup barrier */
+    if (bad_target(target, height ? (height - 1) * tpitch + width : 0, pe))
+        height = 0; /* every thread still meets the workgroup barrier */
     rows_block(bad_pe_target(target, pe), tpitch, (const char *)source, spitch, width, height);
 }
 extern "C" __device__ void nvshmemx_getmem_2d_block(void *target, size_t tpitch, const void *source, size_t spitch,
                                                     size_t width, size_t height, int pe) {
-    if (bad_pe(pe)) height = 0;
+    if (bad_target(source, height ? (height - 1) * spitch + width : 0, pe)) height = 0;
     rows_block((char *)target, tpitch, bad_pe_target(source, pe), spitch, width, height);
 }
 

@@ -106,6 +106,7 @@ void upload_device_state() {
     d.errors = g.d_errors;
     d.trace = g.trace_dev;
     d.trace_enabled = g.trace_dev != nullptr;
+    d.heap_bytes = g.heap_bytes;
     NVSHMEMI_HIP(hipMemcpyToSymbol(HIP_SYMBOL(nvshmemi_dstate), &d, sizeof(d)));
 }
 

@@ -46,6 +46,7 @@ enum nvshmemi_mode { NVSHMEMI_MODE_PROCESS = 0, NVSHMEMI_MODE_LOCAL = 1 };
 #define NVSHMEMI_ERR_BAD_PE 0x4
 #define NVSHMEMI_ERR_TRACE_OVERFLOW 0x8
 #define NVSHMEMI_ERR_TRACE_CONFLICT 0x10
+#define NVSHMEMI_ERR_BAD_ADDRESS 0x20
 
 struct nvshmemi_trace_dev; /* tracing ring, see nvshmemi_trace.h */
 
@@ -62,6 +63,7 @@ typedef struct nvshmemi_device_state {
     int trace_enabled;
     int pad0;
     struct nvshmemi_trace_dev *trace; /* NULL when tracing is off */
+    uint64_t heap_bytes;              /* bytes of each PE's heap (internal region included) */
 } nvshmemi_device_state_t;
 
 #endif /* NVSHMEMI_STATE_H */

@@ -26,6 +26,7 @@ ERR_BARRIER_TIMEOUT = 0x2
 ERR_BAD_PE = 0x4
 ERR_TRACE_OVERFLOW = 0x8
 ERR_TRACE_CONFLICT = 0x10
+ERR_BAD_ADDRESS = 0x20
 
 
 def lib_path(trace: bool = False, variant: str = "") -> Path:

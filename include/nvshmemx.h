@@ -28,6 +28,7 @@ extern "C" {
 #define NVSHMEMX_ERR_BAD_PE 0x4          /* a device call named a PE outside [0, npes); it did nothing */
 #define NVSHMEMX_ERR_TRACE_OVERFLOW 0x8
 #define NVSHMEMX_ERR_TRACE_CONFLICT 0x10
+#define NVSHMEMX_ERR_BAD_ADDRESS 0x20     /* a device call's symmetric range left the heap; it did nothing */
 
 #define NVSHMEMX_MODE_PROCESS 0 /* one PE per process, peers' heaps IPC-mapped */
 #define NVSHMEMX_MODE_LOCAL 1   /* all PEs in this process, workgroup-partitioned */

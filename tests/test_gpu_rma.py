@@ -519,3 +519,44 @@ def test_out_of_range_pe_is_refused_on_device(local_pes, hip, pe):
         lib.nvshmemx_clear_errors()
         hip.free(vals)
         hip.free(out)
+
+
+def test_out_of_heap_range_is_refused_on_device(local_pes, hip):
+    """device calls whose symmetric range leaves the heap move nothing and
+    record NVSHMEMX_ERR_BAD_ADDRESS: per-thread put, p, g (returns 0) and a
+    wavefront put at an address past the heap's end, and a wavefront put that
+    starts inside the heap but runs past its end.  (Local PEs' slots are
+    padded to a power of two, so every address used here stays inside the
+    allocation even if a check were missing.)"""
+    lib = local_pes.start(2, heap="64M")
+    heap = int(lib.nvshmemx_heap_size())
+    sym = lib.nvshmalloc(4096)
+    src = lib.nvshmalloc(4096)
+    past = sym + heap + (8 << 20)
+    out = hip.malloc(64)
+    try:
+        for q in (0, 1):
+            hip.memset(lib.nvshmem_ptr(sym, q), 0x5A, 4096)
+            hip.memset(lib.nvshmem_ptr(past, q), 0x6B, 4096)
+        hip.memset(out, 0x77, 64)
+        hip.sync()
+        local_src = lib.nvshmem_ptr(src, 0)
+        calls = [
+            lambda: lib.nvshmemh_thread_putget(0, 1, past, local_src, 16, 16, 1, None),
+            lambda: lib.nvshmemh_pg(0, 0, past, out, out, 16, 1, None),
+            lambda: lib.nvshmemh_pg(1, 0, past, out, out, 16, 1, None),
+            lambda: lib.nvshmemh_coop_putget(0, 0, past, local_src, 4096, 4096, 1, 64, None),
+            lambda: lib.nvshmemh_coop_putget(0, 0, sym, local_src, heap + (4 << 20), heap + (4 << 20), 1, 64, None),
+        ]
+        for call in calls:
+            lib.nvshmemx_clear_errors()
+            call()
+            hip.sync()
+            assert lib.nvshmemx_get_last_error() == na.ERR_BAD_ADDRESS
+        for q in (0, 1):
+            assert (hip.dtoh(lib.nvshmem_ptr(sym, q), 4096) == 0x5A).all()
+            assert (hip.dtoh(lib.nvshmem_ptr(past, q), 4096) == 0x6B).all()
+        assert (hip.dtoh(out, 64, np.int32) == 0).all()      # g of a bad range returns 0
+    finally:
+        lib.nvshmemx_clear_errors()
+        hip.free(out)
