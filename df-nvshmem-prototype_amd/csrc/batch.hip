@@ -7,12 +7,12 @@
  *  - each request is split into pieces by its choice (pieces x vector width x
  *    wavefront/workgroup mode); piece boundaries fall on 4 KiB offsets of the
  *    request, so every piece keeps the request's relative alignment;
- *  - the pieces are assigned to workers on the host, longest first, each to the
- *    currently least-loaded worker (LPT list scheduling): workgroup pieces to
- *    workgroups (all the workgroup's wavefronts share a piece), wavefront
- *    pieces to single wavefronts.  A piece costs its bytes plus a fixed
- *    per-piece charge standing for the first access's latency.  Per-worker
- *    lists are laid out contiguously with offset tables;
+ *  - the pieces are dealt to workers on the host (batch_plan.h): workgroup
+ *    pieces to workgroups (all the workgroup's wavefronts share a piece) by
+ *    the plan's discipline, wavefront pieces to single wavefronts longest
+ *    first, each to the least-loaded one (LPT list scheduling).  A piece costs
+ *    its bytes plus a fixed per-piece charge standing for the first access's
+ *    latency.  Per-worker lists are laid out contiguously with offset tables;
  *  - one kernel runs every list: workgroup b copies its workgroup pieces with
  *    all its threads, then each of its wavefronts copies its own wavefront
  *    pieces.  No atomics and no per-launch state, so a plan replays from a
@@ -31,8 +31,6 @@
  * each executing the puts it initiated).
  */
 #include <algorithm>
-#include <queue>
-#include <stdlib.h>
 #include <vector>
 
 #include "nvshmem.h"
