@@ -325,7 +325,7 @@ def jacobi_worker(layout, name=None, counter_pass=False):
     # planes per workgroup: one process's launch covers one PE, so it needs
     # shorter chunks (more workgroups) to fill the device than the launch
     # that covers all eight PEs
-    chunk = int(os.environ.get("JACOBI_CHUNK", "64" if layout == "local" else "16"))
+    chunk = int(os.environ.get("JACOBI_CHUNK", "128" if layout == "local" else "16"))
     if counter_pass:  # five steps and one 1 GiB device copy (the counters' scale)
         lib.nvshmemh_jacobi_run(u, v, recv, n, 5, chunk, 0, None)
         ref = hip.malloc(2 << 30)
