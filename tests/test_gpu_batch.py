@@ -7,7 +7,7 @@ the oracle (memcpy per request on host heap images); every destination byte of
 every PE must match, bit-exact.  Covered: the library's rule, random per-request
 choices (every piece count / access width / mode), the owner discipline
 (PE-initiated), replays of a captured hipGraph, the per-call path, and
-unaligned / ragged requests."""
+unaligned / ragged requests, an empty request list and zero-byte requests."""
 import ctypes as C
 
 import numpy as np
@@ -178,3 +178,51 @@ def test_batch_plan_select_picks_the_faster_plan(setup, hip, oracle):
     finally:
         lib.nvshmemx_batch_plan_destroy(fast)
         lib.nvshmemx_batch_plan_destroy(slow)
+
+
+@pytest.mark.parametrize("variant", ["plan", "graph", "per_call"])
+def test_batch_empty_and_zero_byte_requests(setup, hip, oracle, variant):
+    """an empty request list builds a plan of 0 bytes that launches (and is
+    captured and replayed) without error and moves nothing; zero-byte
+    requests mixed into a batch move nothing, and the other requests land as
+    the oracle says"""
+    lib, base, images = setup
+    _reset_dst(lib, hip, base)
+    if variant != "per_call":
+        empty = C.c_void_p()
+        lib.nvshmemx_batch_plan_create(None, 0, None, None, C.byref(empty))
+        try:
+            assert lib.nvshmemx_batch_plan_bytes(empty) == 0
+            if variant == "graph":
+                lib.nvshmemx_batch_plan_capture(empty, 2)
+                lib.nvshmemx_batch_plan_graph_launch(empty, None)
+            else:
+                lib.nvshmemx_batch_plan_launch(empty, None)
+            hip.sync()
+        finally:
+            lib.nvshmemx_batch_plan_destroy(empty)
+        assert lib.nvshmemx_get_last_error() == 0
+        _check(lib, hip, base, images)  # destinations still all zero
+    reqs = na.gen_requests(lib, 23, 256, NPES, SRC, DST)
+    reqs["nbytes"][::3] = 0
+    arr = na.make_requests(lib, reqs, base, 0, SRC)
+    want = _expected(oracle, images, reqs)
+    n = len(arr)
+    if variant == "per_call":
+        lib.nvshmemx_put_requests_on_stream(arr, n, None)
+        hip.sync()
+    else:
+        plan = C.c_void_p()
+        lib.nvshmemx_batch_plan_create(arr, n, None, None, C.byref(plan))
+        try:
+            assert lib.nvshmemx_batch_plan_bytes(plan) == int(reqs["nbytes"].sum())
+            if variant == "graph":
+                lib.nvshmemx_batch_plan_capture(plan, 2)
+                lib.nvshmemx_batch_plan_graph_launch(plan, None)
+            else:
+                lib.nvshmemx_batch_plan_launch(plan, None)
+            hip.sync()
+        finally:
+            lib.nvshmemx_batch_plan_destroy(plan)
+    assert lib.nvshmemx_get_last_error() == 0
+    _check(lib, hip, base, want)
