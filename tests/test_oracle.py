@@ -5,6 +5,7 @@ pinned by a published known-answer vector, a closed form, an invariant, or a
 brute-force enumeration, chosen so that a plausible slip (off-by-one bound,
 wrong sign, transposed argument) fails.  No GPU needed.
 """
+import ctypes as C
 import itertools
 import json
 from pathlib import Path
@@ -117,6 +118,43 @@ def test_put_get_closed_form(oracle):
     back = np.zeros(777, np.uint8)
     oracle.get(heaps, back, 123, 2)
     assert np.array_equal(back, src)
+
+
+def test_p_g_closed_form(oracle):
+    """p stores exactly the 4 or 8 bytes of the value, little-endian (the
+    GPU's byte order), at the target offset of PE pe's heap and nothing else;
+    g returns those bytes of PE pe's heap (reference nvshmem_device.cu:445-448,
+    469-472).  A NaN's payload bits survive both."""
+    L = oracle.lib
+    rng = np.random.default_rng(12)
+    heaps = [rng.integers(0, 256, 256, dtype=np.uint8) for _ in range(3)]
+    before = [h.copy() for h in heaps]
+
+    def ptrs():
+        return (C.c_void_p * 3)(*[h.ctypes.data for h in heaps])
+
+    def only_changed(pe, lo, hi):
+        for q in range(3):
+            changed = set(np.flatnonzero(heaps[q] != before[q]).tolist())
+            assert changed <= (set(range(lo, hi)) if q == pe else set())
+
+    L.oracle_p32(ptrs(), 17, 0xA1B2C3D4, 1)
+    assert heaps[1][17:21].tolist() == [0xD4, 0xC3, 0xB2, 0xA1]
+    only_changed(1, 17, 21)
+    assert L.oracle_g32(ptrs(), 17, 1) == 0xA1B2C3D4
+    assert L.oracle_g32(ptrs(), 17, 0) == int.from_bytes(before[0][17:21].tobytes(), "little")
+    before = [h.copy() for h in heaps]
+    nan_bits = 0x7FF4000000C0FFEE  # a signalling NaN with a payload
+    L.oracle_p64(ptrs(), 101, nan_bits, 2)
+    assert heaps[2][101:109].tobytes() == nan_bits.to_bytes(8, "little")
+    only_changed(2, 101, 109)
+    assert L.oracle_g64(ptrs(), 101, 2) == nan_bits
+    assert L.oracle_g64(ptrs(), 101, 0) == int.from_bytes(before[0][101:109].tobytes(), "little")
+    # a float's bits travel unchanged through p and g
+    f = np.array([np.float32(-0.0), np.float32(np.inf)], np.float32).view(np.uint32)
+    for bits in f.tolist():
+        L.oracle_p32(ptrs(), 200, bits, 0)
+        assert L.oracle_g32(ptrs(), 200, 0) == bits
 
 
 # ---------------------------------------------------------------- waits
@@ -339,6 +377,9 @@ def test_jacobi_run_conserves_sum_and_stays_in_range(oracle):
     u0 = oracle.jacobi(G, 0, 42)
     assert u0.min() >= 0.0 and u0.max() < 1.0
     assert np.all(u0 * 2.0 ** 53 == np.floor(u0 * 2.0 ** 53))
+    # the initial field is the closed form: cell (x, y, z) holds uniform value
+    # number (x * G + y) * G + z of the seeded stream (word pair 2g, 2g + 1)
+    np.testing.assert_array_equal(u0.ravel(), oracle.uniform01_f64(G ** 3, 42))
     u5 = oracle.jacobi(G, 5, 42)
     assert abs(u5.sum() - u0.sum()) < 1e-12 * u0.sum()
     assert u5.min() >= u0.min() and u5.max() <= u0.max()
@@ -393,3 +434,69 @@ def test_strided_and_2d_closed_forms(oracle):
             want[32 + r * tpitch:32 + r * tpitch + width] = src[r * spitch:r * spitch + width]
         np.testing.assert_array_equal(heaps[0], want)
         assert not heaps[1].any()
+
+
+# ---------------------------------------------------------------- coverage
+
+# Each function oracle.h declares, and the CPU test that pins it.
+PINNED_BY = {
+    "oracle_pcg32_fill": ("test_oracle", "test_pcg32_published_known_answer"),
+    "oracle_pcg32_fill_at": ("test_oracle", "test_pcg32_jump_ahead_matches_the_stream"),
+    "oracle_pcg32_uniform_pm1": ("test_oracle", "test_uniform_pm1_is_exact_and_in_range"),
+    "oracle_pcg32_uniform01_f64": ("test_oracle", "test_uniform01_f64_is_exact_and_in_range"),
+    "oracle_heap_init": ("test_oracle", "test_heap_reference_arithmetic"),
+    "oracle_nvshmalloc": ("test_oracle", "test_heap_reference_arithmetic"),
+    "oracle_nvshmcleanup": ("test_oracle", "test_heap_reference_arithmetic"),
+    "oracle_translate": ("test_oracle", "test_translate_process_mode_is_reference_formula"),
+    "oracle_put": ("test_oracle", "test_put_get_closed_form"),
+    "oracle_get": ("test_oracle", "test_put_get_closed_form"),
+    "oracle_p32": ("test_oracle", "test_p_g_closed_form"),
+    "oracle_p64": ("test_oracle", "test_p_g_closed_form"),
+    "oracle_g32": ("test_oracle", "test_p_g_closed_form"),
+    "oracle_g64": ("test_oracle", "test_p_g_closed_form"),
+    "oracle_iput": ("test_oracle", "test_strided_and_2d_closed_forms"),
+    "oracle_iget": ("test_oracle", "test_strided_and_2d_closed_forms"),
+    "oracle_put_2d": ("test_oracle", "test_strided_and_2d_closed_forms"),
+    "oracle_int_wait_done": ("test_oracle", "test_wait_predicates_follow_readme"),
+    "oracle_int_wait_until_done": ("test_oracle", "test_wait_predicates_follow_readme"),
+    "oracle_cmp_holds": ("test_oracle", "test_wait_predicates_follow_readme"),
+    "oracle_ref_barrier_value": ("test_oracle", "test_reference_barrier_value_sequence"),
+    "oracle_ref_barrier_half": ("test_oracle", "test_reference_barrier_value_sequence"),
+    "oracle_gen_barrier_init": ("test_oracle", "test_generation_barrier_exhaustive"),
+    "oracle_gen_barrier_arrive": ("test_oracle", "test_generation_barrier_exhaustive"),
+    "oracle_gen_barrier_released": ("test_oracle", "test_generation_barrier_exhaustive"),
+    "oracle_trace_header": ("test_oracle", "test_trace_csv_known_answer"),
+    "oracle_trace_op_name": ("test_oracle", "test_trace_csv_known_answer"),
+    "oracle_trace_row": ("test_oracle", "test_trace_csv_known_answer"),
+    "oracle_jacobi_init": ("test_oracle", "test_jacobi_run_conserves_sum_and_stays_in_range"),
+    "oracle_jacobi_step": ("test_oracle", "test_jacobi_plane_waves_are_eigenvectors"),
+    "oracle_jacobi_run": ("test_oracle", "test_jacobi_run_conserves_sum_and_stays_in_range"),
+    "oracle_batch_generate": ("test_batch_cpu", "test_generator_matches_oracle"),
+    "oracle_batch_apply": ("test_batch_cpu", "test_batch_apply_matches_sequential_puts"),
+}
+
+
+def test_every_oracle_function_is_pinned():
+    """every function oracle.h declares has a CPU pinning test, named here, that
+    exists and calls it (directly or through its wrapper in oracle_py.py)"""
+    import re
+    import ast
+    tests_dir = Path(__file__).resolve().parent
+    header = (tests_dir.parent / "oracle" / "oracle.h").read_text()
+    declared = set(re.findall(r"\b(oracle_[a-z0-9_]+)\s*\(", header))
+    assert declared == set(PINNED_BY), (sorted(declared - set(PINNED_BY)), sorted(set(PINNED_BY) - declared))
+    # wrapper methods of oracle_py.Oracle and the C functions each one calls
+    wrappers = {}
+    for node in ast.walk(ast.parse((tests_dir / "oracle_py.py").read_text())):
+        if isinstance(node, ast.FunctionDef) and node.name != "__init__":
+            wrappers[node.name] = set(re.findall(r"oracle_[a-z0-9_]+", ast.unparse(node)))
+    for fn, (module, test) in PINNED_BY.items():
+        tree = ast.parse((tests_dir / f"{module}.py").read_text())
+        defs = {n.name: n for n in tree.body if isinstance(n, ast.FunctionDef)}
+        assert test in defs, f"{fn}: {module}.{test} does not exist"
+        body = ast.unparse(defs[test])
+        called = set(re.findall(r"oracle_[a-z0-9_]+", body))
+        for meth, cfns in wrappers.items():
+            if re.search(r"\." + meth + r"\(", body):
+                called |= cfns
+        assert fn in called, f"{fn}: {module}.{test} does not call it"
