@@ -108,6 +108,12 @@ void oracle_jacobi_init(double *u, int G, uint64_t seed);
 void oracle_jacobi_step(const double *u, double *v, int G);
 int oracle_jacobi_run(double *u, double *v, int G, int steps);
 
+/* ---- cost model (oracle_mlp.c): the 4 -> 32 ReLU -> 32 ReLU -> C network's
+ * forward pass and mean-squared-error gradient in double precision ---- */
+int oracle_mlp_num_params(int c);
+void oracle_mlp_forward(const double *P, int C, const double *X, int N, double *Y);
+double oracle_mlp_loss_grad(const double *P, int C, const double *X, const double *T, int N, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "oracle.h"
+
 #define IN 4
 #define HID 32
 

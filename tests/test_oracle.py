@@ -325,6 +325,7 @@ def test_mlp_forward_closed_forms(oracle):
     """zero weights give the output biases; a single active path is linear"""
     c = 3
     npar = oracle.mlp_num_params(c)
+    assert npar == (32 * 4 + 32) + (32 * 32 + 32) + (c * 32 + c)  # W1 b1 W2 b2 W3 b3
     P = np.zeros(npar)
     b3 = npar - c
     P[b3:] = [1.5, -2.0, 0.25]
@@ -473,6 +474,9 @@ PINNED_BY = {
     "oracle_jacobi_run": ("test_oracle", "test_jacobi_run_conserves_sum_and_stays_in_range"),
     "oracle_batch_generate": ("test_batch_cpu", "test_generator_matches_oracle"),
     "oracle_batch_apply": ("test_batch_cpu", "test_batch_apply_matches_sequential_puts"),
+    "oracle_mlp_num_params": ("test_oracle", "test_mlp_forward_closed_forms"),
+    "oracle_mlp_forward": ("test_oracle", "test_mlp_forward_closed_forms"),
+    "oracle_mlp_loss_grad": ("test_oracle", "test_mlp_gradient_matches_finite_differences"),
 }
 
 
