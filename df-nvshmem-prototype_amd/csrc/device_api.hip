@@ -302,6 +302,13 @@ __device__ __forceinline__ void rows_block(char *dst, size_t dpitch, const char 
         if (!bad_target(t, strided_extent(n, tst, sizeof(Type)), pe))                                             \
             strided((Type *)translate(t, pe), s, tst, sst, n, r, c);                                              \
     }                                                                                                             \
+    extern "C" __device__ void nvshmemx_##Name##_iget_wave(Type *t, const Type *s, ptrdiff_t tst, ptrdiff_t sst,  \
+                                                           size_t n, int pe) {                                   \
+        unsigned r, c;                                                                                            \
+        active_lanes(r, c);                                                                                       \
+        if (!bad_target(s, strided_extent(n, sst, sizeof(Type)), pe))                                             \
+            strided(t, (const Type *)translate(s, pe), tst, sst, n, r, c);                                        \
+    }                                                                                                             \
     extern "C" __device__ void nvshmemx_##Name##_iput_block(Type *t, const Type *s, ptrdiff_t tst, ptrdiff_t sst, \
                                                             size_t n, int pe) {                                  \
         if (!bad_target(t, strided_extent(n, tst, sizeof(Type)), pe))                                             \

@@ -211,27 +211,31 @@ __global__ void k_barrier_latency(int iters, long long *samples) {
 
 /* One workgroup calls one strided / 2-D device transfer: op 0 iput_block,
  * 1 iget_block, 2 iput_wave (wavefront 0 only), 3 putmem_2d_block,
- * 4 getmem_2d_block; dtype 0 int, 1 float, 2 double (ops 0-2). */
+ * 4 getmem_2d_block, 5 iget_wave (wavefront 0 only); dtype 0 int, 1 float,
+ * 2 double (ops 0-2 and 5). */
 template <typename T>
 __device__ void strided_op(int op, T *t, const T *s, ptrdiff_t tst, ptrdiff_t sst, size_t n, int pe);
 template <>
 __device__ void strided_op<int>(int op, int *t, const int *s, ptrdiff_t tst, ptrdiff_t sst, size_t n, int pe) {
     if (op == 0) nvshmemx_int_iput_block(t, s, tst, sst, n, pe);
     else if (op == 1) nvshmemx_int_iget_block(t, s, tst, sst, n, pe);
-    else if (threadIdx.x < 64) nvshmemx_int_iput_wave(t, s, tst, sst, n, pe);
+    else if (op == 2 && threadIdx.x < 64) nvshmemx_int_iput_wave(t, s, tst, sst, n, pe);
+    else if (op == 5 && threadIdx.x < 64) nvshmemx_int_iget_wave(t, s, tst, sst, n, pe);
 }
 template <>
 __device__ void strided_op<float>(int op, float *t, const float *s, ptrdiff_t tst, ptrdiff_t sst, size_t n, int pe) {
     if (op == 0) nvshmemx_float_iput_block(t, s, tst, sst, n, pe);
     else if (op == 1) nvshmemx_float_iget_block(t, s, tst, sst, n, pe);
-    else if (threadIdx.x < 64) nvshmemx_float_iput_wave(t, s, tst, sst, n, pe);
+    else if (op == 2 && threadIdx.x < 64) nvshmemx_float_iput_wave(t, s, tst, sst, n, pe);
+    else if (op == 5 && threadIdx.x < 64) nvshmemx_float_iget_wave(t, s, tst, sst, n, pe);
 }
 template <>
 __device__ void strided_op<double>(int op, double *t, const double *s, ptrdiff_t tst, ptrdiff_t sst, size_t n,
                                    int pe) {
     if (op == 0) nvshmemx_double_iput_block(t, s, tst, sst, n, pe);
     else if (op == 1) nvshmemx_double_iget_block(t, s, tst, sst, n, pe);
-    else if (threadIdx.x < 64) nvshmemx_double_iput_wave(t, s, tst, sst, n, pe);
+    else if (op == 2 && threadIdx.x < 64) nvshmemx_double_iput_wave(t, s, tst, sst, n, pe);
+    else if (op == 5 && threadIdx.x < 64) nvshmemx_double_iget_wave(t, s, tst, sst, n, pe);
 }
 
 __global__ void k_strided_device(int op, int dtype, void *target, const void *source, long long tst, long long sst,

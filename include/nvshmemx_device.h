@@ -74,6 +74,12 @@ __device__ void nvshmemx_float_iput_wave(float *target, const float *source, ptr
                                          size_t nelems, int pe);
 __device__ void nvshmemx_double_iput_wave(double *target, const double *source, ptrdiff_t tst, ptrdiff_t sst,
                                           size_t nelems, int pe);
+__device__ void nvshmemx_int_iget_wave(int *target, const int *source, ptrdiff_t tst, ptrdiff_t sst, size_t nelems,
+                                       int pe);
+__device__ void nvshmemx_float_iget_wave(float *target, const float *source, ptrdiff_t tst, ptrdiff_t sst,
+                                         size_t nelems, int pe);
+__device__ void nvshmemx_double_iget_wave(double *target, const double *source, ptrdiff_t tst, ptrdiff_t sst,
+                                          size_t nelems, int pe);
 __device__ void nvshmemx_int_iput_block(int *target, const int *source, ptrdiff_t tst, ptrdiff_t sst, size_t nelems,
                                         int pe);
 __device__ void nvshmemx_float_iput_block(float *target, const float *source, ptrdiff_t tst, ptrdiff_t sst,

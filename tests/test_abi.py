@@ -91,6 +91,38 @@ int main() {
     assert exe.exists()
 
 
+def test_every_declared_device_function_links(tmp_path):
+    """a kernel that takes the address of every __device__ function declared in
+    include/*.h, compiled -fgpu-rdc and linked against libnvshmem.a: each one
+    is defined in the library's device code (the link fails otherwise)"""
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(hipcc).exists():
+        pytest.skip("hipcc not available")
+    names = sorted(na.declared_symbols() - na.declared_host_symbols())
+    assert set(REFERENCE_DEVICE) <= set(names)
+    refs = "\n".join(f"    sink[{i}] = (unsigned long long)(void *)&{n};" for i, n in enumerate(names))
+    src = tmp_path / "every_device.hip"
+    src.write_text(f"""
+#include <hip/hip_runtime.h>
+#include "nvshmem.h"
+#include "nvshmem_device.h"
+#include "nvshmemx_device.h"
+__global__ void k(unsigned long long *sink) {{
+{refs}
+}}
+int main() {{
+    hipLaunchKernelGGL(k, dim3(1), dim3(1), 0, 0, (unsigned long long *)0);
+    return 0;
+}}
+""")
+    exe = tmp_path / "every_device"
+    cmd = [hipcc, "--offload-arch=gfx950", "-O2", "-fgpu-rdc", f"-I{ROOT / 'include'}", str(src), "--hip-link",
+           f"-L{BUILD}", "-lnvshmem", "-lpthread", "-lrt", "-o", str(exe)]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr[-3000:]
+    assert exe.exists()
+
+
 _BEFORE_INIT = {
     "nvshmalloc": "lib.nvshmalloc(64)",
     "nvshmem_ptr": "lib.nvshmem_ptr(None, 0)",

@@ -407,7 +407,9 @@ def test_iput_iget(local_pes, hip, oracle, path, dtype):
             if path == "host":
                 getattr(lib, f"nvshmemx_{dtype}_iget_on_stream")(back + GUARD, target, sst, tst, n, 1, None)
             else:
-                lib.nvshmemh_strided_device(1, code, back + GUARD, target, sst, tst, n, 0, 0, 0, 0, 1, 256, None)
+                wave = path == "device_wave"
+                lib.nvshmemh_strided_device(5 if wave else 1, code, back + GUARD, target, sst, tst, n, 0, 0, 0, 0, 1,
+                                            64 if wave else 256, None)
             hip.sync()
             assert lib.nvshmemx_get_last_error() == 0
             want = np.full(span_s + 2 * GUARD, CANARY, np.uint8)
