@@ -486,20 +486,24 @@ def test_fine_grained_heap_local():
 def test_out_of_range_pe_is_refused_on_device(local_pes, hip, pe):
     """device calls that name a PE outside [0, npes) (2 PEs here; 127 is a
     slot of the base table that no PE fills) move nothing and record
-    NVSHMEMX_ERR_BAD_PE: per-thread put and get, p and g (g returns 0), and the
-    wavefront- and workgroup-cooperative puts"""
+    NVSHMEMX_ERR_BAD_PE: per-thread put and get, p and g (g returns 0), the
+    wavefront- and workgroup-cooperative puts, and the strided and 2-D puts and
+    gets (whose local side stays untouched too)"""
     lib = local_pes.start(2)
     n = 64
     sym = lib.nvshmalloc(n * 4)
     src = lib.nvshmalloc(n * 4)
     vals = hip.malloc(n * 4)
     out = hip.malloc(n * 4)
+    loc = hip.malloc(n * 4)
     try:
         for q in (0, 1):
             hip.memset(lib.nvshmem_ptr(sym, q), 0x5A, n * 4)
             hip.memset(lib.nvshmem_ptr(src, q), 0x33, n * 4)
         hip.memset(out, 0x77, n * 4)
+        hip.memset(loc, 0x44, n * 4)
         hip.sync()
+        lsrc = lib.nvshmem_ptr(src, 0)
         calls = [
             lambda: lib.nvshmemh_thread_putget(0, 1, sym, lib.nvshmem_ptr(src, 0), n, 16, pe, None),
             lambda: lib.nvshmemh_thread_putget(1, 1, lib.nvshmem_ptr(src, 0), sym, n, 16, pe, None),
@@ -507,6 +511,13 @@ def test_out_of_range_pe_is_refused_on_device(local_pes, hip, pe):
             lambda: lib.nvshmemh_pg(1, 0, sym, vals, out, n, pe, None),
             lambda: lib.nvshmemh_coop_putget(0, 0, sym, lib.nvshmem_ptr(src, 0), n * 4, 64, pe, 256, None),
             lambda: lib.nvshmemh_coop_putget(0, 1, sym, lib.nvshmem_ptr(src, 0), n * 4, 128, pe, 256, None),
+            # strided: op 0 iput_block, 1 iget_block, 2 iput_wave, 5 iget_wave; 2-D: 3 put, 4 get
+            lambda: lib.nvshmemh_strided_device(0, 0, sym, lsrc, 1, 1, n, 0, 0, 0, 0, pe, 256, None),
+            lambda: lib.nvshmemh_strided_device(1, 0, loc, sym, 1, 1, n, 0, 0, 0, 0, pe, 256, None),
+            lambda: lib.nvshmemh_strided_device(2, 0, sym, lsrc, 1, 1, n, 0, 0, 0, 0, pe, 64, None),
+            lambda: lib.nvshmemh_strided_device(5, 0, loc, sym, 1, 1, n, 0, 0, 0, 0, pe, 64, None),
+            lambda: lib.nvshmemh_strided_device(3, 0, sym, lsrc, 0, 0, 0, 64, 64, 64, 4, pe, 256, None),
+            lambda: lib.nvshmemh_strided_device(4, 0, loc, sym, 0, 0, 0, 64, 64, 64, 4, pe, 256, None),
         ]
         for call in calls:
             lib.nvshmemx_clear_errors()
@@ -517,38 +528,51 @@ def test_out_of_range_pe_is_refused_on_device(local_pes, hip, pe):
             assert (hip.dtoh(lib.nvshmem_ptr(sym, q), n * 4) == 0x5A).all()
             assert (hip.dtoh(lib.nvshmem_ptr(src, q), n * 4) == 0x33).all()
         assert (hip.dtoh(out, n * 4, np.int32) == 0).all()      # g of a bad PE returns 0
+        assert (hip.dtoh(loc, n * 4) == 0x44).all()
     finally:
         lib.nvshmemx_clear_errors()
         hip.free(vals)
         hip.free(out)
+        hip.free(loc)
 
 
 def test_out_of_heap_range_is_refused_on_device(local_pes, hip):
     """device calls whose symmetric range leaves the heap move nothing and
-    record NVSHMEMX_ERR_BAD_ADDRESS: per-thread put, p, g (returns 0) and a
-    wavefront put at an address past the heap's end, and a wavefront put that
-    starts inside the heap but runs past its end.  (Local PEs' slots are
-    padded to a power of two, so every address used here stays inside the
-    allocation even if a check were missing.)"""
+    record NVSHMEMX_ERR_BAD_ADDRESS: per-thread put, p, g (returns 0), a
+    wavefront put, and the strided and 2-D puts and gets at an address past the
+    heap's end, and a wavefront put and a strided put that start inside the
+    heap but run past its end.  (Local PEs' slots are padded to a power of
+    two, so every address used here stays inside the allocation even if a
+    check were missing.)"""
     lib = local_pes.start(2, heap="64M")
     heap = int(lib.nvshmemx_heap_size())
     sym = lib.nvshmalloc(4096)
     src = lib.nvshmalloc(4096)
     past = sym + heap + (8 << 20)
     out = hip.malloc(64)
+    loc = hip.malloc(4096)
     try:
         for q in (0, 1):
             hip.memset(lib.nvshmem_ptr(sym, q), 0x5A, 4096)
             hip.memset(lib.nvshmem_ptr(past, q), 0x6B, 4096)
         hip.memset(out, 0x77, 64)
+        hip.memset(loc, 0x44, 4096)
         hip.sync()
         local_src = lib.nvshmem_ptr(src, 0)
+        big = heap // 32  # an element stride that takes 16 ints from sym about 7/8 of a heap past its end
         calls = [
             lambda: lib.nvshmemh_thread_putget(0, 1, past, local_src, 16, 16, 1, None),
             lambda: lib.nvshmemh_pg(0, 0, past, out, out, 16, 1, None),
             lambda: lib.nvshmemh_pg(1, 0, past, out, out, 16, 1, None),
             lambda: lib.nvshmemh_coop_putget(0, 0, past, local_src, 4096, 4096, 1, 64, None),
             lambda: lib.nvshmemh_coop_putget(0, 0, sym, local_src, heap + (4 << 20), heap + (4 << 20), 1, 64, None),
+            lambda: lib.nvshmemh_strided_device(0, 0, past, local_src, 1, 1, 16, 0, 0, 0, 0, 1, 256, None),
+            lambda: lib.nvshmemh_strided_device(1, 0, loc, past, 1, 1, 16, 0, 0, 0, 0, 1, 256, None),
+            lambda: lib.nvshmemh_strided_device(2, 0, past, local_src, 1, 1, 16, 0, 0, 0, 0, 1, 64, None),
+            lambda: lib.nvshmemh_strided_device(5, 0, loc, past, 1, 1, 16, 0, 0, 0, 0, 1, 64, None),
+            lambda: lib.nvshmemh_strided_device(2, 0, sym, local_src, big, 1, 16, 0, 0, 0, 0, 1, 64, None),
+            lambda: lib.nvshmemh_strided_device(3, 0, past, local_src, 0, 0, 0, 64, 64, 64, 4, 1, 256, None),
+            lambda: lib.nvshmemh_strided_device(4, 0, loc, past, 0, 0, 0, 64, 64, 64, 4, 1, 256, None),
         ]
         for call in calls:
             lib.nvshmemx_clear_errors()
@@ -559,6 +583,8 @@ def test_out_of_heap_range_is_refused_on_device(local_pes, hip):
             assert (hip.dtoh(lib.nvshmem_ptr(sym, q), 4096) == 0x5A).all()
             assert (hip.dtoh(lib.nvshmem_ptr(past, q), 4096) == 0x6B).all()
         assert (hip.dtoh(out, 64, np.int32) == 0).all()      # g of a bad range returns 0
+        assert (hip.dtoh(loc, 4096) == 0x44).all()
     finally:
         lib.nvshmemx_clear_errors()
         hip.free(out)
+        hip.free(loc)
