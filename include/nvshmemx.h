@@ -164,11 +164,13 @@ int nvshmemx_batch_plan_capture(nvshmemx_batch_plan_t plan, int steps);
 void nvshmemx_batch_plan_graph_launch(nvshmemx_batch_plan_t plan, hipStream_t stream);
 /* Select the fastest of `nplans` plans built for the same exchange (for instance with
  * the cost model's choices, the autotuner table's and the untuned rule): one
- * warm-up launch each, then `reps` rounds of one HIP-event-timed launch of
- * every plan in turn on `stream`; returns the index of the plan with the
- * smallest median launch time and, if ms_out is not NULL, writes every
- * plan's median (ms).  Each launch executes the plan's puts, so select when
- * the sources hold what the exchange sends.  No counterpart in the reference. */
+ * timed warm-up launch each, then `reps` rounds that time every plan in turn
+ * on `stream` over a burst of back-to-back launches (about 50 us, at most
+ * 64 launches, between two HIP events); returns the index of the plan with
+ * the smallest median time per launch and, if ms_out is not NULL, writes
+ * every plan's median (ms per launch).  Each launch executes the plan's puts,
+ * so select when the sources hold what the exchange sends.  No counterpart in
+ * the reference. */
 int nvshmemx_batch_plan_select(const nvshmemx_batch_plan_t *plans, int nplans, int reps, hipStream_t stream,
                               float *ms_out);
 size_t nvshmemx_batch_plan_pieces(nvshmemx_batch_plan_t plan);
