@@ -424,6 +424,17 @@ __global__ void __launch_bounds__(256) k_stream_write(nvshmemi::u32x4 *a, size_t
         if (i < n) __builtin_nontemporal_store(v, &a[i]);
     }
 }
+/* the same stream with the system-coherence store policy (sc1), which a
+ * write-only stream on this part drains faster than non-temporal stores */
+__global__ void __launch_bounds__(256) k_stream_write_sc1(nvshmemi::u32x4 *a, size_t n) {
+    const size_t base = (size_t)blockIdx.x * 512 + threadIdx.x;
+    const nvshmemi::u32x4 v = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const size_t i = base + (size_t)u * 256;
+        if (i < n) asm volatile("global_store_dwordx4 %0, %1, off sc1" : : "v"(&a[i]), "v"(v) : "memory");
+    }
+}
 
 } // namespace
 
@@ -462,11 +473,15 @@ void nvshmemh_stream_read(const void *p, size_t nbytes, unsigned int *sink, hipS
     hipLaunchKernelGGL(k_stream_read, dim3((unsigned)((n + 511) / 512)), dim3(256), 0, s, (const nvshmemi::u32x4 *)p, n, sink);
     NVSHMEMI_HIP(hipGetLastError());
 }
-void nvshmemh_stream_write(void *p, size_t nbytes, hipStream_t s) {
+/* policy 0: non-temporal stores; 1: sc1 stores */
+void nvshmemh_stream_write(void *p, size_t nbytes, int policy, hipStream_t s) {
     if ((nbytes & 15) || ((uintptr_t)p & 15)) NVSHMEMI_FATAL("nvshmemh_stream_write: %zu bytes at %p", nbytes, p);
+    if (policy != 0 && policy != 1) NVSHMEMI_FATAL("nvshmemh_stream_write: store policy %d", policy);
     const size_t n = nbytes / 16;
     if (!n) return;
-    hipLaunchKernelGGL(k_stream_write, dim3((unsigned)((n + 511) / 512)), dim3(256), 0, s, (nvshmemi::u32x4 *)p, n);
+    const dim3 grid((unsigned)((n + 511) / 512));
+    if (policy == 0) hipLaunchKernelGGL(k_stream_write, grid, dim3(256), 0, s, (nvshmemi::u32x4 *)p, n);
+    else hipLaunchKernelGGL(k_stream_write_sc1, grid, dim3(256), 0, s, (nvshmemi::u32x4 *)p, n);
     NVSHMEMI_HIP(hipGetLastError());
 }
 
