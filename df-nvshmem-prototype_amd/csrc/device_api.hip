@@ -284,11 +284,15 @@ __device__ __forceinline__ void strided(T *dst, const T *src, ptrdiff_t dst_st, 
                                         size_t lane, size_t lanes) {
     for (size_t i = lane; i < n; i += lanes) dst[(ptrdiff_t)i * dst_st] = src[(ptrdiff_t)i * src_st];
 }
-/* rows go to the workgroup's wavefronts in turn; each row is a wavefront copy */
+/* rows go to the workgroup's wavefronts in turn; each row is a wavefront copy.
+ * A workgroup whose size is not a multiple of 64 ends with a partial
+ * wavefront, which copies its rows with the lanes it has. */
 __device__ __forceinline__ void rows_block(char *dst, size_t dpitch, const char *src, size_t spitch, size_t width,
                                            size_t height) {
-    const unsigned waves = (block_threads() + 63u) / 64u, w = flat_tid() / 64u, lane = flat_tid() & 63u;
-    for (size_t r = w; r < height; r += waves) copy<16, 4>(dst + r * dpitch, src + r * spitch, width, lane, 64);
+    const unsigned threads = block_threads();
+    const unsigned waves = (threads + 63u) / 64u, w = flat_tid() / 64u, lane = flat_tid() & 63u;
+    const unsigned lanes = threads - w * 64u < 64u ? threads - w * 64u : 64u;
+    for (size_t r = w; r < height; r += waves) copy<16, 4>(dst + r * dpitch, src + r * spitch, width, lane, lanes);
     fence_release();
     __syncthreads();
 }

@@ -288,9 +288,27 @@ extern "C" void *nvshmem_ptr(void *local, int pe) {
     return g.heap_base[pe] + off;
 }
 
+namespace nvshmemi {
+/* A device wait or offload barrier that outlived NVSHMEM_WAIT_TIMEOUT_MS gave
+ * up with its condition still false, so whatever ran after it may have used
+ * stale data.  The reference's barrier prints and carries on
+ * (nvshmem_kernels.cu:65-68); here the host's next synchronising call ends the
+ * process with the reference's error convention instead.  Expects a
+ * synchronised device. */
+static void fail_on_recorded_timeout(const char *fn) {
+    int err = 0;
+    NVSHMEMI_HIP(hipMemcpy(&err, g.d_errors, sizeof(int), hipMemcpyDeviceToHost));
+    if (err & (NVSHMEMI_ERR_WAIT_TIMEOUT | NVSHMEMI_ERR_BARRIER_TIMEOUT))
+        NVSHMEMI_FATAL("%s: a device %s outlived its %llu ms budget (errors 0x%x); results after it cannot be "
+                       "trusted", fn, (err & NVSHMEMI_ERR_BARRIER_TIMEOUT) ? "barrier" : "wait",
+                       (unsigned long long)g.wait_timeout_ms, err);
+}
+} // namespace nvshmemi
+
 extern "C" void nvshmem_barrier_all(void) {
     require_init("nvshmem_barrier_all");
     NVSHMEMI_HIP(hipDeviceSynchronize());
+    fail_on_recorded_timeout("nvshmem_barrier_all");
     trace_flush();
     g.boot->barrier();
 }
@@ -303,6 +321,7 @@ extern "C" void nvshmem_barrier_all_offload(void) {
 extern "C" void nvstop_pes(void) {
     require_init("nvstop_pes");
     NVSHMEMI_HIP(hipDeviceSynchronize());
+    fail_on_recorded_timeout("nvstop_pes");
     trace_flush();
     g.boot->barrier(); /* every PE is done with every heap before any is unmapped */
     int err = 0;

@@ -19,6 +19,36 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
 
 
+# GPU files in the order they run: the device API and bulk put/get first (the
+# hot path), then the processes sharing one GPU, tracing, the applications, and
+# last whatever reports speed as well as results.  Files not listed keep their
+# collection order after these; test_zz_* files always run last.
+_GPU_ORDER = [
+    "test_gpu_rma.py",
+    "test_gpu_config2.py",
+    "test_gpu_multiprocess.py",
+    "test_gpu_trace.py",
+    "test_gpu_examples.py",
+    "test_gpu_idealize.py",
+    "test_gpu_batch.py",
+    "test_gpu_config4.py",
+    "test_gpu_persistent.py",
+    "test_gpu_jacobi.py",
+    "test_gpu_costmodel.py",
+    "test_gpu_learned.py",
+]
+
+
+def pytest_collection_modifyitems(session, config, items):
+    def rank(item):
+        name = Path(str(item.fspath)).name
+        if name.startswith("test_zz_"):
+            return len(_GPU_ORDER) + 1
+        return _GPU_ORDER.index(name) if name in _GPU_ORDER else len(_GPU_ORDER)
+
+    items[:] = sorted(items, key=rank)  # stable: keeps the order inside each file
+
+
 @pytest.fixture(scope="session")
 def oracle():
     import oracle_py

@@ -13,6 +13,11 @@ Scenarios (every PE runs the same collective sequence):
            launches it and replays it from a hipGraph; each PE's targets are
            checked bit-exact against the oracle's batch
   jacobi   the 2x2x2 Jacobi application, one PE per process
+  config2  2 PEs: int/float/double x put/get x 8 B .. 2^max_log2 B through the
+           host-launched grid copy and the device per-thread calls, each
+           landing checked against the oracle with guard bytes
+  barrier_timeout  PE 0 alone enters the offload barrier with a 50 ms budget;
+           its next host barrier must end the process
 """
 import json
 import sys
@@ -129,6 +134,76 @@ def main():
         out["batch_ok"] = bool(np.array_equal(hip.dtoh(base + SRC, DST), imgs[me][SRC:]))
         out["requests_issued"] = k
         lib.nvshmem_barrier_all()
+    elif scenario == "config2":
+        # every dtype x {put, get} x 2^3 .. 2^max_log2 bytes between the two
+        # processes, through the host-launched typed calls (the grid copy) and
+        # the device per-thread calls; each PE checks what landed in its own
+        # memory against the oracle, guard bytes on both sides included
+        import time
+        import oracle_py
+        orc = oracle_py.Oracle()
+        max_log2, paths = int(sys.argv[2]), sys.argv[3].split(",")
+        G, CANARY = 256, 0xA5
+        top = 1 << max_log2
+        region = top + 2 * G
+        sym = lib.nvshmalloc(region)
+        src = hip.malloc(region)            # private put source
+        dst = hip.malloc(region)            # private get target
+        peer = 1 - me
+        checked, failures, t0 = 0, [], time.time()
+        for dt_name, dt in na.DTYPES.items():
+            w = na.DTYPE_BYTES[dt_name]
+            for lg in range(3, max_log2 + 1):
+                nbytes = 1 << lg
+                n = nbytes // w
+                mine = orc.pcg32(nbytes // 4, 7_000_000 + 10_000 * me + 100 * dt + lg).view(np.uint8)
+                theirs = orc.pcg32(nbytes // 4, 7_000_000 + 10_000 * peer + 100 * dt + lg).view(np.uint8)
+                hip.htod(src + G, mine)
+                for path in paths:
+                    hip.memset(sym, CANARY, nbytes + 2 * G)
+                    hip.memset(dst, CANARY, nbytes + 2 * G)
+                    hip.sync()
+                    lib.nvshmem_barrier_all()
+                    if path == "host":
+                        getattr(lib, f"nvshmemx_{dt_name}_put_on_stream")(sym + G, src + G, n, peer, None)
+                    else:  # one device call per thread, 4096 elements each
+                        lib.nvshmemh_thread_putget(0, dt, sym + G, src + G, n, 4096, peer, None)
+                    hip.sync()
+                    lib.nvshmem_barrier_all()
+                    img = np.full(nbytes + 2 * G, CANARY, np.uint8)
+                    orc.put([img, img], G, theirs, me)     # what the peer's put leaves on this PE
+                    if not np.array_equal(hip.dtoh(sym, nbytes + 2 * G), img):
+                        failures.append(f"put {path} {dt_name} {nbytes}")
+                    # the peer's symmetric copy now holds this PE's bytes: get them back
+                    if path == "host":
+                        getattr(lib, f"nvshmemx_{dt_name}_get_on_stream")(dst + G, sym + G, n, peer, None)
+                    else:
+                        lib.nvshmemh_thread_putget(1, dt, dst + G, sym + G, n, 4096, peer, None)
+                    hip.sync()
+                    peer_img = np.full(nbytes + 2 * G, CANARY, np.uint8)
+                    orc.put([peer_img, peer_img], G, mine, peer)
+                    got = np.full(nbytes + 2 * G, CANARY, np.uint8)
+                    orc.get([peer_img, peer_img], got[G:G + nbytes], G, peer)
+                    if not np.array_equal(hip.dtoh(dst, nbytes + 2 * G), got):
+                        failures.append(f"get {path} {dt_name} {nbytes}")
+                    checked += 2
+                    lib.nvshmem_barrier_all()
+                print(f"pe {me}: {dt_name} 2^{lg} B done ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
+        out.update(checked=checked, failures=failures)
+        hip.free(src)
+        hip.free(dst)
+    elif scenario == "barrier_timeout":
+        # PE 0 enters the offload barrier with a 50 ms budget; no other PE
+        # does, so the barrier gives up; PE 0's next nvshmem_barrier_all must
+        # end the process (the others then time out in the bootstrap barrier)
+        lib.nvshmem_barrier_all()
+        if me == 0:
+            lib.nvshmemx_set_wait_timeout_ms(50)
+            lib.nvshmem_barrier_all_offload()
+            hip.sync()
+            print("offload barrier returned", flush=True)
+        lib.nvshmem_barrier_all()
+        print("continued", flush=True)
     elif scenario == "jacobi":
         n, steps, outdir = int(sys.argv[2]), int(sys.argv[3]), Path(sys.argv[4])
         fb = lib.nvshmemh_jacobi_field_bytes(n)

@@ -89,3 +89,28 @@ def test_multiprocess_fine_grained_heap(scenario, npes, args):
             assert o["final_flag"] == args[0], o
         if scenario == "barrier":
             assert o["violations"] == 0 and o["slots"] == [args[0]] * npes, o
+
+
+def test_multiprocess_barrier_timeout_ends_the_process():
+    """an offload barrier that only PE 0 enters gives up after its budget and
+    PE 0's next nvshmem_barrier_all exits non-zero; PE 1, left in the host
+    barrier, ends at the bootstrap's own time-out"""
+    name = f"nvshmem_test_{os.getpid()}_{uuid.uuid4().hex[:8]}"
+    env_extra = {"NVSHMEM_HEAP_SIZE": "16M", "NVSHMEM_WAIT_TIMEOUT_MS": "5000", "NVSHMEM_BOOTSTRAP_TIMEOUT_S": "5"}
+    procs = [subprocess.Popen([sys.executable, str(HERE / "pe_worker.py"), "barrier_timeout"],
+                              env=na.shm_job_env(r, 2, name, env_extra), stdout=subprocess.PIPE,
+                              stderr=subprocess.PIPE, text=True) for r in range(2)]
+    try:
+        res = [p.communicate(timeout=90) for p in procs]
+    finally:
+        for p in procs:
+            if p.poll() is None:
+                p.kill()
+        shm = Path("/dev/shm") / name
+        if shm.exists():
+            shm.unlink()
+    (o0, e0), (o1, e1) = res
+    assert "offload barrier returned" in o0, e0[-2000:]
+    assert procs[0].returncode != 0 and "continued" not in o0
+    assert "nvshmem_barrier_all: a device barrier outlived its 50 ms budget" in e0, e0[-2000:]
+    assert procs[1].returncode != 0 and "continued" not in o1

@@ -327,6 +327,22 @@ def test_wait_timeout_is_reported(local_pes, hip, until):
         lib.nvshmemx_set_wait_timeout_ms(5000)
 
 
+@pytest.mark.parametrize("case", ["wait_timeout_barrier", "wait_timeout_stop"])
+def test_recorded_timeout_ends_the_process(case):
+    """a device wait that gave up (its condition still false) makes the next
+    nvshmem_barrier_all or nvstop_pes exit non-zero with the reference's error
+    convention, instead of the run carrying on with stale data and exiting 0
+    (the reference's barrier prints and continues, nvshmem_kernels.cu:65-68)"""
+    env = {k: v for k, v in os.environ.items() if not k.startswith("NVSHMEM_")}
+    env["NVSHMEM_HEAP_SIZE"] = "64M"
+    p = subprocess.run([sys.executable, str(Path(__file__).with_name("misuse_worker.py")), case], env=env,
+                       capture_output=True, text=True, timeout=120)
+    assert "wait returned" in p.stdout, p.stderr[-2000:]
+    assert p.returncode != 0 and "launched" not in p.stdout
+    fn = "nvshmem_barrier_all" if case == "wait_timeout_barrier" else "nvstop_pes"
+    assert f"{fn}: a device wait outlived its 20 ms budget" in p.stderr, p.stderr[-2000:]
+
+
 @pytest.mark.parametrize("case", ["get_private_source", "put_private_target", "put_past_heap_end",
                                   "put_into_slot_padding", "heap_exhausted", "too_many_pes", "symmetric_ok"])
 def test_non_symmetric_ranges_are_refused(case):
@@ -420,10 +436,12 @@ def test_iput_iget(local_pes, hip, oracle, path, dtype):
             hip.free(back)
 
 
-@pytest.mark.parametrize("path", ["device", "host"])
+@pytest.mark.parametrize("path", ["device", "device96", "device160", "host"])
 def test_put_get_2d(local_pes, hip, oracle, path):
     """pitched 2-D put into PE 1 and 2-D get back, narrow and wide rows,
-    relatively misaligned pitches, against the oracle"""
+    relatively misaligned pitches, against the oracle; the device path also
+    with workgroups of 96 and 160 threads, whose last wavefront is partial"""
+    block = {"device96": 96, "device160": 160}.get(path, 256)
     lib = local_pes.start(2, heap="64M")
     for width, height, tpitch, spitch in [(1, 1, 1, 1), (5, 7, 13, 9), (4096, 5, 5000, 4100),
                                           (100000, 3, 100008, 100001), (64, 300, 64, 128), (0, 4, 8, 8)]:
@@ -443,7 +461,7 @@ def test_put_get_2d(local_pes, hip, oracle, path):
         if path == "host":
             lib.nvshmemx_putmem_2d_on_stream(target, tpitch, src_local, spitch, width, height, 1, None)
         else:
-            lib.nvshmemh_strided_device(3, 0, target, src_local, 0, 0, 0, tpitch, spitch, width, height, 1, 256, None)
+            lib.nvshmemh_strided_device(3, 0, target, src_local, 0, 0, 0, tpitch, spitch, width, height, 1, block, None)
         hip.sync()
         assert lib.nvshmemx_get_last_error() == 0
         images = [np.full(region, CANARY, np.uint8) for _ in range(2)]
@@ -456,7 +474,7 @@ def test_put_get_2d(local_pes, hip, oracle, path):
                 lib.nvshmemx_getmem_2d_on_stream(back + GUARD, spitch, target, tpitch, width, height, 1, None)
             else:
                 lib.nvshmemh_strided_device(4, 0, back + GUARD, target, 0, 0, 0, spitch, tpitch, width, height, 1,
-                                            256, None)
+                                            block, None)
             hip.sync()
             assert lib.nvshmemx_get_last_error() == 0
             want = np.full(span_s + 2 * GUARD, CANARY, np.uint8)
