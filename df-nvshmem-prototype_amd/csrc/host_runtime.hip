@@ -213,6 +213,7 @@ static void start_local_mode(int npes, int wgs_per_pe) {
 static void finish_start() {
     g.trace_dev = trace_init();
     upload_device_state();
+    persistent_init();
     NVSHMEMI_HIP(hipDeviceSynchronize());
     g.initialized = true;
     g.boot->barrier();

@@ -144,6 +144,10 @@ void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, co
 void plan_launch(nvshmemx_batch_plan *p, hipStream_t s);
 nvshmemx_batch_plan *plan_new();
 
+/* persistent.hip: per-device set-up of the persistent batch kernel (called
+ * once the PE layout is known, with the device current) */
+void persistent_init();
+
 /* trace_host.hip (no-ops unless built with NVSHMEM_TRACING); trace_init
  * returns the device-side ring descriptor, or nullptr when tracing is off */
 nvshmemi_trace_dev_t *trace_init();

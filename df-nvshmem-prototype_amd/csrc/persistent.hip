@@ -1,0 +1,478 @@
+/*
+ * persistent.hip -- a batch of puts executed by one persistent kernel whose
+ * workgroups are the PEs that issued the puts.
+ *
+ * No counterpart in the reference, whose puts are one call per thread
+ * (nvshmem_device.cu:487-512).  The fused plans of batch.hip are cut into
+ * pieces and dealt to workers on the host; here the device does that itself,
+ * every launch, from the request list as it is:
+ *
+ *  - the grid holds only as many workgroups as the device runs at once, so
+ *    every workgroup is resident for the whole launch.  Workgroup b belongs to
+ *    PE b mod P (P = the local PEs, or 1 in process mode), so workgroups
+ *    0 .. P-1 -- dispatched first -- are the PEs' leaders;
+ *  - each leader reads the whole request list and takes the requests its PE
+ *    issued (those whose source lies in its heap; a private source counts for
+ *    PE i mod P): it lays them end to end in a virtual byte stream, each
+ *    padded to a 4 KiB granule, writes its entries (request index, start in
+ *    the stream) and, for every chunk of the stream, the entry the chunk
+ *    starts in; then publishes its queue with a release flag.  Leaders agree
+ *    on where each PE's entries and chunks go by counting every PE's requests
+ *    in the same pass, so no leader waits for another;
+ *  - every workgroup claims 128 KiB chunks of its own PE's queue with one
+ *    atomic fetch-add each (the next claim is issued before the current chunk
+ *    is copied), and once that queue is drained, takes chunks from the queue
+ *    with the most left.  Inside a chunk, a request segment of 16 KiB or more
+ *    is copied by the whole workgroup, shorter ones by one wavefront each, in
+ *    turn, with 16-byte accesses where the alignment allows, eight loads in
+ *    flight per lane and non-temporal stores;
+ *  - the last workgroup to finish clears the claim counters and flags, so
+ *    the next launch (or graph replay) starts from zero with no memset.
+ *
+ * Contention and coherence: each PE's claim counter, ready flag and queue
+ * record sit in a 4 KiB slot of their own, so the claims of different PEs do
+ * not queue on one cache line.  Readers never take an agent-scope acquire
+ * (which invalidates the XCD's L2 under the streams the other workgroups are
+ * running): the leader publishes with one release, and everything it
+ * published is read with coherent (agent-scope relaxed) loads after the flag
+ * was seen set.
+ *
+ * Launch shape, from measurements of the 4096-put step of 64 PEs (1.57 GB
+ * payload): two workgroups of 512 threads per CU with eight loads in flight
+ * per lane ran the step in 0.64 ms; three or four per CU, or four loads in
+ * flight, 0.68-0.72 ms; every wavefront working alone on 32 KiB chunks, 1.07
+ * ms (each of ~8000 concurrent streams waited ~12 us per access: the DRAM
+ * pages thrash); claim counters sharing one cache line and an acquire per
+ * poll, 1.49 ms.  The leaders take about 24 us before the first chunk is
+ * claimed.  The host-planned fused launch of batch.hip (source-ordered
+ * pieces, no device-side planning) stays the faster path for a static
+ * exchange: 0.51 ms.
+ *
+ * Scratch (per launch, device memory): the control slots, the chunk table
+ * (65,536 entries; the chunk grows from 128 KiB so that a batch of any size
+ * fits) and one 8-byte entry per request.
+ * nvshmemx_batch_put_persistent_on_stream allocates it (grown on demand).
+ * Calls that overlap on one GPU share the scratch and race.
+ */
+#include <algorithm>
+
+#include "nvshmem.h"
+#include "nvshmemx.h"
+#include "nvshmemi_device.h"
+#include "nvshmemi_host.h"
+
+namespace nvshmemi {
+
+#ifndef NVSHMEMI_PCHUNK_UNITS
+#define NVSHMEMI_PCHUNK_UNITS 32
+#endif
+#ifndef NVSHMEMI_PUNROLL
+#define NVSHMEMI_PUNROLL 8
+#endif
+constexpr int kPBlock = 512;
+constexpr int kPWaves = kPBlock / 64;
+constexpr unsigned kUnit = 4096;                           /* granule of the virtual stream, bytes */
+constexpr unsigned kChunkUnitsMin = NVSHMEMI_PCHUNK_UNITS; /* chunk unless the table needs larger */
+constexpr size_t kWaveSegMax = 16384; /* shorter request segments go to one wavefront */
+constexpr int kPUnroll = NVSHMEMI_PUNROLL;                 /* loads in flight per lane */
+constexpr unsigned kTableChunks = 65536;                   /* chunk-table entries */
+constexpr unsigned kLeadCache = 4096;                      /* requests a leader keeps in LDS between passes */
+constexpr int kPPerCU = 2;                                 /* workgroups per CU (see the launch shape above) */
+constexpr int kMaxDevices = 64;
+
+struct pinfo {            /* PE p's queue, published by its leader */
+    unsigned entry0;      /* first entry */
+    unsigned nown;        /* entries */
+    unsigned chunk0;      /* first chunk-table slot */
+    unsigned nchunks;
+    unsigned chunk_units; /* chunk size in granules, the same for every PE */
+    unsigned pad[3];
+};
+struct pentry {
+    unsigned req;    /* request index */
+    unsigned vstart; /* start in the PE's virtual stream, granules */
+};
+
+/* scratch layout: per-PE 4 KiB slots (claim, ready, queue record), the done
+ * counter, the chunk table, the entries */
+constexpr size_t kSlot = 4096;
+constexpr size_t kClaimOff = 0, kReadyOff = 128, kInfoOff = 256;
+constexpr size_t kDoneOff = kSlot * NVSHMEMI_MAX_PES;
+constexpr size_t kCtlBytes = kDoneOff + 256; /* zero between launches */
+constexpr size_t kTableOff = kCtlBytes;
+constexpr size_t kEntryOff = kTableOff + sizeof(unsigned) * kTableChunks;
+constexpr size_t scratch_bytes(size_t nreq) { return kEntryOff + sizeof(pentry) * (nreq ? nreq : 1); }
+
+struct pscr {
+    unsigned char *base;
+    __device__ __forceinline__ unsigned *claim(int q) const { return (unsigned *)(base + q * kSlot + kClaimOff); }
+    __device__ __forceinline__ unsigned *ready(int q) const { return (unsigned *)(base + q * kSlot + kReadyOff); }
+    __device__ __forceinline__ pinfo *info(int q) const { return (pinfo *)(base + q * kSlot + kInfoOff); }
+    __device__ __forceinline__ unsigned *done() const { return (unsigned *)(base + kDoneOff); }
+    __device__ __forceinline__ unsigned *table() const { return (unsigned *)(base + kTableOff); }
+    __device__ __forceinline__ pentry *entries() const { return (pentry *)(base + kEntryOff); }
+};
+
+/* ------------------------------------------------------------- device -- */
+
+__device__ __forceinline__ unsigned coherent_u32(const unsigned *p) { return load_relaxed(p); }
+__device__ __forceinline__ pinfo coherent_info(const pinfo *p) {
+    pinfo r;
+    r.entry0 = load_relaxed(&p->entry0);
+    r.nown = load_relaxed(&p->nown);
+    r.chunk0 = load_relaxed(&p->chunk0);
+    r.nchunks = load_relaxed(&p->nchunks);
+    r.chunk_units = load_relaxed(&p->chunk_units);
+    return r;
+}
+__device__ __forceinline__ pentry coherent_entry(const pentry *p) {
+    const unsigned long long v = load_relaxed((const long long *)p);
+    return pentry{(unsigned)v, (unsigned)(v >> 32)};
+}
+
+/* poll a ready flag with relaxed loads (back to back, then the sleep
+ * ladder) until it is set or the wait budget runs out; no acquire: what the
+ * flag publishes is read with coherent loads.  Returns the flag. */
+__device__ __forceinline__ unsigned wait_ready(const unsigned *flag) {
+    unsigned v = load_relaxed(flag);
+    for (int i = 0; i < NVSHMEMI_SPIN_POLLS && !v; ++i) v = load_relaxed(flag);
+    if (!v) {
+        const uint64_t budget = nvshmemi_dstate.wait_timeout_ticks;
+        const uint64_t t0 = wall_clock64();
+        sleep_ladder pace;
+        while (!(v = load_relaxed(flag))) {
+            pace.pause();
+            if ((uint64_t)(wall_clock64() - t0) > budget) {
+                set_error(NVSHMEMI_ERR_WAIT_TIMEOUT);
+                break;
+            }
+        }
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST); /* no later load moves above the poll */
+    return v;
+}
+
+__device__ __forceinline__ int req_issuer(const nvshmemx_put_request_t &r, unsigned i, int P) {
+    if (P == 1) return 0;
+    const nvshmemi_device_state_t &s = nvshmemi_dstate;
+    const uint64_t stride = s.offset_mask + 1;
+    const uint64_t off = (uint64_t)((const char *)r.source - s.heap_base[0]);
+    if (off < stride * (uint64_t)s.npes) return (int)(off / stride);
+    return (int)(i % (unsigned)P);
+}
+
+/* granules a request occupies in its PE's stream; 0 (and an error bit, set
+ * by leader 0) for a request naming a bad PE or a target range outside the
+ * heap, which is then skipped */
+__device__ __forceinline__ unsigned req_units(const nvshmemx_put_request_t &r, bool report) {
+    const nvshmemi_device_state_t &s = nvshmemi_dstate;
+    if ((unsigned)r.pe >= (unsigned)s.npes) {
+        if (report) set_error(NVSHMEMI_ERR_BAD_PE);
+        return 0;
+    }
+    const uint64_t off = ((uint64_t)((const char *)r.target - s.anchor)) & s.offset_mask;
+    if (off >= s.heap_bytes || r.nbytes > s.heap_bytes - off) {
+        if (report) set_error(NVSHMEMI_ERR_BAD_ADDRESS);
+        return 0;
+    }
+    return (unsigned)((r.nbytes + kUnit - 1) / kUnit);
+}
+
+/* exclusive block scan of two counters over kPBlock threads; ta/tb get the
+ * block totals.  Uses s_w[2][kPWaves]; every thread must call it. */
+__device__ __forceinline__ void block_scan2(unsigned a, unsigned b, unsigned &xa, unsigned &xb, unsigned &ta,
+                                            unsigned &tb, unsigned (*s_w)[kPWaves]) {
+    const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    unsigned ia = a, ib = b;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned na = __shfl_up(ia, d, 64), nb = __shfl_up(ib, d, 64);
+        if (lane >= (unsigned)d) ia += na, ib += nb;
+    }
+    if (lane == 63) s_w[0][w] = ia, s_w[1][w] = ib;
+    __syncthreads();
+    unsigned oa = 0, ob = 0;
+    ta = 0, tb = 0;
+#pragma unroll
+    for (int k = 0; k < kPWaves; ++k) {
+        if ((unsigned)k < w) oa += s_w[0][k], ob += s_w[1][k];
+        ta += s_w[0][k], tb += s_w[1][k];
+    }
+    xa = oa + ia - a;
+    xb = ob + ib - b;
+    __syncthreads();
+}
+
+/* leader of PE p: build and publish the PE's queue */
+__device__ void lead(const nvshmemx_put_request_t *__restrict__ reqs, unsigned nreq, int P, int p, const pscr &S) {
+    /* issuer (high 7 bits) and granules (low 25 bits) of the first requests,
+     * kept from the first pass so the second reads no request again */
+    __shared__ unsigned s_req[kLeadCache];
+    __shared__ unsigned s_cnt[NVSHMEMI_MAX_PES];
+    __shared__ unsigned long long s_units[NVSHMEMI_MAX_PES];
+    __shared__ unsigned s_w[2][kPWaves];
+    __shared__ pinfo s_me;
+    unsigned *table = S.table();
+    pentry *entries = S.entries();
+    for (int q = threadIdx.x; q < P; q += kPBlock) s_cnt[q] = 0, s_units[q] = 0;
+    __syncthreads();
+    /* a PE's entries are its requests that move bytes: zero-byte and invalid
+     * requests get none, so no chunk can reach them */
+    for (unsigned i = threadIdx.x; i < nreq; i += kPBlock) {
+        const nvshmemx_put_request_t r = reqs[i];
+        const int q = req_issuer(r, i, P);
+        const unsigned u = req_units(r, p == 0);
+        if (u) {
+            atomicAdd(&s_cnt[q], 1u);
+            atomicAdd(&s_units[q], (unsigned long long)u);
+        }
+        if (i < kLeadCache) s_req[i] = u < (1u << 25) ? ((unsigned)q << 25) | u : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long total = 0;
+        for (int q = 0; q < P; ++q) total += s_units[q];
+        /* sum over PEs of ceil(units / cu) <= total / cu + P <= table size */
+        const unsigned long long need = (total + (kTableChunks - P) - 1) / (kTableChunks - P);
+        const unsigned cu = (unsigned)(need > kChunkUnitsMin ? need : kChunkUnitsMin);
+        unsigned e0 = 0, c0 = 0;
+        for (int q = 0; q < p; ++q) e0 += s_cnt[q], c0 += (unsigned)((s_units[q] + cu - 1) / cu);
+        s_me.entry0 = e0;
+        s_me.nown = s_cnt[p];
+        s_me.chunk0 = c0;
+        s_me.nchunks = (unsigned)((s_units[p] + cu - 1) / cu);
+        s_me.chunk_units = cu;
+    }
+    __syncthreads();
+    const unsigned cu = s_me.chunk_units, e0 = s_me.entry0, c0 = s_me.chunk0;
+    unsigned rc = 0, ru = 0; /* own entries and granules of earlier tiles */
+    for (unsigned base = 0; base < nreq; base += kPBlock) {
+        const unsigned i = base + threadIdx.x;
+        unsigned u = 0;
+        if (i < nreq) {
+            const unsigned c = i < kLeadCache ? s_req[i] : 0xFFFFFFFFu;
+            if (c != 0xFFFFFFFFu) {
+                if ((int)(c >> 25) == p) u = c & ((1u << 25) - 1);
+            } else {
+                const nvshmemx_put_request_t r = reqs[i];
+                if (req_issuer(r, i, P) == p) u = req_units(r, false);
+            }
+        }
+        const bool own = u != 0;
+        unsigned xo, xu, to, tu;
+        block_scan2(own ? 1u : 0u, u, xo, xu, to, tu, s_w);
+        if (own) {
+            const unsigned e = e0 + rc + xo, vs = ru + xu;
+            entries[e] = pentry{i, vs};
+            /* every chunk that starts inside [vs, vs + u) starts in entry e */
+            for (unsigned k = (vs + cu - 1) / cu; k * cu < vs + u; ++k) table[c0 + k] = e;
+        }
+        rc += to;
+        ru += tu;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *S.info(p) = s_me;
+        store_release(S.ready(p), 1u); /* writes back this XCD's L2 first */
+    }
+}
+
+/* a workgroup copies chunk k of a queue: a request segment of kWaveSegMax
+ * bytes or more with all its threads, shorter ones one wavefront each, in
+ * turn */
+__device__ __forceinline__ void run_chunk(const nvshmemx_put_request_t *__restrict__ reqs, const pinfo &inf, unsigned k,
+                                          const pscr &S) {
+    const uint64_t c_lo = (uint64_t)k * inf.chunk_units * kUnit, c_hi = c_lo + (uint64_t)inf.chunk_units * kUnit;
+    const unsigned e_end = inf.entry0 + inf.nown;
+    const unsigned wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    unsigned small = 0;
+    for (unsigned e = coherent_u32(S.table() + inf.chunk0 + k); e < e_end; ++e) {
+        const pentry ent = coherent_entry(S.entries() + e);
+        const uint64_t vs = (uint64_t)ent.vstart * kUnit;
+        if (vs >= c_hi) break;
+        const nvshmemx_put_request_t r = reqs[ent.req];
+        const uint64_t lo = (c_lo > vs ? c_lo : vs) - vs;
+        const uint64_t end = vs + r.nbytes;
+        const uint64_t hi = (c_hi < end ? c_hi : end) - vs;
+        if (hi <= lo) continue; /* the chunk starts in the request's padding */
+        char *dst = translate(r.target, r.pe) + lo;
+        const char *src = (const char *)r.source + lo;
+        const size_t n = (size_t)(hi - lo);
+        if (n >= kWaveSegMax) {
+            copy<16, kPUnroll, NT_STORE>(dst, src, n, threadIdx.x, kPBlock);
+        } else {
+            if (small % kPWaves == wave) copy<16, kPUnroll, NT_STORE>(dst, src, n, lane, 64);
+            ++small;
+        }
+    }
+}
+
+/* a workgroup drains queue q: thread 0 claims chunks, the next claim issued
+ * before the current chunk is copied; claims pass through two LDS slots, one
+ * barrier per chunk */
+__device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__ reqs, const pscr &S, const pinfo &inf,
+                                      int q, unsigned *s_k) {
+    if (threadIdx.x == 0) s_k[0] = atomicAdd(S.claim(q), 1u);
+    __syncthreads();
+    unsigned it = 0;
+    unsigned k = __builtin_amdgcn_readfirstlane(s_k[0]);
+    while (k < inf.nchunks) {
+        unsigned next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(S.claim(q), 1u);
+        run_chunk(reqs, inf, k, S);
+        if (threadIdx.x == 0) s_k[(it + 1) & 1] = next;
+        __syncthreads();
+        ++it;
+        k = __builtin_amdgcn_readfirstlane(s_k[it & 1]);
+    }
+}
+
+__global__ void __launch_bounds__(kPBlock) k_batch_persistent(const nvshmemx_put_request_t *__restrict__ reqs,
+                                                              unsigned nreq, int P, unsigned char *scr) {
+    const pscr S{scr};
+    __shared__ unsigned s_k[2];
+    __shared__ int s_q[2];
+    const int p = (int)(blockIdx.x % (unsigned)P);
+    if ((int)blockIdx.x < P) lead(reqs, nreq, P, p, S);
+
+    /* own queue first (a wait that timed out leaves the flag 0: the record
+     * then still holds an earlier launch's queue, and is not used) */
+    if (threadIdx.x == 0) s_q[0] = wait_ready(S.ready(p)) ? p : -1;
+    __syncthreads();
+    if (s_q[0] >= 0) {
+        const pinfo inf = coherent_info(S.info(p));
+        drain(reqs, S, inf, p, s_k);
+    }
+    /* then the queue with the most chunks left (chosen by wavefront 0); a
+     * queue not yet published (its leader still reading the list) is waited
+     * for only when no other queue has work */
+    for (unsigned round = 0;; ++round) {
+        if (threadIdx.x < 64) {
+            unsigned best = 0;
+            int best_q = -1, pending = -1;
+            for (int q = (int)threadIdx.x; q < P; q += 64) {
+                if (coherent_u32(S.ready(q)) == 0u) {
+                    pending = q;
+                    continue;
+                }
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                const unsigned nc = coherent_u32(&S.info(q)->nchunks), cl = coherent_u32(S.claim(q));
+                const unsigned rem = nc > cl ? nc - cl : 0u;
+                if (rem > best) best = rem, best_q = q;
+            }
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) {
+                const unsigned ob = __shfl_xor(best, d, 64);
+                const int oq = __shfl_xor(best_q, d, 64), op = __shfl_xor(pending, d, 64);
+                if (ob > best || (ob == best && oq > best_q)) best = ob, best_q = oq;
+                pending = op > pending ? op : pending;
+            }
+            if (threadIdx.x == 0) {
+                int v = best_q >= 0 ? best_q : -1;
+                if (best_q < 0 && pending >= 0) v = wait_ready(S.ready(pending)) ? -2 : -1; /* -2: look again */
+                s_q[(round + 1) & 1] = v;
+            }
+        }
+        __syncthreads();
+        const int v = s_q[(round + 1) & 1];
+        if (v == -1) break;
+        if (v >= 0) {
+            const pinfo inf = coherent_info(S.info(v));
+            drain(reqs, S, inf, v, s_k);
+        }
+    }
+    /* the last workgroup out clears the control slots for the next launch */
+    if (threadIdx.x == 0) {
+        fence_release();
+        const unsigned old = __hip_atomic_fetch_add(S.done(), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == gridDim.x - 1) {
+            for (int q = 0; q < P; ++q) {
+                store_relaxed(S.claim(q), 0u);
+                store_relaxed(S.ready(q), 0u);
+            }
+            store_relaxed(S.done(), 0u);
+        }
+    }
+}
+
+/* zeroes the control slots (set-up only; a fill kernel, not a memset) */
+__global__ void k_persist_clear(unsigned *ctl, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        ctl[i] = 0u;
+}
+
+/* --------------------------------------------------------------- host -- */
+
+struct PersistDevice {
+    unsigned char *alloc_ptr = nullptr; /* library-allocated scratch */
+    size_t alloc_reqs = 0;
+    int capacity = 0; /* workgroups of the persistent kernel resident at once */
+};
+static PersistDevice g_pdev[kMaxDevices];
+
+static PersistDevice &pdev() {
+    if (g.device < 0 || g.device >= kMaxDevices) NVSHMEMI_FATAL("device %d beyond the persistent-kernel table", g.device);
+    return g_pdev[g.device];
+}
+
+static void clear_ctl(unsigned char *base) {
+    hipLaunchKernelGGL(k_persist_clear, dim3(64), dim3(256), 0, 0, (unsigned *)base, kCtlBytes / sizeof(unsigned));
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+/* Library set-up, once per device with that device current: how many
+ * workgroups of the kernel the device runs at once (at most kPPerCU per CU). */
+void persistent_init() {
+    PersistDevice &d = pdev();
+    if (!d.capacity) {
+        int per_cu = 0, cus = 0;
+        NVSHMEMI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g.device));
+        NVSHMEMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_batch_persistent, kPBlock, 0));
+        d.capacity = std::max(1, std::min(per_cu, kPPerCU)) * std::max(1, cus);
+    }
+}
+
+static int local_issuers() { return g.mode == NVSHMEMI_MODE_LOCAL ? g.npes : 1; }
+
+static void persistent_launch(const nvshmemx_put_request_t *reqs, int nreq, unsigned char *scr, hipStream_t s) {
+    const PersistDevice &d = pdev();
+    const int P = local_issuers();
+    if (d.capacity < P)
+        NVSHMEMI_FATAL("the persistent batch kernel needs one resident workgroup per PE: %d PEs, %d resident", P,
+                       d.capacity);
+    const int grid = P * (d.capacity / P);
+    hipLaunchKernelGGL(k_batch_persistent, dim3((unsigned)grid), dim3(kPBlock), 0, s, reqs, (unsigned)nreq, P, scr);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+static void check_args(const nvshmemx_put_request_t *reqs, int nreq, const char *fn) {
+    require_init(fn);
+    if (nreq < 0 || (nreq > 0 && !reqs)) NVSHMEMI_FATAL("%s: bad arguments (%d requests at %p)", fn, nreq, (const void *)reqs);
+}
+
+} // namespace nvshmemi
+
+using namespace nvshmemi;
+
+extern "C" int nvshmemx_batch_put_persistent_on_stream(const nvshmemx_put_request_t *reqs, int nreq, hipStream_t s) {
+    check_args(reqs, nreq, "nvshmemx_batch_put_persistent_on_stream");
+    if (nreq == 0) return 0;
+    PersistDevice &d = pdev();
+    if ((size_t)nreq > d.alloc_reqs) {
+        NVSHMEMI_HIP(hipDeviceSynchronize()); /* the old scratch may be in use */
+        if (d.alloc_ptr) NVSHMEMI_HIP(hipFree(d.alloc_ptr));
+        const size_t want = std::max<size_t>((size_t)nreq, 4096);
+        NVSHMEMI_HIP(hipMalloc((void **)&d.alloc_ptr, scratch_bytes(want)));
+        d.alloc_reqs = want;
+        clear_ctl(d.alloc_ptr);
+        NVSHMEMI_HIP(hipDeviceSynchronize());
+    }
+    persistent_launch(reqs, nreq, d.alloc_ptr, s);
+    return 0;
+}
+
+extern "C" int nvshmemx_batch_persistent_grid(void) {
+    require_init("nvshmemx_batch_persistent_grid");
+    const int P = local_issuers();
+    return P * std::max(1, pdev().capacity / P);
+}

@@ -143,6 +143,8 @@ SIGNATURES.update({
     "nvshmemx_put_requests_on_stream": (None, [_P, _I, _P]),
     "nvshmemh_gen_requests": (_I, [_U64, _I, _I, _U64, _U64, _I, _I, _P, _P, _P, _P, _P, _P]),
     "nvshmemh_verify_requests": (_U64, [_P, _I]),
+    "nvshmemx_batch_put_persistent_on_stream": (_I, [_P, _I, _P]),
+    "nvshmemx_batch_persistent_grid": (_I, []),
     "nvshmemx_tune_candidate_count": (_I, []),
     "nvshmemx_tune_candidate": (None, [_I, _P]),
     "nvshmemx_align_class": (_I, [_P, _P]),
@@ -405,6 +407,12 @@ class Hip:
             ("hipGetErrorString", C.c_char_p, [_I]),
             ("hipGetDeviceCount", _I, [C.POINTER(_I)]),
             ("hipSetDevice", _I, [_I]),
+            ("hipStreamBeginCapture", _I, [_P, _I]),
+            ("hipStreamEndCapture", _I, [_P, C.POINTER(_P)]),
+            ("hipGraphInstantiate", _I, [C.POINTER(_P), _P, _P, _P, _SZ]),
+            ("hipGraphLaunch", _I, [_P, _P]),
+            ("hipGraphExecDestroy", _I, [_P]),
+            ("hipGraphDestroy", _I, [_P]),
         ]:
             fn = getattr(L, name)
             fn.restype = res

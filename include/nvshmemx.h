@@ -181,6 +181,28 @@ void nvshmemx_batch_default_choice(size_t nbytes, int elem_bytes, nvshmemx_batch
 /* The per-call path: one grid put launch per request. */
 void nvshmemx_put_requests_on_stream(const nvshmemx_put_request_t *reqs, int nreq, hipStream_t stream);
 
+/* ---- batched puts in one persistent kernel of the issuing PEs ---- */
+
+/* Execute `nreq` puts (the request array in DEVICE memory) with one
+ * persistent kernel on `stream`.  The kernel's workgroups are the PEs that
+ * issued the puts -- in local mode workgroup b belongs to PE b mod npes, and a
+ * request belongs to the PE whose heap holds its source (a private source
+ * counts for PE i mod npes); in process mode every workgroup is this PE.
+ * Each PE cuts its own requests into chunks on the device and its workgroups
+ * copy them; a PE whose chunks are done takes chunks from the PE with the
+ * most left.  Destinations must not overlap (puts are unordered, as in the
+ * fused plans).  A request naming a bad PE or a target range outside the heap
+ * is skipped and sets NVSHMEMX_ERR_BAD_PE / NVSHMEMX_ERR_BAD_ADDRESS.
+ * The library allocates the kernel's scratch (a control block, a 256 KiB
+ * chunk table and 8 bytes per request), growing it on demand; the kernel
+ * clears its counters itself, so a captured call replays without a memset.
+ * The first call (or one with more requests than any before) allocates and
+ * so cannot be captured.  Calls that overlap on one GPU share the scratch
+ * and race.  Returns 0. */
+int nvshmemx_batch_put_persistent_on_stream(const nvshmemx_put_request_t *reqs, int nreq, hipStream_t stream);
+/* workgroups of the persistent kernel's grid (every one resident at once) */
+int nvshmemx_batch_persistent_grid(void);
+
 /* ---- exhaustive autotuner of batch choices ---- */
 
 /* The 54 candidates: pieces {1,2,..,256} x access {4,8,16} B x {wave, workgroup}. */

@@ -16,6 +16,8 @@ Scenarios (every PE runs the same collective sequence):
   config2  2 PEs: int/float/double x put/get x 8 B .. 2^max_log2 B through the
            host-launched grid copy and the device per-thread calls, each
            landing checked against the oracle with guard bytes
+  persistent  like batch, through the persistent kernel (requests in device
+           memory, each process's kernel the only PE of its grid)
   barrier_timeout  PE 0 alone enters the offload barrier with a 50 ms budget;
            its next host barrier must end the process
 """
@@ -133,6 +135,37 @@ def main():
         orc.batch_apply(imgs, reqs, 0, SRC)
         out["batch_ok"] = bool(np.array_equal(hip.dtoh(base + SRC, DST), imgs[me][SRC:]))
         out["requests_issued"] = k
+        lib.nvshmem_barrier_all()
+    elif scenario == "persistent":
+        # every PE issues its share of a shared uneven-put batch (sources in
+        # its own heap, targets in the others' IPC-mapped heaps) through the
+        # persistent kernel; each PE checks its destinations against the oracle
+        import oracle_py
+        orc = oracle_py.Oracle()
+        SRC, DST = 4 << 20, 24 << 20
+        nreq = int(sys.argv[2])
+        base = lib.nvshmalloc(SRC + DST)
+        lib.nvshmemh_fill_pcg32(base, SRC // 4, 100 + me, 54, None)
+        hip.memset(base + SRC, 0, DST)
+        hip.sync()
+        reqs = na.gen_requests(lib, 13, nreq, n, SRC, DST, 6, 20)
+        mine = {k: v[reqs["src_pe"] == me] for k, v in reqs.items()}
+        arr = na.make_requests(lib, mine, base, 0, SRC)
+        raw = np.frombuffer(bytes(arr), np.uint8)
+        d_reqs = hip.malloc(max(1, raw.size))
+        if raw.size:
+            hip.htod(d_reqs, raw)
+        lib.nvshmem_barrier_all()
+        assert lib.nvshmemx_batch_put_persistent_on_stream(d_reqs, len(arr), None) == 0
+        hip.sync()
+        lib.nvshmem_barrier_all()
+        imgs = [np.zeros(SRC + DST, np.uint8) for _ in range(n)]
+        for pe in range(n):
+            imgs[pe][:SRC] = orc.pcg32(SRC // 4, 100 + pe).view(np.uint8)
+        orc.batch_apply(imgs, reqs, 0, SRC)
+        out["batch_ok"] = bool(np.array_equal(hip.dtoh(base + SRC, DST), imgs[me][SRC:]))
+        out["requests_issued"] = len(arr)
+        hip.free(d_reqs)
         lib.nvshmem_barrier_all()
     elif scenario == "config2":
         # every dtype x {put, get} x 2^3 .. 2^max_log2 bytes between the two

@@ -66,12 +66,53 @@ def _check(lib, hip, base, want):
             raise AssertionError(f"PE {pe}: {bad.size} bytes differ, first at destination offset {bad[0]}")
 
 
-@pytest.mark.parametrize("path", ["fused", "graph", "owner", "per_call"])
+def device_requests(hip, arr):
+    """the request array copied to device memory (the persistent kernel reads
+    its requests there)"""
+    raw = np.frombuffer(bytes(arr), np.uint8)
+    d = hip.malloc(max(1, raw.size))
+    if raw.size:
+        hip.htod(d, raw)
+    return d
+
+
+@pytest.mark.parametrize("path", ["fused", "graph", "owner", "per_call", "persistent", "persistent_graph"])
 def test_config4_full_size(full, hip, path):
     lib, base, arr, want = full
     n = len(arr)
     _reset(lib, hip, base)
-    if path == "per_call":
+    if path.startswith("persistent"):
+        d_reqs = device_requests(hip, arr)
+        try:
+            lib.nvshmemx_batch_put_persistent_on_stream(d_reqs, n, None)
+            hip.sync()
+            if path == "persistent_graph":
+                # capture two calls on a stream, replay twice with the
+                # destinations cleared in between; eager calls in between too
+                s = hip.stream()
+                graph, exe = C.c_void_p(), C.c_void_p()
+                hip.check(hip.lib.hipStreamBeginCapture(s, 1), "hipStreamBeginCapture")  # thread-local mode
+                for _ in range(2):
+                    lib.nvshmemx_batch_put_persistent_on_stream(d_reqs, n, s)
+                hip.check(hip.lib.hipStreamEndCapture(s, C.byref(graph)), "hipStreamEndCapture")
+                hip.check(hip.lib.hipGraphInstantiate(C.byref(exe), graph, None, None, 0), "hipGraphInstantiate")
+                try:
+                    for _ in range(2):
+                        _reset(lib, hip, base)
+                        hip.check(hip.lib.hipGraphLaunch(exe, s), "hipGraphLaunch")
+                        hip.stream_sync(s)
+                        _check(lib, hip, base, want)
+                        _reset(lib, hip, base)
+                        lib.nvshmemx_batch_put_persistent_on_stream(d_reqs, n, None)
+                        hip.sync()
+                        _check(lib, hip, base, want)
+                finally:
+                    hip.lib.hipGraphExecDestroy(exe)
+                    hip.lib.hipGraphDestroy(graph)
+                    hip.stream_destroy(s)
+        finally:
+            hip.free(d_reqs)
+    elif path == "per_call":
         lib.nvshmemx_put_requests_on_stream(arr, n, None)
         hip.sync()
     else:

@@ -1,0 +1,172 @@
+"""GPU parity of the persistent batch kernel (nvshmemx_batch_put_persistent_on_stream)
+against the oracle: the PEs that issued the puts cut them into chunks on the
+device and copy them, stealing from each other's queues.
+
+8 workgroup-partitioned PEs in one process; each PE's heap holds a 32 MiB
+source region (PCG32 words, seed 100 + pe) and a 64 MiB destination region.
+Every destination byte of every PE is compared with the oracle's sequential
+application of the same requests.  Covered: a batch of the uneven-put
+workload, byte-granular (every alignment class) requests mixed with
+zero-byte ones, sources outside the heap, thousands of tiny requests (many
+per chunk, the wavefront path), one request larger than many chunks (the
+workgroup path, every PE's workgroups stealing from one queue), repeated
+calls (the kernel clears its own counters), an empty list, requests naming a
+bad PE or leaving the heap (skipped, error bit set), and two processes each
+issuing puts into the other's IPC-mapped heap."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import nvshmem_amd as na
+from test_gpu_config4 import device_requests
+from test_gpu_multiprocess import run_job
+
+pytestmark = pytest.mark.gpu
+NPES, WGS = 8, 4
+SRC, DST = 32 << 20, 64 << 20
+
+
+@pytest.fixture(scope="module")
+def setup(local_pes, hip, oracle):
+    lib = local_pes.start(NPES, wgs_per_pe=WGS, heap="96M")
+    base = lib.nvshmalloc(SRC + DST)
+    images = []
+    for pe in range(NPES):
+        p = lib.nvshmem_ptr(base, pe)
+        lib.nvshmemh_fill_pcg32(p, SRC // 4, 100 + pe, 54, None)
+        img = np.zeros(SRC + DST, np.uint8)
+        img[:SRC] = oracle.pcg32(SRC // 4, 100 + pe).view(np.uint8)
+        images.append(img)
+    hip.sync()
+    return lib, base, images
+
+
+def _reset(lib, hip, base):
+    for pe in range(NPES):
+        hip.memset(lib.nvshmem_ptr(base, pe) + SRC, 0, DST)
+    hip.sync()
+
+
+def _check(lib, hip, base, want):
+    for pe in range(NPES):
+        got = hip.dtoh(lib.nvshmem_ptr(base, pe) + SRC, DST)
+        if not np.array_equal(got, want[pe][SRC:]):
+            bad = np.flatnonzero(got != want[pe][SRC:])
+            raise AssertionError(f"PE {pe}: {bad.size} bytes differ, first at dst offset {bad[0]}")
+
+
+def _expected(oracle, images, reqs):
+    imgs = [im.copy() for im in images]
+    oracle.batch_apply(imgs, reqs, 0, SRC)
+    return imgs
+
+
+def _run(lib, hip, arr, n, times=1):
+    d = device_requests(hip, arr)
+    try:
+        for _ in range(times):
+            assert lib.nvshmemx_batch_put_persistent_on_stream(d, n, None) == 0
+        hip.sync()
+    finally:
+        hip.free(d)
+
+
+def _ragged(rng, n, max_nb, sizes=None):
+    reqs = {"src_pe": rng.integers(0, NPES, n).astype(np.int32), "dst_pe": np.zeros(n, np.int32),
+            "src_off": np.zeros(n, np.uint64), "dst_off": np.zeros(n, np.uint64), "nbytes": np.zeros(n, np.uint64),
+            "dtype": np.zeros(n, np.int32)}
+    cursor = np.zeros(NPES, np.int64)
+    for i in range(n):
+        d = (int(reqs["src_pe"][i]) + 1 + int(rng.integers(0, NPES - 1))) % NPES
+        nb = int(rng.choice(sizes)) if sizes is not None else int(rng.integers(0, max_nb + 1))
+        reqs["dst_pe"][i] = d
+        reqs["nbytes"][i] = nb
+        reqs["src_off"][i] = int(rng.integers(0, SRC - nb - 16))
+        reqs["dst_off"][i] = int(cursor[d]) + int(rng.integers(0, 16))
+        cursor[d] = int(reqs["dst_off"][i]) + nb + 1
+    assert cursor.max() <= DST
+    return reqs
+
+
+@pytest.mark.parametrize("case", ["workload", "ragged", "tiny", "one_large", "repeated"])
+def test_persistent_matches_oracle(setup, hip, oracle, case):
+    lib, base, images = setup
+    rng = np.random.default_rng(31)
+    if case == "workload":
+        reqs = na.gen_requests(lib, 41, 1024, NPES, SRC, DST)
+    elif case == "ragged":
+        reqs = _ragged(rng, 700, 0, sizes=[0, 1, 2, 3, 5, 15, 17, 255, 1023, 4095, 4097, 16383, 16385, 70001,
+                                           131071, 131073, 300007, 1 << 20])
+    elif case == "tiny":
+        reqs = _ragged(rng, 20000, 2048)
+    elif case == "one_large":
+        reqs = {"src_pe": np.array([3], np.int32), "dst_pe": np.array([5], np.int32),
+                "src_off": np.array([4096 + 16], np.uint64), "dst_off": np.array([256 + 16], np.uint64),
+                "nbytes": np.array([(30 << 20) + 12], np.uint64), "dtype": np.zeros(1, np.int32)}
+    else:
+        reqs = na.gen_requests(lib, 43, 512, NPES, SRC, DST)
+    arr = na.make_requests(lib, reqs, base, 0, SRC)
+    want = _expected(oracle, images, reqs)
+    _reset(lib, hip, base)
+    _run(lib, hip, arr, len(arr), times=3 if case == "repeated" else 1)
+    assert lib.nvshmemx_get_last_error() == 0
+    _check(lib, hip, base, want)
+
+
+def test_persistent_private_sources(setup, hip, oracle):
+    """sources outside every heap (a private allocation) count for PE i mod
+    npes; the bytes land as the oracle's puts of those sources say"""
+    lib, base, images = setup
+    rng = np.random.default_rng(5)
+    reqs = na.gen_requests(lib, 45, 256, NPES, SRC, DST)
+    arr = na.make_requests(lib, reqs, base, 0, SRC)
+    priv_bytes = oracle.pcg32((8 << 20) // 4, 999).view(np.uint8)
+    priv = hip.malloc(8 << 20)
+    try:
+        hip.htod(priv, priv_bytes)
+        imgs = [im.copy() for im in images]
+        for i in range(len(arr)):
+            if i % 3 == 0:  # every third request reads the private buffer instead
+                off = int(rng.integers(0, (8 << 20) - int(arr[i].nbytes) + 1))
+                arr[i].source = priv + off
+                src = priv_bytes[off:off + int(arr[i].nbytes)]
+            else:
+                s_pe, s_off = int(reqs["src_pe"][i]), int(reqs["src_off"][i])
+                src = images[s_pe][s_off:s_off + int(reqs["nbytes"][i])]
+            d_pe, d_off = int(reqs["dst_pe"][i]), SRC + int(reqs["dst_off"][i])
+            imgs[d_pe][d_off:d_off + src.size] = src
+        _reset(lib, hip, base)
+        _run(lib, hip, arr, len(arr))
+        assert lib.nvshmemx_get_last_error() == 0
+        _check(lib, hip, base, imgs)
+    finally:
+        hip.free(priv)
+
+
+def test_persistent_empty_and_invalid(setup, hip, oracle):
+    lib, base, images = setup
+    _reset(lib, hip, base)
+    assert lib.nvshmemx_batch_put_persistent_on_stream(None, 0, None) == 0
+    hip.sync()
+    _check(lib, hip, base, images)
+    reqs = na.gen_requests(lib, 47, 64, NPES, SRC, DST)
+    arr = na.make_requests(lib, reqs, base, 0, SRC)
+    want = _expected(oracle, images, {k: v[2:] for k, v in reqs.items()})
+    arr[0].pe = NPES + 3                      # no such PE
+    arr[1].target = arr[1].target + (200 << 20)  # past the end of the heap
+    _reset(lib, hip, base)
+    _run(lib, hip, arr, len(arr))
+    err = lib.nvshmemx_get_last_error()
+    lib.nvshmemx_clear_errors()
+    assert err & na.ERR_BAD_PE and err & na.ERR_BAD_ADDRESS, hex(err)
+    _check(lib, hip, base, want)
+
+
+def test_persistent_between_processes():
+    """two processes, each the only PE of its own kernel: every process's puts
+    (sources in its own heap) land in the other's IPC-mapped heap as the
+    oracle's batch says"""
+    outs = run_job(2, "persistent", 300)
+    for o in outs:
+        assert o["errors"] == 0 and o["batch_ok"], o
