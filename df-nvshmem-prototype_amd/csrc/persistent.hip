@@ -51,8 +51,11 @@
  * Scratch (per launch, device memory): the control slots, the chunk table
  * (65,536 entries; the chunk grows from 128 KiB so that a batch of any size
  * fits) and one 8-byte entry per request.
- * nvshmemx_batch_put_persistent_on_stream allocates it (grown on demand).
- * Calls that overlap on one GPU share the scratch and race.
+ * nvshmemx_batch_put_persistent_on_stream allocates it (grown on demand);
+ * nvshmemx_batch_put_persistent_noalloc_on_stream uses a static array of this
+ * file instead (up to 65,536 requests, 1.25 MiB per GPU, held until the
+ * process ends), for callers that cannot allocate.  Calls that overlap on one
+ * GPU share the scratch and race.
  */
 #include <algorithm>
 
@@ -77,6 +80,7 @@ constexpr size_t kWaveSegMax = 16384; /* shorter request segments go to one wave
 constexpr int kPUnroll = NVSHMEMI_PUNROLL;                 /* loads in flight per lane */
 constexpr unsigned kTableChunks = 65536;                   /* chunk-table entries */
 constexpr unsigned kLeadCache = 4096;                      /* requests a leader keeps in LDS between passes */
+constexpr int kMaxReqsNoalloc = 65536;                     /* requests per call without allocation */
 constexpr int kPPerCU = 2;                                 /* workgroups per CU (see the launch shape above) */
 constexpr int kMaxDevices = 64;
 
@@ -102,6 +106,11 @@ constexpr size_t kCtlBytes = kDoneOff + 256; /* zero between launches */
 constexpr size_t kTableOff = kCtlBytes;
 constexpr size_t kEntryOff = kTableOff + sizeof(unsigned) * kTableChunks;
 constexpr size_t scratch_bytes(size_t nreq) { return kEntryOff + sizeof(pentry) * (nreq ? nreq : 1); }
+
+/* the no-allocation entry point's scratch: each GPU holds its own copy.  In
+ * host code this name is only a stand-in: kernels get the address that
+ * hipGetSymbolAddress returned for the current device. */
+static __device__ __align__(256) unsigned char scratch[scratch_bytes(kMaxReqsNoalloc)];
 
 struct pscr {
     unsigned char *base;
@@ -404,7 +413,8 @@ __global__ void k_persist_clear(unsigned *ctl, size_t n) {
 /* --------------------------------------------------------------- host -- */
 
 struct PersistDevice {
-    unsigned char *alloc_ptr = nullptr; /* library-allocated scratch */
+    unsigned char *static_ptr = nullptr; /* this device's copy of `scratch` */
+    unsigned char *alloc_ptr = nullptr;  /* library-allocated scratch */
     size_t alloc_reqs = 0;
     int capacity = 0; /* workgroups of the persistent kernel resident at once */
 };
@@ -420,10 +430,20 @@ static void clear_ctl(unsigned char *base) {
     NVSHMEMI_HIP(hipGetLastError());
 }
 
-/* Library set-up, once per device with that device current: how many
- * workgroups of the kernel the device runs at once (at most kPPerCU per CU). */
+/* Library set-up, once per device with that device current: the address of
+ * this device's copy of `scratch` (its control slots cleared by a fill
+ * kernel), and how many workgroups of the kernel the device runs at once (at
+ * most kPPerCU per CU). */
 void persistent_init() {
     PersistDevice &d = pdev();
+    if (!d.static_ptr) {
+        void *ptr = nullptr;
+        NVSHMEMI_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) NVSHMEMI_FATAL("hipGetSymbolAddress returned no address for the persistent kernel's scratch");
+        d.static_ptr = (unsigned char *)ptr;
+        clear_ctl(d.static_ptr);
+        NVSHMEMI_HIP(hipDeviceSynchronize());
+    }
     if (!d.capacity) {
         int per_cu = 0, cus = 0;
         NVSHMEMI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g.device));
@@ -470,6 +490,26 @@ extern "C" int nvshmemx_batch_put_persistent_on_stream(const nvshmemx_put_reques
     persistent_launch(reqs, nreq, d.alloc_ptr, s);
     return 0;
 }
+
+extern "C" int nvshmemx_batch_put_persistent_noalloc_on_stream(const nvshmemx_put_request_t *reqs, int nreq,
+                                                                hipStream_t s) {
+    check_args(reqs, nreq, "nvshmemx_batch_put_persistent_noalloc_on_stream");
+    if (nreq > kMaxReqsNoalloc) {
+        fprintf(stderr,
+                "[nvshmem] [%d] nvshmemx_batch_put_persistent_noalloc_on_stream: %d requests exceed the %d its "
+                "static scratch holds; nothing was launched\n",
+                g.mype, nreq, kMaxReqsNoalloc);
+        fflush(stderr);
+        return -1;
+    }
+    if (nreq == 0) return 0;
+    const PersistDevice &d = pdev();
+    if (!d.static_ptr) NVSHMEMI_FATAL("nvshmemx_batch_put_persistent_noalloc_on_stream: the library's set-up did not run");
+    persistent_launch(reqs, nreq, d.static_ptr, s);
+    return 0;
+}
+
+extern "C" int nvshmemx_batch_persistent_max_requests_noalloc(void) { return kMaxReqsNoalloc; }
 
 extern "C" int nvshmemx_batch_persistent_grid(void) {
     require_init("nvshmemx_batch_persistent_grid");

@@ -145,6 +145,8 @@ SIGNATURES.update({
     "nvshmemh_verify_requests": (_U64, [_P, _I]),
     "nvshmemx_batch_put_persistent_on_stream": (_I, [_P, _I, _P]),
     "nvshmemx_batch_persistent_grid": (_I, []),
+    "nvshmemx_batch_put_persistent_noalloc_on_stream": (_I, [_P, _I, _P]),
+    "nvshmemx_batch_persistent_max_requests_noalloc": (_I, []),
     "nvshmemx_tune_candidate_count": (_I, []),
     "nvshmemx_tune_candidate": (None, [_I, _P]),
     "nvshmemx_align_class": (_I, [_P, _P]),

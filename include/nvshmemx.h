@@ -202,6 +202,20 @@ void nvshmemx_put_requests_on_stream(const nvshmemx_put_request_t *reqs, int nre
 int nvshmemx_batch_put_persistent_on_stream(const nvshmemx_put_request_t *reqs, int nreq, hipStream_t stream);
 /* workgroups of the persistent kernel's grid (every one resident at once) */
 int nvshmemx_batch_persistent_grid(void);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same kernel and results as
+ * nvshmemx_batch_put_persistent_on_stream, with its scratch in a static
+ * per-GPU array of the library (1.25 MiB, set up by nvstart_pes /
+ * nvshmemx_start_local_pes) instead of allocated memory; the call itself
+ * allocates nothing and only launches the kernel, so even a first call can be
+ * captured.  At most nvshmemx_batch_persistent_max_requests_noalloc()
+ * (65,536) requests: a larger list is refused with a message and -1, nothing
+ * launched.  Every call on a GPU shares that GPU's array (the kernel clears
+ * its counters at the end of each launch); calls that overlap on one GPU
+ * race.  Returns 0. */
+int nvshmemx_batch_put_persistent_noalloc_on_stream(const nvshmemx_put_request_t *reqs, int nreq, hipStream_t stream);
+int nvshmemx_batch_persistent_max_requests_noalloc(void);
 
 /* ---- exhaustive autotuner of batch choices ---- */
 
