@@ -55,7 +55,19 @@ def test_jacobi_local_bitexact(local_pes, hip, oracle, n, steps, chunk):
                                  f"{blk[tuple(bad[0])]!r}")
 
 
-@pytest.mark.parametrize("n", [16, 48, 13, 130])
+@pytest.mark.parametrize("n,steps,chunk", [(128, 10, None), (128, 5, 2), (128, 4, 5), (128, 3, 127)])
+def test_jacobi_chunks_bitexact(local_pes, hip, oracle, n, steps, chunk):
+    """whole tiles in j and k with chunks of planes from 2 to 127 (a ragged
+    last chunk) and the whole column"""
+    lib = local_pes.start(8, heap="320M")
+    out = _run_local(lib, hip, n, steps, chunk)
+    want = oracle.jacobi(2 * n, steps, SEED)
+    for pe in range(8):
+        got = na.jacobi_interior(hip, lib.nvshmem_ptr(out, pe), n, lib)
+        np.testing.assert_array_equal(got.view(np.uint64), na.jacobi_block(want, pe, n).view(np.uint64))
+
+
+@pytest.mark.parametrize("n", [16, 48, 13, 130, 128])
 def test_jacobi_separate_kernels_bitexact(local_pes, hip, oracle, n):
     """the unfused step (pack+put of u's faces, unpack into u's halo, stencil
     reading the halo from u) gives the same bits as the fused step; n = 48 has
@@ -83,12 +95,12 @@ def test_jacobi_separate_kernels_bitexact(local_pes, hip, oracle, n):
         np.testing.assert_array_equal(got.view(np.uint64), na.jacobi_block(want, pe, n).view(np.uint64))
 
 
-def test_jacobi_processes_bitexact(tmp_path, oracle):
-    n, steps = 16, 40
+@pytest.mark.parametrize("n,steps", [(16, 40), (128, 6)])
+def test_jacobi_processes_bitexact(tmp_path, oracle, n, steps):
     name = f"nvshmem_jacobi_{os.getpid()}_{uuid.uuid4().hex[:8]}"
     procs = [subprocess.Popen([sys.executable, str(HERE / "pe_worker.py"), "jacobi", str(n), str(steps),
                                str(tmp_path)],
-                              env=na.shm_job_env(r, 8, name, {"NVSHMEM_HEAP_SIZE": "16M",
+                              env=na.shm_job_env(r, 8, name, {"NVSHMEM_HEAP_SIZE": "16M" if n <= 16 else "64M",
                                                               "NVSHMEM_WAIT_TIMEOUT_MS": "5000",
                                                               "NVSHMEM_BOOTSTRAP_TIMEOUT_S": "60"}),
                               stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for r in range(8)]
