@@ -54,6 +54,7 @@ template <int W, int NT, int MODE> static copy_kernel_t pick_unroll(int u) {
 }
 
 template <int W, int MODE> static copy_kernel_t pick_nt(int nt, int u) {
+    if (nt & ST_SC1) return (nt & NT_LOAD) ? pick_unroll<W, NT_LOAD | ST_SC1, MODE>(u) : pick_unroll<W, ST_SC1, MODE>(u);
     switch (nt & 3) {
     case NT_LOAD: return pick_unroll<W, NT_LOAD, MODE>(u);
     case NT_STORE: return pick_unroll<W, NT_STORE, MODE>(u);
@@ -74,32 +75,41 @@ static copy_kernel_t pick_copy_kernel(const CopyConfig &c) {
 }
 
 /* Launch shape by size, from sweeps on MI355X of {interleaved, chunked,
- * tiled} x block {256,512,1024} x grid x unroll {1,2,4,8} x streams x NT
- * (copies rotated over a 1 GiB footprint so the 256 MB MALL cannot serve
- * them; means of three passes):
- *   >= 512 MiB        tiled, 256 threads x 2 accesses (8 KiB tiles), 1 run: 6.5 TB/s at 1 GiB,
+ * tiled} x block {256,512,1024} x grid x unroll {1,2,4,8} x streams x store
+ * policy (copies rotated over a 1 GiB footprint so the 256 MB MALL cannot
+ * serve them; streams of back-to-back launches):
+ *   >= 256 MiB        tiled, 256 threads x 2 accesses (8 KiB tiles):    6.5 TB/s at 256 MiB, 6.6 at 512 MiB - 1 GiB,
  *                     also into a peer process's IPC-mapped heap
- *   128 .. 512 MiB    interleaved, 16384 x 512, 8 in flight:             6.4 TB/s at 256 MiB
- *   32 .. 128 MiB     interleaved, 4096 x 512, 8 in flight:              5.9 TB/s at 64 MiB
+ *   128 .. 256 MiB    tiled, 512 threads x 2 accesses (16 KiB tiles):   6.26 TB/s at 128 MiB
+ *   32 .. 128 MiB     interleaved, 8192 x 512, 2 in flight:            6.04 TB/s at 64 MiB
  *   below             interleaved, grid sized to the copy (<= 4096)
- * Non-temporal loads and stores won at every size >= 8 MiB. */
+ * From 32 MiB up, loads are non-temporal and stores are written through the
+ * L2 (sc1): a kernel then leaves no dirty lines for its end-of-kernel
+ * release, worth 1.5-3% over non-temporal stores at 64 MiB - 1 GiB (64 MiB
+ * 5.93 -> 6.04, 128 MiB 6.13 -> 6.26, 1 GiB 6.5 -> 6.6 TB/s, same box).
+ * No shape or policy brought 64 MiB to 6.4 TB/s (80% of 8 TB/s): each launch
+ * pays about 1.5 us of ramp, tail and release, 7% of a 21 us copy.  Puts
+ * alternated over two or more streams (they may overlap: puts are unordered
+ * until quiet) reached 6.12 TB/s at 64 MiB with two and lost at 256 MiB, so
+ * the library keeps one stream. */
 CopyConfig default_copy_config(size_t nbytes) {
     CopyConfig c;
     c.streams = 1;
     c.vec_bytes = 16;
-    c.nontemporal = NT_BOTH;
-    if (nbytes >= ((size_t)512 << 20)) {
+    c.nontemporal = NT_LOAD | ST_SC1;
+    if (nbytes >= ((size_t)256 << 20)) {
         c.mode = 2, c.block = 256, c.workgroups = 0, c.unroll = 2, c.streams = 1;
         return c;
     }
     if (nbytes >= ((size_t)128 << 20)) {
-        c.mode = 0, c.block = 512, c.workgroups = 16384, c.unroll = 8;
+        c.mode = 2, c.block = 512, c.workgroups = 0, c.unroll = 2, c.streams = 1;
         return c;
     }
     if (nbytes >= ((size_t)32 << 20)) {
-        c.mode = 0, c.block = 512, c.workgroups = 4096, c.unroll = 8;
+        c.mode = 0, c.block = 512, c.workgroups = 8192, c.unroll = 2;
         return c;
     }
+    c.nontemporal = NT_BOTH;
     c.mode = 0;
     c.block = nbytes >= ((size_t)16 << 20) ? 512 : 256;
     c.unroll = nbytes >= ((size_t)16 << 20) ? 8 : 4;

@@ -193,9 +193,24 @@ template <bool NT, typename V> __device__ __forceinline__ void vstore(V *p, V v)
     else *p = v;
 }
 
-/* Non-temporal policy bits: NT_LOAD streams loads past the caches' normal
- * retention, NT_STORE does the same for stores (global_load/store ... nt). */
-enum { NT_NONE = 0, NT_LOAD = 1, NT_STORE = 2, NT_BOTH = 3 };
+/* Cache policy bits: NT_LOAD streams loads past the caches' normal
+ * retention, NT_STORE does the same for stores (global_load/store ... nt);
+ * ST_SC1 writes stores through the L2 at device scope (global_store ... sc1)
+ * instead, so a kernel leaves no dirty lines for its end-of-kernel release
+ * to write back. */
+enum { NT_NONE = 0, NT_LOAD = 1, NT_STORE = 2, NT_BOTH = 3, ST_SC1 = 4 };
+
+/* a store under the policy bits of P (ST_SC1 takes precedence over NT_STORE;
+ * the write-through form exists for 4-, 8- and 16-byte vectors) */
+template <int P, typename V> __device__ __forceinline__ void vstore_p(V *p, V v) {
+    if constexpr ((P & ST_SC1) != 0 && sizeof(V) == 16)
+        asm volatile("global_store_dwordx4 %0, %1, off sc1" : : "v"(p), "v"(v) : "memory");
+    else if constexpr ((P & ST_SC1) != 0 && sizeof(V) == 8)
+        asm volatile("global_store_dwordx2 %0, %1, off sc1" : : "v"(p), "v"(v) : "memory");
+    else if constexpr ((P & ST_SC1) != 0 && sizeof(V) == 4)
+        asm volatile("global_store_dword %0, %1, off sc1" : : "v"(p), "v"(v) : "memory");
+    else vstore<(P & NT_STORE) != 0>(p, v);
+}
 
 /* Stream vectors [lo, hi) with `n` cooperating lanes (lane index `lane`),
  * UNROLL independent loads in flight per lane; lanes are interleaved so each
@@ -210,10 +225,10 @@ __device__ __forceinline__ void stream_vectors(V *d, const V *s, size_t lo, size
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) r[u] = vload<(NT & NT_LOAD) != 0>(s + i + u * n);
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) vstore<(NT & NT_STORE) != 0>(d + i + u * n, r[u]);
+            for (int u = 0; u < UNROLL; ++u) vstore_p<NT>(d + i + u * n, r[u]);
         }
     }
-    for (; i < hi; i += n) vstore<(NT & NT_STORE) != 0>(d + i, vload<(NT & NT_LOAD) != 0>(s + i));
+    for (; i < hi; i += n) vstore_p<NT>(d + i, vload<(NT & NT_LOAD) != 0>(s + i));
 }
 
 /* Split of a W-aligned copy: `head` unaligned leading bytes, `nvec` W-byte
@@ -288,7 +303,7 @@ __device__ __forceinline__ void copy_w_tiled(char *dst, const char *src, size_t 
             if (base + u * n < sp.nvec) r[u] = vload<(NT & NT_LOAD) != 0>(s + base + u * n);
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u)
-            if (base + u * n < sp.nvec) vstore<(NT & NT_STORE) != 0>(d + base + u * n, r[u]);
+            if (base + u * n < sp.nvec) vstore_p<NT>(d + base + u * n, r[u]);
     }
 }
 

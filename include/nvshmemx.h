@@ -82,7 +82,7 @@ typedef struct nvshmemx_copy_config {
     int block;       /* threads per workgroup (64..1024, multiple of 64) */
     int vec_bytes;   /* widest access: 4, 8 or 16 */
     int unroll;      /* independent accesses in flight per lane: 1, 2, 4 or 8 */
-    int nontemporal; /* bit 0: non-temporal loads, bit 1: non-temporal stores */
+    int nontemporal; /* bit 0: non-temporal loads, bit 1: non-temporal stores, bit 2: stores written through the L2 (sc1) instead */
     int mode;        /* 0: lanes of the whole grid interleave; 1: one contiguous slice per workgroup;
                         2: tiles of unroll * block accesses alternating between `streams` address runs
                            (workgroups 0: one tile per workgroup) */
