@@ -150,17 +150,19 @@ def batch_worker(counter_pass=False):
 
 def learned_worker(n_batches, train_steps):
     """Learned config choice on the uneven-put workload (64 workgroup PEs):
-    1. exhaustive autotuner: every one of the 54 candidates measured at every
-       point of a 1782-point feature grid (96,228 samples);
-    2. the MLP cost model trained on those samples on the GPU;
-    3. `n_batches` held-out batches (seeds 1000..) of 4096 puts each, planned
-       with the MLP's choices, the table's choices (exhaustive autotuner) and
-       the library's untuned rule, each timed as one fused launch; the per-call
-       path is timed on the first 10;
-    4. 100 batches each of 16 and 128 puts from the same size distribution,
-       where how many puts share the launch decides the best split.
-    Every batch is also selected on the device: the plans of the three choices timed on the
-    device by nvshmemx_batch_plan_select, the fastest kept ("selected").
+    1. per-request autotuner: every one of the 54 candidates measured at every
+       point of a 1782-point feature grid (96,228 samples), and the
+       per-request MLP trained on them on the GPU;
+    2. batch-level autotuner: the 12 batch policies (the rule, the
+       per-request network's choices, uniform wavefront / workgroup pieces of
+       4 KiB .. 1 MiB, one piece per request) of self-generated batches of 8
+       to 8192 puts timed on the device, and the batch-level MLP trained on
+       them (inputs: log2 batch bytes, log2 request count, mean and largest
+       log2 request size);
+    3. held-out batches of 4096 (`n_batches`), 128 and 16 puts: every policy
+       timed on the device (the exhaustive batch-level judge) and the batch
+       network's pick read from the same timing; the per-call path timed on
+       the first 10 4096-put batches.
     Prints one JSON line."""
     import ctypes as C
     import numpy as np
@@ -192,105 +194,83 @@ def learned_worker(n_batches, train_steps):
     out["grid_top1_agreement"] = round(float(np.mean(pred.argmin(axis=1) == times.argmin(axis=1))), 3)
     out["grid_mean_regret"] = round(float(np.mean(regret)), 4)
 
+    def batch_of(seed, nreq, hi):
+        try:
+            reqs = na.gen_requests(lib, seed, nreq, BATCH_NPES, BATCH_SRC, BATCH_DST, 6, hi)
+        except ValueError:
+            return None
+        return na.make_requests(lib, reqs, base, 0, BATCH_SRC)
+
+    # batch-level training set, timed by the library's exhaustive judge
+    t0 = time.time()
+    X, Y = [], []
+    for nreq in (8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192):
+        for hi in (16, 18, 20, 22):
+            for sd in range(6):
+                arr = batch_of(20000 + 1000 * nreq + 10 * hi + sd, nreq, hi)
+                if arr is None:
+                    continue
+                ms, _ = na.measure_batch(lib, arr, len(arr), model, 5)
+                X.append(na.batch_features(lib, arr, len(arr)))
+                Y.append(np.log(ms * 1e3))
+    X, Y = np.array(X, np.float32), np.array(Y, np.float32)
+    npol = lib.nvshmemx_batch_policy_count()
+    bmodel = na.CostModel(lib, seed=3, ncand=npol)
+    untrained = na.CostModel(lib, seed=3, ncand=npol)
+    bmodel.train(X, Y, 6000, 3e-3)
+    bmodel.train(X, Y, 3000, 1e-3)
+    bpred = bmodel.predict(X)
+    out["batch_training"] = {
+        "batches": int(len(X)), "policies": int(npol), "seconds": round(time.time() - t0, 1),
+        "mse_log": round(float(np.mean((bpred - Y) ** 2)), 5),
+        "mse_log_untrained": round(float(np.mean((untrained.predict(X) - Y) ** 2)), 5),
+        "fit_regret": round(float(np.mean(np.exp(Y[np.arange(len(Y)), bpred.argmin(1)] - Y.min(1)))), 4)}
+    untrained.close()
+
     def timed_launches(fn):
-        """ms per launch over back-to-back launches spanning about 50 us (2 to
-        64 of them), so that a batch of a few microseconds is not timed by
-        the events' resolution and launch gaps"""
         first = timed(hip, fn, 1, 1)
         return timed(hip, fn, int(min(64, max(2, math.ceil(0.05 / max(first, 1e-4))))), 0)
 
-    def timed_plan(arr, n, choices):
-        plan = C.c_void_p()
-        lib.nvshmemx_batch_plan_create(arr, n, choices, None, C.byref(plan))
-        ms = timed_launches(lambda: lib.nvshmemx_batch_plan_launch(plan, None))
-        lib.nvshmemx_batch_plan_destroy(plan)
-        return ms
-
-    def selected_index(arr, n, choice_sets):
-        """plans for every choice set, timed by the library's selection on the
-        device (3 rounds); returns the index of the one it picks.  The time
-        recorded for "selected" is the picked method's time measured in the
-        same batch like every other method's, so the comparison is between
-        decisions, not between two timings of one plan."""
-        plans = [C.c_void_p() for _ in choice_sets]
-        for p, ch in zip(plans, choice_sets):
-            lib.nvshmemx_batch_plan_create(arr, n, ch, None, C.byref(p))
-        best = lib.nvshmemx_batch_plan_select((C.c_void_p * len(plans))(*[p.value for p in plans]), len(plans), 3,
-                                             None, None)
-        for p in plans:
-            lib.nvshmemx_batch_plan_destroy(p)
-        return best
-
-    select_names = ("mlp", "table", "rule")
-    rec = {"mlp": [], "table": [], "rule": [], "selected": [], "one_piece": [], "per_call": []}
-    picks = {k: 0 for k in select_names}
-    choose_ms = []
-    bad = 0
+    # held-out batches: the judge (every policy timed on the device), the
+    # batch network's pick, the per-request network, the rule
+    held = {}
+    per_call, bad = [], 0
     t0 = time.time()
-    for b in range(n_batches):
-        reqs = na.gen_requests(lib, 1000 + b, BATCH_REQS, BATCH_NPES, BATCH_SRC, BATCH_DST)
-        arr = na.make_requests(lib, reqs, base, 0, BATCH_SRC)
-        n = len(arr)
-        tc = time.perf_counter()
-        ch_mlp = model.choose(arr, n, 0)          # per-request effective concurrency
-        choose_ms.append((time.perf_counter() - tc) * 1e3)
-        ch_tab = (na.BatchChoice * n)()
-        lib.nvshmemx_tune_table_choose(feats.ctypes.data, times.ctypes.data, len(times), arr, n, 0, ch_tab)
-        one = (na.BatchChoice * n)()
-        for i in range(n):
-            one[i].pieces, one[i].vec_bytes, one[i].mode = 1, 16, 1
-        rec["mlp"].append(timed_plan(arr, n, ch_mlp))
-        rec["table"].append(timed_plan(arr, n, ch_tab))
-        rec["rule"].append(timed_plan(arr, n, None))
-        jb = selected_index(arr, n, (ch_mlp, ch_tab, None))
-        rec["selected"].append(rec[select_names[jb]][-1])
-        picks[select_names[jb]] += 1
-        rec["one_piece"].append(timed_plan(arr, n, one))
-        if b < 10:
-            rec["per_call"].append(timed(hip, lambda: lib.nvshmemx_put_requests_on_stream(arr, n, None), 1, 0))
-        if b < 3:
-            bad += int(lib.nvshmemh_verify_requests(arr, n))
-    # fewer concurrent puts from the same size distribution: the concurrency
-    # input is what the untuned rule cannot see
-    small = {}
-    for nreq in (16, 128):
-        rs = {"mlp": [], "table": [], "rule": [], "selected": []}
-        spicks = {k: 0 for k in select_names}
-        for b in range(100):
-            reqs = na.gen_requests(lib, 5000 + 1000 * nreq + b, nreq, BATCH_NPES, BATCH_SRC, BATCH_DST)
-            arr = na.make_requests(lib, reqs, base, 0, BATCH_SRC)
-            ch_mlp = model.choose(arr, nreq, 0)
-            ch_tab = (na.BatchChoice * nreq)()
-            lib.nvshmemx_tune_table_choose(feats.ctypes.data, times.ctypes.data, len(times), arr, nreq, 0, ch_tab)
-            rs["mlp"].append(timed_plan(arr, nreq, ch_mlp))
-            rs["table"].append(timed_plan(arr, nreq, ch_tab))
-            rs["rule"].append(timed_plan(arr, nreq, None))
-            jb = selected_index(arr, nreq, (ch_mlp, ch_tab, None))
-            rs["selected"].append(rs[select_names[jb]][-1])
-            spicks[select_names[jb]] += 1
-        small[str(nreq)] = {"mean_ms": {k: round(float(np.mean(v)), 4) for k, v in rs.items()},
-                            "mlp_over_table": round(float(np.mean(rs["mlp"]) / np.mean(rs["table"])), 4),
-                            "mlp_gain_over_rule": round(float(np.mean(rs["rule"]) / np.mean(rs["mlp"])), 4),
-                            "selected_gain_over_rule": round(float(np.mean(rs["rule"]) / np.mean(rs["selected"])), 4),
-                            "selection_picks": spicks}
-    out["fewer_puts"] = small
+    for nreq, nb, seed0 in ((BATCH_REQS, n_batches, 1000), (128, 100, 5128000), (16, 100, 5016000)):
+        r = {"judge": [], "batch_net": [], "request_net": [], "rule": [], "one_piece": []}
+        picks = np.zeros(npol, np.int64)
+        for b in range(nb):
+            arr = batch_of(seed0 + b, nreq, 22)
+            n = len(arr)
+            ms, _ = na.measure_batch(lib, arr, n, model, 5)
+            _, pol = bmodel.choose_batch(model, arr, n)
+            picks[pol] += 1
+            r["judge"].append(float(ms.min()))
+            r["batch_net"].append(float(ms[pol]))
+            r["request_net"].append(float(ms[1]))
+            r["rule"].append(float(ms[0]))
+            r["one_piece"].append(float(ms[npol - 1]))
+            if nreq == BATCH_REQS and b < 10:
+                per_call.append(timed(hip, lambda: lib.nvshmemx_put_requests_on_stream(arr, n, None), 1, 0))
+            if b < 2:
+                bad += int(lib.nvshmemh_verify_requests(arr, n))
+        j = np.array(r["judge"])
+        rec = {"mean_ms": {k: round(float(np.mean(v)), 4) for k, v in r.items()},
+               "batch_net_over_judge": round(float(np.mean(np.array(r["batch_net"]) / j)), 4),
+               "batch_net_within_5pct_frac": round(float(np.mean(np.array(r["batch_net"]) <= 1.05 * j)), 3),
+               "request_net_over_judge": round(float(np.mean(np.array(r["request_net"]) / j)), 4),
+               "rule_over_judge": round(float(np.mean(np.array(r["rule"]) / j)), 4),
+               "batch_net_gain_over_rule": round(float(np.mean(r["rule"]) / np.mean(r["batch_net"])), 4),
+               "request_net_gain_over_rule": round(float(np.mean(r["rule"]) / np.mean(r["request_net"])), 4),
+               "batches": nb, "policy_picks": picks.tolist()}
+        if nreq == BATCH_REQS:
+            rec["batch_net_speedup_vs_per_call"] = round(float(np.mean(per_call) / np.mean(r["batch_net"][:len(per_call)])), 2)
+        held[str(nreq)] = rec
+    out["held_out"] = held
     out["eval_seconds"] = round(time.time() - t0, 1)
-    m, t, r = (np.array(rec[k]) for k in ("mlp", "table", "rule"))
-    pc = np.array(rec["per_call"])
-    out.update({
-        "mean_ms": {k: round(float(np.mean(v)), 4) for k, v in rec.items()},
-        "mlp_over_table": round(float(np.mean(m) / np.mean(t)), 4),
-        "mlp_within_5pct_of_table_frac": round(float(np.mean(m <= 1.05 * t)), 3),
-        "mlp_gain_over_rule": round(float(np.mean(r) / np.mean(m)), 4),
-        "mlp_gain_over_one_piece": round(float(np.mean(rec["one_piece"]) / np.mean(m)), 4),
-        "mlp_speedup_vs_per_call": round(float(np.mean(pc) / np.mean(m[:len(pc)])), 2),
-        "mlp_choose_ms_per_batch": round(float(np.median(choose_ms)), 3),
-        "selected_gain_over_rule": round(float(np.mean(r) / np.mean(rec["selected"])), 4),
-        "selected_over_table": round(float(np.mean(rec["selected"]) / np.mean(t)), 4),
-        "selection_picks": picks,
-        "bitexact": bad == 0,
-        "errors": lib.nvshmemx_get_last_error(),
-    })
+    out["bitexact"] = bad == 0
+    out["errors"] = lib.nvshmemx_get_last_error()
+    bmodel.close()
     model.close()
     lib.nvstop_pes()
     print(json.dumps(out), flush=True)

@@ -259,6 +259,7 @@ extern "C" int nvshmemx_costmodel_create(int ncand, unsigned long long seed, nvs
 }
 
 extern "C" int nvshmemx_costmodel_num_params(nvshmemx_costmodel_t m) { return m->np; }
+extern "C" int nvshmemx_costmodel_num_outputs(nvshmemx_costmodel_t m) { return m->C; }
 
 extern "C" void nvshmemx_costmodel_get_params(nvshmemx_costmodel_t m, float *out) {
     NVSHMEMI_HIP(hipMemcpy(out, m->d_params, m->np * sizeof(float), hipMemcpyDeviceToHost));

@@ -165,6 +165,12 @@ SIGNATURES.update({
     "nvshmemx_costmodel_save": (_I, [_P, C.c_char_p]),
     "nvshmemx_costmodel_load": (_I, [_P, C.c_char_p]),
     "nvshmemx_costmodel_destroy": (None, [_P]),
+    "nvshmemx_costmodel_num_outputs": (_I, [_P]),
+    "nvshmemx_batch_policy_count": (_I, []),
+    "nvshmemx_batch_features": (None, [_P, _I, _P]),
+    "nvshmemx_batch_policy_choices": (_I, [_P, _I, _I, _P, _P]),
+    "nvshmemx_tune_measure_batch": (_I, [_P, _I, _P, _I, _P]),
+    "nvshmemx_costmodel_choose_batch": (_I, [_P, _P, _P, _I, _P, _P]),
 })
 
 SIGNATURES.update({
@@ -289,6 +295,14 @@ class CostModel:
         self.lib.nvshmemx_costmodel_predict(self.h, X.ctypes.data, len(X), Y.ctypes.data)
         return Y
 
+    def choose_batch(self, request_model, reqs_arr, n):
+        """per-request choices of the policy this batch model predicts fastest, and the policy"""
+        out = (BatchChoice * n)()
+        pol = C.c_int()
+        self.lib.nvshmemx_costmodel_choose_batch(self.h, request_model.h if request_model else None, reqs_arr, n,
+                                                 out, C.byref(pol))
+        return out, pol.value
+
     def choose(self, reqs_arr, n, concurrency):
         out = (BatchChoice * n)()
         self.lib.nvshmemx_costmodel_choose(self.h, reqs_arr, n, concurrency, out)
@@ -298,6 +312,22 @@ class CostModel:
         if self.h:
             self.lib.nvshmemx_costmodel_destroy(self.h)
             self.h = C.c_void_p()
+
+
+def batch_features(lib, reqs_arr, n):
+    import numpy as np
+    f = np.zeros(4, np.float32)
+    lib.nvshmemx_batch_features(reqs_arr, n, f.ctypes.data)
+    return f
+
+
+def measure_batch(lib, reqs_arr, n, request_model=None, reps=5):
+    """every batch policy's plan timed on the device: (ms per policy, fastest policy)"""
+    import numpy as np
+    ms = np.zeros(lib.nvshmemx_batch_policy_count(), np.float32)
+    best = lib.nvshmemx_tune_measure_batch(reqs_arr, n, request_model.h if request_model else None, reps,
+                                           ms.ctypes.data)
+    return ms, best
 
 
 def gen_requests(lib, seed, nreq, npes, src_region, dst_region, min_log2=6, max_log2=22):

@@ -259,6 +259,35 @@ int nvshmemx_costmodel_choose(nvshmemx_costmodel_t model, const nvshmemx_put_req
 int nvshmemx_costmodel_save(nvshmemx_costmodel_t model, const char *path);
 int nvshmemx_costmodel_load(nvshmemx_costmodel_t model, const char *path);
 void nvshmemx_costmodel_destroy(nvshmemx_costmodel_t model);
+int nvshmemx_costmodel_num_outputs(nvshmemx_costmodel_t model);
+
+/* ---- batch-level policy: how a whole batch is cut into pieces ---- */
+
+/* Policies: 0 the library's rule, 1 the per-request model's choices,
+ * 2..5 every request cut into wavefront pieces of about 4/8/16/32 KiB,
+ * 6..10 into workgroup pieces of about 64/128/256/512 KiB and 1 MiB,
+ * 11 one workgroup piece per request. */
+int nvshmemx_batch_policy_count(void);
+/* the four normalised batch-level inputs of the policy model: log2 of the
+ * batch's bytes, of its request count, mean and largest log2 request size */
+void nvshmemx_batch_features(const nvshmemx_put_request_t *reqs, int nreq, float *features);
+/* per-request choices of `policy` (policy 1 needs the per-request model) */
+int nvshmemx_batch_policy_choices(const nvshmemx_put_request_t *reqs, int nreq, int policy,
+                                  nvshmemx_costmodel_t request_model, nvshmemx_batch_choice_t *out);
+/* Exhaustive batch-level judge: every policy's plan timed on the device
+ * (nvshmemx_batch_plan_select with `reps` rounds; the puts run, so the
+ * sources must hold the batch's data).  ms_out: one time per policy.
+ * Returns the fastest policy.  Without a per-request model policy 1 repeats
+ * the rule. */
+int nvshmemx_tune_measure_batch(const nvshmemx_put_request_t *reqs, int nreq, nvshmemx_costmodel_t request_model,
+                                int reps, float *ms_out);
+/* The policy model's choice (a cost model created with
+ * nvshmemx_batch_policy_count() outputs, trained on batch features against
+ * log-times of the policies): the policy with the smallest predicted time
+ * plans the batch.  policy_out (optional) receives it. */
+int nvshmemx_costmodel_choose_batch(nvshmemx_costmodel_t batch_model, nvshmemx_costmodel_t request_model,
+                                    const nvshmemx_put_request_t *reqs, int nreq, nvshmemx_batch_choice_t *out,
+                                    int *policy_out);
 
 /* Tracing (library built with NVSHMEM_TRACING): drain the ring now; report
  * whether records are being kept.  Both are no-ops / 0 otherwise. */
