@@ -82,3 +82,45 @@ def test_table_choice_prefers_longer_pieces_within_tolerance(lib):
     assert _choose(lib, feats, times, [1 << 20]) == [_cand(lib, 45)]
     times[0, 45] = 1.06
     assert _choose(lib, feats, times, [1 << 20]) == [_cand(lib, 8)]
+
+
+def _reqs(sizes):
+    arr = (na.PutRequest * len(sizes))()
+    for i, s in enumerate(sizes):
+        arr[i].target, arr[i].source, arr[i].nbytes, arr[i].pe, arr[i].elem_bytes = 0x10000, 0x20000, s, 0, 4
+    return arr
+
+
+def test_batch_features(lib):
+    """closed form: log2 of the batch's bytes and of its request count, mean
+    and largest log2 request size, each shifted and scaled as documented"""
+    sizes = [64, 4096, 1 << 20, 3]
+    f = na.batch_features(lib, _reqs(sizes), len(sizes))
+    logs = np.log2(np.maximum(sizes, 1))
+    want = [(np.log2(sum(sizes)) - 24) / 8, (np.log2(len(sizes)) - 7) / 4, (logs.mean() - 14) / 8, (logs.max() - 14) / 8]
+    np.testing.assert_allclose(f, want, rtol=1e-6, atol=1e-6)
+
+
+def test_batch_policy_choices(lib):
+    """the rule's choices equal nvshmemx_batch_default_choice; a uniform
+    policy cuts a request into ceil(bytes / granularity) pieces (a wavefront
+    policy counts four wavefront pieces per piece), 1..256, 16-byte accesses;
+    the last policy is one workgroup piece per request"""
+    assert lib.nvshmemx_batch_policy_count() == 12
+    sizes = [1, 4095, 4096, 65536, 65537, 1 << 20, (1 << 22) - 4, 1 << 30]
+    arr, n = _reqs(sizes), len(sizes)
+    out = (na.BatchChoice * n)()
+    lib.nvshmemx_batch_policy_choices(arr, n, 0, None, out)
+    for i, s in enumerate(sizes):
+        d = na.BatchChoice()
+        lib.nvshmemx_batch_default_choice(s, 4, C.byref(d))
+        assert (out[i].pieces, out[i].vec_bytes, out[i].mode) == (d.pieces, d.vec_bytes, d.mode)
+    grans = {2: (4 << 10, 0), 3: (8 << 10, 0), 4: (16 << 10, 0), 5: (32 << 10, 0), 6: (64 << 10, 1), 7: (128 << 10, 1),
+             8: (256 << 10, 1), 9: (512 << 10, 1), 10: (1 << 20, 1)}
+    for p, (g, mode) in grans.items():
+        lib.nvshmemx_batch_policy_choices(arr, n, p, None, out)
+        per = g * (4 if mode == 0 else 1)
+        for i, s in enumerate(sizes):
+            assert (out[i].pieces, out[i].vec_bytes, out[i].mode) == (max(1, min(256, -(-s // per))), 16, mode), (p, s)
+    lib.nvshmemx_batch_policy_choices(arr, n, 11, None, out)
+    assert all((out[i].pieces, out[i].mode) == (1, 1) for i in range(n))
