@@ -93,10 +93,13 @@ def batch_worker(counter_pass=False):
     """Uneven-put workload on 64 workgroup-partitioned PEs in one process:
     4096 puts per step, log-uniform 64 B..4 MiB, seed 7.  Times one fused
     launch, the same under a captured hipGraph, the owner discipline (each PE's
-    workgroups run the puts it initiated) and the per-call path (one launch per
-    put).  Prints one JSON line.  counter_pass: only five fused launches and
-    one 1 GiB device-to-device copy (the scale of a counter pass)."""
+    workgroups run the puts it initiated), the persistent kernel whose
+    workgroup PEs plan their own puts on the device (library-allocated and
+    static scratch) and the per-call path (one launch per put).  Prints one
+    JSON line.  counter_pass: only five fused launches and one 1 GiB
+    device-to-device copy (the scale of a counter pass)."""
     import ctypes as C
+    import numpy as np
     import nvshmem_amd as na
     os.environ["NVSHMEM_HEAP_SIZE"] = "96M"
     lib, hip = na.load(), na.Hip()
@@ -141,6 +144,18 @@ def batch_worker(counter_pass=False):
     owner = plan_of(na.BatchLaunch(8192, 0, 2, na.BATCH_OWNER))
     out["owner_ms"] = timed(hip, lambda: lib.nvshmemx_batch_plan_launch(owner, None), 10, 2)
     lib.nvshmemx_batch_plan_destroy(owner)
+    # the persistent kernel: each workgroup PE plans its own puts on the
+    # device from the request list in device memory (and the same kernel with
+    # its scratch in the library's static array)
+    raw = np.frombuffer(bytes(arr), np.uint8)
+    d_reqs = hip.malloc(raw.size)
+    hip.htod(d_reqs, raw)
+    out["persistent_ms"] = timed(hip, lambda: lib.nvshmemx_batch_put_persistent_on_stream(d_reqs, n, None), 20, 3)
+    out["persistent_noalloc_ms"] = timed(hip, lambda: lib.nvshmemx_batch_put_persistent_noalloc_on_stream(
+        d_reqs, n, None), 20, 3)
+    out["persistent_grid"] = int(lib.nvshmemx_batch_persistent_grid())
+    out["bitexact"] = out["bitexact"] and int(lib.nvshmemh_verify_requests(arr, n)) == 0
+    hip.free(d_reqs)
     out["per_call_ms"] = timed(hip, lambda: lib.nvshmemx_put_requests_on_stream(arr, n, None), 3, 1)
     out["bitexact"] = out["bitexact"] and int(lib.nvshmemh_verify_requests(arr, n)) == 0
     out["errors"] = lib.nvshmemx_get_last_error()
@@ -327,9 +342,17 @@ def jacobi_worker(layout, name=None, counter_pass=False):
     res = u if JACOBI_STEPS % 2 == 0 else v
     lib.nvshmem_barrier_all()
     s1 = total(res)
+    # unfused steps (pack+put, barrier, unpack, stencil), the faces stored
+    # directly or sent through the library's 2-D and strided device puts; an
+    # even number of steps leaves the result back in u
+    sep = {}
+    for how, api in (("direct", 0), ("device_put_api", 1)):
+        sep[how] = round(timed(hip, lambda: lib.nvshmemh_jacobi_run_separate(u, v, recv, n, 20, chunk, api, None),
+                               1, 0) / 20, 4)
+    lib.nvshmem_barrier_all()
     err = lib.nvshmemx_get_last_error()
     out = {"layout": layout, "pes": 8, "cells_per_pe": n ** 3, "steps": JACOBI_STEPS, "chunk": chunk, "ms_per_step": ms,
-           "sum_before": s0, "sum_after": s1, "errors": err}
+           "sum_before": s0, "sum_after": s1, "unfused_ms_per_step": sep, "errors": err}
     lib.nvstop_pes()
     if pe0 == 0:
         print(json.dumps(out), flush=True)
@@ -473,8 +496,12 @@ def run_batch_leg():
     best = min(rec["fused_ms"], rec["graph_ms"])
     rec["fused_gbps"] = round(2 * rec["payload_bytes"] / (best * 1e-3) / 1e9, 1)
     rec["speedup_vs_per_call"] = round(rec["per_call_ms"] / best, 2)
-    for k in ("fused_ms", "graph_ms", "owner_ms", "per_call_ms"):
-        rec[k] = round(rec[k], 4)
+    for k in ("fused_ms", "graph_ms", "owner_ms", "per_call_ms", "persistent_ms", "persistent_noalloc_ms"):
+        if k in rec:
+            rec[k] = round(rec[k], 4)
+    if "persistent_ms" in rec:
+        rec["persistent_gbps"] = round(2 * rec["payload_bytes"] / (rec["persistent_ms"] * 1e-3) / 1e9, 1)
+        rec["persistent_speedup_vs_per_call"] = round(rec["per_call_ms"] / rec["persistent_ms"], 2)
     return rec
 
 

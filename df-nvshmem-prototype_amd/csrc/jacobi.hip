@@ -122,6 +122,40 @@ __global__ void k_pack_put(double *u_sym, double *recv_sym, int n, int pe0, int 
     dst[((size_t)parity * 6 + (size_t)(d ^ 1)) * (size_t)n * n + cell] = u[g.idx(i, j, k)];
 }
 
+/* The same faces sent through the library's device put API instead of
+ * direct stores: blockIdx.z = PE offset, blockIdx.y = face, blockIdx.x = a
+ * band of kApiBand rows (x and y faces) or planes (z faces).  An x face is n
+ * rows of n contiguous doubles one row stride apart, a y face n rows one
+ * plane stride apart: one pitched 2-D put of the band's rows
+ * (nvshmemx_putmem_2d_block) into the neighbour's receive slot; a z face is
+ * one column of n doubles a row stride apart per plane: one strided put per
+ * plane of the band (nvshmemx_double_iput_block).  The receive slot layout is
+ * k_pack_put's. */
+constexpr int kApiBand = 16;
+__global__ void __launch_bounds__(256) k_pack_put_api(double *u_sym, double *recv_sym, int n, int pe0, int parity) {
+    const geom g(n, pe0 + (int)blockIdx.z);
+    const int d = blockIdx.y;
+    const int r0 = (int)blockIdx.x * kApiBand;
+    if (r0 >= n) return; /* the whole workgroup: the device calls below are workgroup-collective */
+    const int rows = kApiBand < n - r0 ? kApiBand : n - r0;
+    const double *u = (const double *)nvshmemx_ptr_d(u_sym, g.pe);
+    const int plane = (d & 1) ? n : 1;
+    double *slot = recv_sym + ((size_t)parity * 6 + (size_t)(d ^ 1)) * (size_t)n * n; /* symmetric */
+    const int nb = g.neighbour(d);
+    const size_t row_bytes = (size_t)n * sizeof(double);
+    if (d < 2) {
+        nvshmemx_putmem_2d_block(slot + (size_t)r0 * n, row_bytes, u + g.idx(plane, r0 + 1, 1),
+                                 g.row_stride() * sizeof(double), row_bytes, (size_t)rows, nb);
+    } else if (d < 4) {
+        nvshmemx_putmem_2d_block(slot + (size_t)r0 * n, row_bytes, u + g.idx(r0 + 1, plane, 1),
+                                 (size_t)(n + 2) * g.row_stride() * sizeof(double), row_bytes, (size_t)rows, nb);
+    } else {
+        for (int i = r0 + 1; i <= r0 + rows; ++i)
+            nvshmemx_double_iput_block(slot + (size_t)(i - 1) * n, u + g.idx(i, 1, plane), 1,
+                                       (ptrdiff_t)g.row_stride(), (size_t)n, nb);
+    }
+}
+
 __global__ void k_unpack(double *u_sym, const double *recv_sym, int n, int pe0, int parity) {
     const geom g(n, pe0 + (int)blockIdx.z);
     const int s = blockIdx.y;
@@ -370,6 +404,14 @@ void nvshmemh_jacobi_pack_put(double *u_sym, double *recv_sym, int n, int pe0, i
     NVSHMEMI_HIP(hipGetLastError());
 }
 
+/* the faces through the library's 2-D and strided device puts */
+void nvshmemh_jacobi_pack_put_api(double *u_sym, double *recv_sym, int n, int pe0, int npes_here, int parity,
+                                  hipStream_t s) {
+    hipLaunchKernelGGL(k_pack_put_api, dim3((unsigned)((n + kApiBand - 1) / kApiBand), 6, npes_here), dim3(256), 0, s,
+                       u_sym, recv_sym, n, pe0, parity);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
 void nvshmemh_jacobi_unpack(double *u_sym, double *recv_sym, int n, int pe0, int npes_here, int parity,
                             hipStream_t s) {
     hipLaunchKernelGGL(k_unpack, dim3((unsigned)((n * n + 255) / 256), 6, npes_here), dim3(256), 0, s, u_sym,
@@ -409,6 +451,33 @@ double *nvshmemh_jacobi_run(double *u_sym, double *v_sym, double *recv_sym, int 
         const int parity = (first_step + t) & 1;
         nvshmemh_jacobi_stencil_fused(a, b, recv_sym, n, pe0, here, chunk, parity, parity ^ 1, s);
         if (!local) nvshmemx_barrier_all_on_stream(s);
+        double *tmp = a;
+        a = b;
+        b = tmp;
+    }
+    return a;
+}
+
+/* `steps` unfused steps: the faces of the current field put into the
+ * neighbours' receive slots of the step's parity (use_api 0: the direct-store
+ * pack kernel; 1: the library's 2-D and strided device puts), the barrier
+ * (nvshmem_barrier_all_offload between processes, the kernel boundary in
+ * local mode), the halo unpacked into the field, the stencil.  Returns the
+ * buffer holding the result. */
+double *nvshmemh_jacobi_run_separate(double *u_sym, double *v_sym, double *recv_sym, int n, int steps, int chunk,
+                                     int use_api, hipStream_t s) {
+    const bool local = nvshmemx_mode() == NVSHMEMX_MODE_LOCAL;
+    const int pe0 = local ? 0 : nvshmem_my_pe();
+    const int here = local ? nvshmem_n_pes() : 1;
+    if (nvshmem_n_pes() != 8) NVSHMEMI_FATAL("jacobi needs exactly 8 PEs (2x2x2), found %d", nvshmem_n_pes());
+    check_n(n);
+    double *a = u_sym, *b = v_sym;
+    for (int t = 0; t < steps; ++t) {
+        if (use_api) nvshmemh_jacobi_pack_put_api(a, recv_sym, n, pe0, here, t & 1, s);
+        else nvshmemh_jacobi_pack_put(a, recv_sym, n, pe0, here, t & 1, s);
+        if (!local) nvshmemx_barrier_all_on_stream(s);
+        nvshmemh_jacobi_unpack(a, recv_sym, n, pe0, here, t & 1, s);
+        nvshmemh_jacobi_stencil(a, b, n, pe0, here, chunk, s);
         double *tmp = a;
         a = b;
         b = tmp;

@@ -183,6 +183,8 @@ SIGNATURES.update({
     "nvshmemh_jacobi_stencil": (None, [_P, _P, _I, _I, _I, _I, _P]),
     "nvshmemh_jacobi_stencil_fused": (None, [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P]),
     "nvshmemh_jacobi_run": (_P, [_P, _P, _P, _I, _I, _I, _I, _P]),
+    "nvshmemh_jacobi_pack_put_api": (None, [_P, _P, _I, _I, _I, _I, _P]),
+    "nvshmemh_jacobi_run_separate": (_P, [_P, _P, _P, _I, _I, _I, _I, _P]),
 })
 
 

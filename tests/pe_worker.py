@@ -12,7 +12,8 @@ Scenarios (every PE runs the same collective sequence):
            is its own heap (targets in the other processes' IPC-mapped heaps),
            launches it and replays it from a hipGraph; each PE's targets are
            checked bit-exact against the oracle's batch
-  jacobi   the 2x2x2 Jacobi application, one PE per process
+  jacobi   the 2x2x2 Jacobi application, one PE per process (fused steps, or
+           unfused ones whose faces go through the device put API)
   config2  2 PEs: int/float/double x put/get x 8 B .. 2^max_log2 B through the
            host-launched grid copy and the device per-thread calls, each
            landing checked against the oracle with guard bytes
@@ -239,6 +240,7 @@ def main():
         print("continued", flush=True)
     elif scenario == "jacobi":
         n, steps, outdir = int(sys.argv[2]), int(sys.argv[3]), Path(sys.argv[4])
+        how = sys.argv[5] if len(sys.argv) > 5 else "fused"
         fb = lib.nvshmemh_jacobi_field_bytes(n)
         u, v = lib.nvshmalloc(fb), lib.nvshmalloc(fb)
         recv = lib.nvshmalloc(lib.nvshmemh_jacobi_recv_bytes(n))
@@ -247,7 +249,10 @@ def main():
         lib.nvshmemh_jacobi_init(u, n, me, 1, 42, None)
         hip.sync()
         lib.nvshmem_barrier_all()
-        res = lib.nvshmemh_jacobi_run(u, v, recv, n, steps, 0, 0, None)
+        if how == "fused":
+            res = lib.nvshmemh_jacobi_run(u, v, recv, n, steps, 0, 0, None)
+        else:  # unfused steps, faces through the library's device puts
+            res = lib.nvshmemh_jacobi_run_separate(u, v, recv, n, steps, 0, 1, None)
         hip.sync()
         lib.nvshmem_barrier_all()
         np.save(outdir / f"jacobi_pe{me}.npy", na.jacobi_interior(hip, res, n, lib))

@@ -68,10 +68,14 @@ def test_jacobi_chunks_bitexact(local_pes, hip, oracle, n, steps, chunk):
 
 
 @pytest.mark.parametrize("n", [16, 48, 13, 130, 128])
-def test_jacobi_separate_kernels_bitexact(local_pes, hip, oracle, n):
+@pytest.mark.parametrize("pack", ["direct", "api"])
+def test_jacobi_separate_kernels_bitexact(local_pes, hip, oracle, n, pack):
     """the unfused step (pack+put of u's faces, unpack into u's halo, stencil
     reading the halo from u) gives the same bits as the fused step; n = 48 has
-    partial stencil tiles in k, n = 13 and 130 in j and k"""
+    partial stencil tiles in k, n = 13 and 130 in j and k.  The faces go out
+    either by the direct-store pack kernel or through the library's own
+    device puts (pitched 2-D puts for the x and y faces, strided puts for the
+    z faces)"""
     lib = local_pes.start(8, heap="16M" if n <= 48 else "96M")
     fb = lib.nvshmemh_jacobi_field_bytes(n)
     u, v = lib.nvshmalloc(fb), lib.nvshmalloc(fb)
@@ -80,13 +84,8 @@ def test_jacobi_separate_kernels_bitexact(local_pes, hip, oracle, n):
         for p in (u, v):
             hip.memset(lib.nvshmem_ptr(p, pe), 0, fb)
     lib.nvshmemh_jacobi_init(u, n, 0, 8, SEED, None)
-    a, b = u, v
     steps = 6
-    for t in range(steps):
-        lib.nvshmemh_jacobi_pack_put(a, recv, n, 0, 8, t & 1, None)
-        lib.nvshmemh_jacobi_unpack(a, recv, n, 0, 8, t & 1, None)
-        lib.nvshmemh_jacobi_stencil(a, b, n, 0, 8, 7, None)
-        a, b = b, a
+    a = lib.nvshmemh_jacobi_run_separate(u, v, recv, n, steps, 7, 1 if pack == "api" else 0, None)
     hip.sync()
     assert lib.nvshmemx_get_last_error() == 0
     want = oracle.jacobi(2 * n, steps, SEED)
@@ -95,11 +94,14 @@ def test_jacobi_separate_kernels_bitexact(local_pes, hip, oracle, n):
         np.testing.assert_array_equal(got.view(np.uint64), na.jacobi_block(want, pe, n).view(np.uint64))
 
 
-@pytest.mark.parametrize("n,steps", [(16, 40), (128, 6)])
-def test_jacobi_processes_bitexact(tmp_path, oracle, n, steps):
+@pytest.mark.parametrize("n,steps,how", [(16, 40, "fused"), (128, 6, "fused"), (16, 12, "api"), (48, 5, "api")])
+def test_jacobi_processes_bitexact(tmp_path, oracle, n, steps, how):
+    """8 processes, one PE each: the fused step (faces stored straight into
+    the neighbours' receive slots) and the unfused step whose faces go through
+    the library's 2-D and strided device puts, the offload barrier between"""
     name = f"nvshmem_jacobi_{os.getpid()}_{uuid.uuid4().hex[:8]}"
     procs = [subprocess.Popen([sys.executable, str(HERE / "pe_worker.py"), "jacobi", str(n), str(steps),
-                               str(tmp_path)],
+                               str(tmp_path), how],
                               env=na.shm_job_env(r, 8, name, {"NVSHMEM_HEAP_SIZE": "16M" if n <= 16 else "64M",
                                                               "NVSHMEM_WAIT_TIMEOUT_MS": "5000",
                                                               "NVSHMEM_BOOTSTRAP_TIMEOUT_S": "60"}),
