@@ -134,6 +134,7 @@ def batch_worker(counter_pass=False):
         lib.nvstop_pes()
         print(json.dumps(out), flush=True)
         return
+    _log(f"batch: {n} requests, {payload} payload bytes planned")
     out["fused_ms"] = timed(hip, lambda: lib.nvshmemx_batch_plan_launch(plan, None), 20, 3)
     out["pieces"] = int(lib.nvshmemx_batch_plan_pieces(plan))
     out["bitexact"] = int(lib.nvshmemh_verify_requests(arr, n)) == 0
@@ -156,6 +157,7 @@ def batch_worker(counter_pass=False):
     out["persistent_grid"] = int(lib.nvshmemx_batch_persistent_grid())
     out["bitexact"] = out["bitexact"] and int(lib.nvshmemh_verify_requests(arr, n)) == 0
     hip.free(d_reqs)
+    _log("batch: fused, graph, owner and persistent timed; per-call path next")
     out["per_call_ms"] = timed(hip, lambda: lib.nvshmemx_put_requests_on_stream(arr, n, None), 3, 1)
     out["bitexact"] = out["bitexact"] and int(lib.nvshmemh_verify_requests(arr, n)) == 0
     out["errors"] = lib.nvshmemx_get_last_error()
@@ -196,6 +198,7 @@ def learned_worker(n_batches, train_steps):
     feats = na.tune_features(lib, grid)
     out["tune_points"], out["tune_samples"] = int(len(times)), int(times.size)
     out["tune_seconds"] = round(time.time() - t0, 1)
+    _log(f"learned: {len(times)} feature points x 54 candidates measured in {out['tune_seconds']} s")
     t0 = time.time()
     model = na.CostModel(lib, seed=1)
     T = np.log(np.maximum(times, 1e-3)).astype(np.float32)
@@ -241,6 +244,7 @@ def learned_worker(n_batches, train_steps):
         "mse_log_untrained": round(float(np.mean((untrained.predict(X) - Y) ** 2)), 5),
         "fit_regret": round(float(np.mean(np.exp(Y[np.arange(len(Y)), bpred.argmin(1)] - Y.min(1)))), 4)}
     untrained.close()
+    _log(f"learned: batch policy model trained on {len(X)} batches")
 
     def timed_launches(fn):
         first = timed(hip, fn, 1, 1)
@@ -269,6 +273,8 @@ def learned_worker(n_batches, train_steps):
                 per_call.append(timed(hip, lambda: lib.nvshmemx_put_requests_on_stream(arr, n, None), 1, 0))
             if b < 2:
                 bad += int(lib.nvshmemh_verify_requests(arr, n))
+            if (b + 1) % 100 == 0:
+                _log(f"learned: {b + 1} of {nb} held-out {nreq}-put batches judged")
         j = np.array(r["judge"])
         rec = {"mean_ms": {k: round(float(np.mean(v)), 4) for k, v in r.items()},
                "batch_net_over_judge": round(float(np.mean(np.array(r["batch_net"]) / j)), 4),
@@ -343,8 +349,8 @@ def jacobi_worker(layout, name=None, counter_pass=False):
     lib.nvshmem_barrier_all()
     s1 = total(res)
     # unfused steps (pack+put, barrier, unpack, stencil), the faces stored
-    # directly or sent through the library's 2-D and strided device puts; an
-    # even number of steps leaves the result back in u
+    # directly or sent through the library's contiguous and strided device
+    # puts; an even number of steps leaves the result back in u
     sep = {}
     for how, api in (("direct", 0), ("device_put_api", 1)):
         sep[how] = round(timed(hip, lambda: lib.nvshmemh_jacobi_run_separate(u, v, recv, n, 20, chunk, api, None),
@@ -363,10 +369,10 @@ def run_jacobi_leg():
     env = {k: v for k, v in os.environ.items() if not k.startswith("NVSHMEM_")}
     out = {}
     try:
-        p = subprocess.run([sys.executable, __file__, "--jacobi-worker", "local"], capture_output=True, text=True,
+        p = subprocess.run([sys.executable, __file__, "--jacobi-worker", "local"], stdout=subprocess.PIPE, text=True,
                            timeout=600, env=env)
         lines = [l for l in p.stdout.splitlines() if l.startswith("{")]
-        out["local"] = json.loads(lines[-1]) if lines else {"error": p.stderr[-800:]}
+        out["local"] = json.loads(lines[-1]) if lines else {"error": f"local jacobi worker exited {p.returncode}"}
         name = f"nvshmem_jacobi_{os.getpid()}_{uuid.uuid4().hex[:8]}"
         procs = [subprocess.Popen([sys.executable, __file__, "--jacobi-worker", "process"],
                                   env=na.shm_job_env(r, 8, name, {"NVSHMEM_HEAP_SIZE": "320M"}), stdout=subprocess.PIPE,
@@ -421,6 +427,7 @@ def sweep_worker():
             get = getattr(lib, f"nvshmemx_{dtype}_get_on_stream")
             for op in ("put", "get"):
                 out["points"][f"{op}_{dtype}"] = []
+            _log(f"sweep: {dtype}")
             for lg in range(3, 31):
                 size = 1 << lg
                 n = size // w
@@ -457,10 +464,10 @@ def run_sweep_leg():
     try:
         procs = [subprocess.Popen([sys.executable, __file__, "--sweep-worker"],
                                   env=na.shm_job_env(r, 2, name, {"NVSHMEM_HEAP_SIZE": HEAP}), stdout=subprocess.PIPE,
-                                  stderr=subprocess.PIPE, text=True) for r in range(2)]
+                                  text=True) for r in range(2)]
         outs = [pr.communicate(timeout=600) for pr in procs]
         lines = [l for o, _ in outs for l in o.splitlines() if l.startswith("{")]
-        return json.loads(lines[-1]) if lines else {"error": outs[0][1][-800:]}
+        return json.loads(lines[-1]) if lines else {"error": f"sweep workers exited {[pr.returncode for pr in procs]}"}
     except Exception as exc:
         return {"error": str(exc)[-500:]}
     finally:
@@ -472,11 +479,13 @@ def run_sweep_leg():
 def run_learned_leg(n_batches, train_steps):
     env = {k: v for k, v in os.environ.items() if not k.startswith("NVSHMEM_")}
     try:
+        # the worker's progress lines (stderr) pass through: the leg runs for minutes
         p = subprocess.run([sys.executable, __file__, "--learned-worker", "--learned-batches", str(n_batches),
-                            "--train-steps", str(train_steps)], capture_output=True, text=True, timeout=900, env=env)
+                            "--train-steps", str(train_steps)], stdout=subprocess.PIPE, text=True, timeout=900,
+                           env=env)
         lines = [l for l in p.stdout.splitlines() if l.startswith("{")]
         if p.returncode != 0 or not lines:
-            return {"error": f"learned worker exited {p.returncode}: {p.stderr[-800:]}"}
+            return {"error": f"learned worker exited {p.returncode} (its messages are on stderr)"}
         return json.loads(lines[-1])
     except Exception as exc:
         return {"error": str(exc)[-500:]}
@@ -485,11 +494,11 @@ def run_learned_leg(n_batches, train_steps):
 def run_batch_leg():
     try:
         env = {k: v for k, v in os.environ.items() if not k.startswith("NVSHMEM_")}
-        p = subprocess.run([sys.executable, __file__, "--batch-worker"], capture_output=True, text=True, timeout=600,
-                           env=env)
+        p = subprocess.run([sys.executable, __file__, "--batch-worker"], stdout=subprocess.PIPE, text=True,
+                           timeout=600, env=env)
         lines = [l for l in p.stdout.splitlines() if l.startswith("{")]
         if p.returncode != 0 or not lines:
-            return {"error": f"batch worker exited {p.returncode}: {p.stderr[-800:]}"}
+            return {"error": f"batch worker exited {p.returncode} (its messages are on stderr)"}
         rec = json.loads(lines[-1])
     except Exception as exc:
         return {"error": str(exc)[-500:]}
@@ -534,6 +543,7 @@ def cpu_baseline():
     cmd = [str(mpirun), "-n", "2", "-bind-to", "core", str(exe), "putget", "put", "float", str(NBYTES), str(NBYTES),
            "1000", "10"]
     def run(args, timeout=240):
+        _log(f"cpu baseline: 2 ranks {' '.join(args[:3])}")
         out = subprocess.run([str(mpirun), "-n", "2", "-bind-to", "core", str(exe)] + args, capture_output=True,
                              text=True, timeout=timeout, cwd=ROOT)
         return [json.loads(l) for l in out.stdout.splitlines() if l.startswith("{")]
@@ -558,6 +568,7 @@ def cpu_baseline():
         res["sweep_error"] = str(exc)[-300:]
 
     def run8(args, timeout=300):
+        _log(f"cpu baseline: 8 ranks {' '.join(args[:3])}")
         out = subprocess.run([str(mpirun), "-n", "8", "-bind-to", "core", str(exe)] + args, capture_output=True,
                              text=True, timeout=timeout, cwd=ROOT)
         return [json.loads(l) for l in out.stdout.splitlines() if l.startswith("{")]
@@ -776,11 +787,19 @@ def main():
         if shm.exists():
             shm.unlink()
 
-    batched = None if args.no_batch else run_batch_leg()
-    learned = None if args.no_learned else run_learned_leg(args.learned_batches, args.train_steps)
-    jacobi = None if args.no_jacobi else run_jacobi_leg()
-    sweep = None if args.no_sweep else run_sweep_leg()
-    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    def leg(name, skip, fn, *a):
+        if skip:
+            return None
+        _log(f"{name} leg")
+        t = time.time()
+        rec = fn(*a)
+        _log(f"{name} leg done in {time.time() - t:.0f} s")
+        return rec
+    batched = leg("batch", args.no_batch, run_batch_leg)
+    learned = leg("learned", args.no_learned, run_learned_leg, args.learned_batches, args.train_steps)
+    jacobi = leg("jacobi", args.no_jacobi, run_jacobi_leg)
+    sweep = leg("sweep", args.no_sweep, run_sweep_leg)
+    cpu = leg("cpu baseline", args.no_cpu_baseline, cpu_baseline)
     r = result
     line = {
         "metric": "put/get GB/s & HBM-roofline frac; p50 p+wait ns; batched vs per-call puts",

@@ -74,8 +74,8 @@ def test_jacobi_separate_kernels_bitexact(local_pes, hip, oracle, n, pack):
     reading the halo from u) gives the same bits as the fused step; n = 48 has
     partial stencil tiles in k, n = 13 and 130 in j and k.  The faces go out
     either by the direct-store pack kernel or through the library's own
-    device puts (pitched 2-D puts for the x and y faces, strided puts for the
-    z faces)"""
+    device puts (contiguous wavefront puts for the rows of the x and y faces,
+    strided wavefront puts for the z faces)"""
     lib = local_pes.start(8, heap="16M" if n <= 48 else "96M")
     fb = lib.nvshmemh_jacobi_field_bytes(n)
     u, v = lib.nvshmalloc(fb), lib.nvshmalloc(fb)
@@ -98,7 +98,8 @@ def test_jacobi_separate_kernels_bitexact(local_pes, hip, oracle, n, pack):
 def test_jacobi_processes_bitexact(tmp_path, oracle, n, steps, how):
     """8 processes, one PE each: the fused step (faces stored straight into
     the neighbours' receive slots) and the unfused step whose faces go through
-    the library's 2-D and strided device puts, the offload barrier between"""
+    the library's contiguous and strided device puts, the offload barrier
+    between"""
     name = f"nvshmem_jacobi_{os.getpid()}_{uuid.uuid4().hex[:8]}"
     procs = [subprocess.Popen([sys.executable, str(HERE / "pe_worker.py"), "jacobi", str(n), str(steps),
                                str(tmp_path), how],
