@@ -463,7 +463,7 @@ double *nvshmemh_jacobi_run(double *u_sym, double *v_sym, double *recv_sym, int 
 
 /* `steps` unfused steps: the faces of the current field put into the
  * neighbours' receive slots of the step's parity (use_api 0: the direct-store
- * pack kernel; 1: the library's 2-D and strided device puts), the barrier
+ * pack kernel; 1: the library's contiguous and strided device puts), the barrier
  * (nvshmem_barrier_all_offload between processes, the kernel boundary in
  * local mode), the halo unpacked into the field, the stencil.  Returns the
  * buffer holding the result. */
