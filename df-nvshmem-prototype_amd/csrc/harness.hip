@@ -384,6 +384,16 @@ __global__ void k_traced_ops(int *ints_sym, float *floats_sym, double *doubles_s
     if (threadIdx.x == 0) nvshmem_int_put(ints_sym, mine, 4, pe);
 }
 
+/* traced element puts from many lanes at once: threads below `lanes` each
+ * put one int (rep r, thread t: value r * lanes + t into element t) to PE
+ * `pe`, `reps` times, between trace_begin and trace_end */
+__global__ void k_traced_lanes(int *ints_sym, int pe, int lanes, int reps) {
+    nvshmem_trace_begin();
+    if ((int)threadIdx.x < lanes)
+        for (int r = 0; r < reps; ++r) nvshmem_int_p(ints_sym + threadIdx.x, r * lanes + (int)threadIdx.x, pe);
+    nvshmem_trace_end();
+}
+
 /* bench self-check: workgroup i compares request i's target bytes with its
  * source bytes (the copy is bitwise), counting mismatching bytes */
 struct verify_item {
@@ -651,6 +661,13 @@ void nvshmemh_traced_ops(int *ints_sym, float *floats_sym, double *doubles_sym, 
     if (grid < 1) NVSHMEMI_FATAL("nvshmemh_traced_ops: grid %d", grid);
     hipLaunchKernelGGL(k_traced_ops, dim3((unsigned)grid), dim3(block), 0, s, ints_sym, floats_sym, doubles_sym,
                        flag_sym, me, pe, reps);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+void nvshmemh_traced_lanes(int *ints_sym, int pe, int lanes, int reps, int block, int grid, hipStream_t s) {
+    if (grid < 1 || block < 1 || block > 1024 || lanes < 0 || lanes > block)
+        NVSHMEMI_FATAL("nvshmemh_traced_lanes: grid %d block %d lanes %d", grid, block, lanes);
+    hipLaunchKernelGGL(k_traced_lanes, dim3((unsigned)grid), dim3((unsigned)block), 0, s, ints_sym, pe, lanes, reps);
     NVSHMEMI_HIP(hipGetLastError());
 }
 

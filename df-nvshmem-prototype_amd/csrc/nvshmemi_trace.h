@@ -7,14 +7,21 @@
  * to nothing and the device API pays no branch.  When built, the runtime
  * switch NVSHMEM_ENABLE_TRACING selects whether records are kept.
  *
- * Design: one ring of fixed 64-byte records per PE in device memory.  A record
- * is claimed with a compare-and-swap on the ring head that refuses to pass the
- * host's consumed count (so a full ring drops records and raises the overflow
- * flag instead of overwriting unread ones), stamped with s_memrealtime
- * (wall_clock64, constant rate) and published by writing its sequence number
- * last.  As in the reference's block-level mode, only operations issued
- * between nvshmem_trace_begin and nvshmem_trace_end are recorded; the active
- * table slot of a block is (grid id mod G) * B + (block id mod B).
+ * Design: one ring of fixed 64-byte records per PE in pinned, host-coherent
+ * memory, so a host thread can read it while kernels run (the role of the
+ * reference's log_backup thread, nvshmem.cu:206-330).  The claim counter
+ * (head) lives in device memory.  A wavefront claims one slot per recording
+ * lane with a single compare-and-swap, which refuses to pass the host's
+ * consumed count (tail, in host memory; the device keeps its last reading in
+ * device memory and only reads the host again when the ring looks full).
+ * When the ring is full the wavefront waits for the host to drain it, up to
+ * full_wait_ticks, and then drops its records and raises the overflow flag
+ * instead of overwriting unread ones.  A record is stamped with s_memrealtime
+ * (wall_clock64, constant rate) and published by a system-scope release store
+ * of its sequence number, written last.  As in the reference's block-level
+ * mode, only operations issued between nvshmem_trace_begin and
+ * nvshmem_trace_end are recorded; the active table slot of a block is
+ * (grid id mod G) * B + (block id mod B).
  */
 #ifndef NVSHMEMI_TRACE_H
 #define NVSHMEMI_TRACE_H
@@ -47,11 +54,13 @@ typedef struct nvshmemi_trace_record {
 } nvshmemi_trace_record_t;
 
 typedef struct nvshmemi_trace_dev {
-    nvshmemi_trace_record_t *ring; /* capacity records, device memory */
+    nvshmemi_trace_record_t *ring; /* capacity records, pinned host memory (device address) */
     unsigned int *head;            /* claim counter (device memory) */
-    unsigned int *tail;            /* records consumed by the host (device memory) */
+    unsigned int *tail_seen;       /* the device's last reading of *host_tail (device memory) */
+    unsigned int *host_tail;       /* records consumed by the host (pinned host memory) */
     unsigned long long *active;    /* [grids * blocks] grid id + 1 of the owning block */
     unsigned int *flags;           /* [0] overflow, [1] slot conflict */
+    unsigned long long full_wait_ticks; /* how long a claim waits for the host when the ring is full */
     unsigned int capacity;         /* power of two */
     unsigned int grid_mask;
     unsigned int block_mask;
@@ -71,6 +80,10 @@ __device__ __forceinline__ unsigned int trace_slot(const nvshmemi_trace_dev_t *t
     return (unsigned int)((((gid >> 6) & t->grid_mask) << t->block_shift) | (flat_block() & t->block_mask));
 }
 
+/* Claim n consecutive slots (one lane of the wavefront calls it; defined in
+ * trace.hip); 0 when they were dropped. */
+__device__ int trace_claim(nvshmemi_trace_dev_t *t, unsigned int n, unsigned int *base);
+
 __device__ __forceinline__ void trace_record(int type, const void *local, const void *remote, int size,
                                              int pe, int64_t tspent) {
     if (!nvshmemi_dstate.trace_enabled) return;
@@ -78,19 +91,16 @@ __device__ __forceinline__ void trace_record(int type, const void *local, const 
     const uint64_t gid = grid_id();
     const unsigned int slot = trace_slot(t, gid);
     if (__hip_atomic_load(&t->active[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gid + 1) return;
-    unsigned int h = __hip_atomic_load(t->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (;;) {
-        const unsigned int tl = __hip_atomic_load(t->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (h - tl >= t->capacity) {
-            __hip_atomic_store(&t->flags[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return;
-        }
-        unsigned int expect = h;
-        if (__hip_atomic_compare_exchange_strong(t->head, &expect, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT))
-            break;
-        h = expect;
-    }
+    /* the lanes recording now take consecutive slots, in lane order */
+    const unsigned long long live = __ballot(1);
+    const int leader = __ffsll((long long)live) - 1;
+    const unsigned int rank = (unsigned int)__popcll(live & __lanemask_lt());
+    unsigned int base = 0;
+    int ok = 0;
+    if ((int)__lane_id() == leader) ok = trace_claim(t, (unsigned int)__popcll(live), &base);
+    ok = __builtin_amdgcn_readlane(ok, leader);
+    if (!ok) return;
+    const unsigned int h = (unsigned int)__builtin_amdgcn_readlane((int)base, leader) + rank;
     nvshmemi_trace_record_t *r = &t->ring[h & (t->capacity - 1)];
     r->tstamp = (int64_t)wall_clock64();
     r->gridid = gid;
@@ -102,7 +112,7 @@ __device__ __forceinline__ void trace_record(int type, const void *local, const 
     r->type = type;
     r->size = size;
     r->remote_pe = pe;
-    __hip_atomic_store(&r->seq, h + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&r->seq, h + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 } // namespace nvshmemi
 

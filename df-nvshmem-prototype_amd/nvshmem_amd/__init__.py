@@ -109,6 +109,7 @@ SIGNATURES = {
     "nvshmemh_publish_round": (None, [_P, _I, _P]),
     "nvshmemh_wait_once": (None, [_P, _I, _I, _P, _P]),
     "nvshmemh_traced_ops": (None, [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P]),
+    "nvshmemh_traced_lanes": (None, [_P, _I, _I, _I, _I, _I, _P]),
     "nvshmemh_check_round": (None, [_P, _I, _P, _P]),
 }
 
