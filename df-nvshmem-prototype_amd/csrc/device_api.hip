@@ -35,7 +35,8 @@ using namespace nvshmemi;
     } while (0)
 #endif
 
-/* A PE outside [0, npes), or a symmetric range that leaves the heap, makes
+/* A PE outside [0, npes), or a symmetric range that leaves the heap (or an
+ * address outside every PE's heap, such as a private allocation), makes
  * the call do nothing and records NVSHMEMI_ERR_BAD_PE or
  * NVSHMEMI_ERR_BAD_ADDRESS (read by nvshmemx_get_last_error) instead of
  * touching an address outside every PE's mapping (the reference does not
@@ -49,8 +50,9 @@ __device__ __forceinline__ bool bad_pe(int pe) {
 __device__ __forceinline__ bool bad_target(const void *sym, size_t nbytes, int pe) {
     if (bad_pe(pe)) return true;
     const nvshmemi_device_state_t &s = nvshmemi_dstate;
-    const uint64_t off = ((uint64_t)((const char *)sym - s.anchor)) & s.offset_mask;
-    if (off < s.heap_bytes && nbytes <= s.heap_bytes - off) return false;
+    const uint64_t rel = (uint64_t)((const char *)sym - s.anchor);
+    const uint64_t off = rel & s.offset_mask;
+    if (rel < s.anchor_span && off < s.heap_bytes && nbytes <= s.heap_bytes - off) return false;
     set_error(NVSHMEMI_ERR_BAD_ADDRESS);
     return true;
 }

@@ -107,6 +107,10 @@ void upload_device_state() {
     d.trace = g.trace_dev;
     d.trace_enabled = g.trace_dev != nullptr;
     d.heap_bytes = g.heap_bytes;
+    /* local mode: every PE's slot of the one allocation (the offset mask alone
+     * would let any address alias into some PE's heap); process mode: this
+     * PE's heap */
+    d.anchor_span = g.mode == NVSHMEMI_MODE_LOCAL ? g.heap_stride * (uint64_t)g.npes : g.heap_bytes;
     NVSHMEMI_HIP(hipMemcpyToSymbol(HIP_SYMBOL(nvshmemi_dstate), &d, sizeof(d)));
 }
 

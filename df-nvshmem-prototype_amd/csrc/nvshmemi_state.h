@@ -64,6 +64,7 @@ typedef struct nvshmemi_device_state {
     int pad0;
     struct nvshmemi_trace_dev *trace; /* NULL when tracing is off */
     uint64_t heap_bytes;              /* bytes of each PE's heap (internal region included) */
+    uint64_t anchor_span;             /* a symmetric address lies in [anchor, anchor + anchor_span) */
 } nvshmemi_device_state_t;
 
 #endif /* NVSHMEMI_STATE_H */

@@ -179,8 +179,9 @@ __device__ __forceinline__ unsigned req_units(const nvshmemx_put_request_t &r, b
         if (report) set_error(NVSHMEMI_ERR_BAD_PE);
         return 0;
     }
-    const uint64_t off = ((uint64_t)((const char *)r.target - s.anchor)) & s.offset_mask;
-    if (off >= s.heap_bytes || r.nbytes > s.heap_bytes - off) {
+    const uint64_t rel = (uint64_t)((const char *)r.target - s.anchor);
+    const uint64_t off = rel & s.offset_mask;
+    if (rel >= s.anchor_span || off >= s.heap_bytes || r.nbytes > s.heap_bytes - off) {
         if (report) set_error(NVSHMEMI_ERR_BAD_ADDRESS);
         return 0;
     }

@@ -152,15 +152,24 @@ def test_persistent_empty_and_invalid(setup, hip, oracle):
     _check(lib, hip, base, images)
     reqs = na.gen_requests(lib, 47, 64, NPES, SRC, DST)
     arr = na.make_requests(lib, reqs, base, 0, SRC)
-    want = _expected(oracle, images, {k: v[2:] for k, v in reqs.items()})
+    want = _expected(oracle, images, {k: v[3:] for k, v in reqs.items()})
     arr[0].pe = NPES + 3                      # no such PE
     arr[1].target = arr[1].target + (200 << 20)  # past the end of the heap
-    _reset(lib, hip, base)
-    _run(lib, hip, arr, len(arr))
-    err = lib.nvshmemx_get_last_error()
-    lib.nvshmemx_clear_errors()
-    assert err & na.ERR_BAD_PE and err & na.ERR_BAD_ADDRESS, hex(err)
-    _check(lib, hip, base, want)
+    # a private address at the same offset within a PE slot as a real target
+    stride = lib.nvshmem_ptr(base, 1) - lib.nvshmem_ptr(base, 0)
+    priv = hip.malloc(stride + (1 << 20))
+    try:
+        arr[2].target = priv + ((arr[2].target - priv) % stride)
+        hip.memset(priv, 0x3C, stride + (1 << 20))
+        _reset(lib, hip, base)
+        _run(lib, hip, arr, len(arr))
+        err = lib.nvshmemx_get_last_error()
+        lib.nvshmemx_clear_errors()
+        assert err & na.ERR_BAD_PE and err & na.ERR_BAD_ADDRESS, hex(err)
+        _check(lib, hip, base, want)
+        assert (hip.dtoh(priv, stride + (1 << 20)) == 0x3C).all()
+    finally:
+        hip.free(priv)
 
 
 def test_persistent_between_processes():

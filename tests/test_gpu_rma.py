@@ -554,6 +554,52 @@ def test_out_of_range_pe_is_refused_on_device(local_pes, hip, pe):
         hip.free(loc)
 
 
+def test_private_address_is_refused_on_device(local_pes, hip):
+    """in local mode every PE's heap is one slot of a single allocation and a
+    symmetric address is translated by its offset within its slot; a private
+    allocation's address whose offset within a slot lands on a live symmetric
+    object is still not symmetric: device puts, p, g and wavefront puts
+    naming it move nothing and record NVSHMEMX_ERR_BAD_ADDRESS"""
+    lib = local_pes.start(2, heap="64M")
+    sym = lib.nvshmalloc(4096)
+    src = lib.nvshmalloc(4096)
+    stride = lib.nvshmem_ptr(sym, 1) - lib.nvshmem_ptr(sym, 0)
+    assert stride > 0 and stride & (stride - 1) == 0
+    priv = hip.malloc(stride + 8192)
+    out = hip.malloc(64)
+    try:
+        # the address in the private buffer that lies at sym's offset within a slot
+        alias = priv + ((sym - priv) % stride)
+        assert priv <= alias < priv + stride and (alias - sym) % stride == 0
+        assert not (lib.nvshmem_ptr(sym, 0) <= alias < lib.nvshmem_ptr(sym, 0) + 2 * stride)
+        for q in (0, 1):
+            hip.memset(lib.nvshmem_ptr(sym, q), 0x5A, 4096)
+        hip.memset(alias, 0x3C, 4096)
+        hip.memset(out, 0x77, 64)
+        hip.sync()
+        local_src = lib.nvshmem_ptr(src, 0)
+        calls = [
+            lambda: lib.nvshmemh_thread_putget(0, 1, alias, local_src, 16, 16, 1, None),
+            lambda: lib.nvshmemh_pg(0, 0, alias, out, out, 16, 1, None),
+            lambda: lib.nvshmemh_pg(1, 0, alias, out, out, 16, 1, None),
+            lambda: lib.nvshmemh_coop_putget(0, 0, alias, local_src, 4096, 4096, 1, 64, None),
+            lambda: lib.nvshmemh_coop_putget(1, 1, out, alias, 64, 64, 0, 256, None),
+        ]
+        for call in calls:
+            lib.nvshmemx_clear_errors()
+            call()
+            hip.sync()
+            assert lib.nvshmemx_get_last_error() == na.ERR_BAD_ADDRESS
+        for q in (0, 1):
+            assert (hip.dtoh(lib.nvshmem_ptr(sym, q), 4096) == 0x5A).all()
+        assert (hip.dtoh(alias, 4096) == 0x3C).all()
+        assert (hip.dtoh(out, 64, np.int32) == 0).all()      # g of a bad address returns 0
+    finally:
+        lib.nvshmemx_clear_errors()
+        hip.free(priv)
+        hip.free(out)
+
+
 def test_out_of_heap_range_is_refused_on_device(local_pes, hip):
     """device calls whose symmetric range leaves the heap move nothing and
     record NVSHMEMX_ERR_BAD_ADDRESS: per-thread put, p, g (returns 0), a
