@@ -333,10 +333,8 @@ def jacobi_worker(layout, name=None, counter_pass=False):
                    for pe in pes_here)
     s0 = total(u)
     lib.nvshmem_barrier_all()
-    # planes per workgroup: one process's launch covers one PE, so it needs
-    # shorter chunks (more workgroups) to fill the device than the launch
-    # that covers all eight PEs
-    chunk = int(os.environ.get("JACOBI_CHUNK", "128" if layout == "local" else "16"))
+    # planes per workgroup (0: the library's default, 6)
+    chunk = int(os.environ.get("JACOBI_CHUNK", "0"))
     if counter_pass:  # five steps and one 1 GiB device copy (the counters' scale)
         lib.nvshmemh_jacobi_run(u, v, recv, n, 5, chunk, 0, None)
         ref = hip.malloc(2 << 30)
@@ -357,7 +355,7 @@ def jacobi_worker(layout, name=None, counter_pass=False):
                                1, 0) / 20, 4)
     lib.nvshmem_barrier_all()
     err = lib.nvshmemx_get_last_error()
-    out = {"layout": layout, "pes": 8, "cells_per_pe": n ** 3, "steps": JACOBI_STEPS, "chunk": chunk, "ms_per_step": ms,
+    out = {"layout": layout, "pes": 8, "cells_per_pe": n ** 3, "steps": JACOBI_STEPS, "chunk": chunk or "library default", "ms_per_step": ms,
            "sum_before": s0, "sum_after": s1, "unfused_ms_per_step": sep, "errors": err}
     lib.nvstop_pes()
     if pe0 == 0:
