@@ -512,6 +512,40 @@ def run_batch_leg():
     return rec
 
 
+ANATOMY_BYTES, ANATOMY_LAUNCHES = 64 << 20, 32
+
+
+def launch_anatomy(lib, hip, dst, src):
+    """Where the time of back-to-back 64 MiB copies goes: the library's
+    interleaved copy with every workgroup's start and end stamped from the
+    constant clock, 32 launches rotated over 1 GiB.  Per launch: the span from
+    the first workgroup's start to the last one's end, the last tenth of the
+    workgroup ends (the drain), and the gap from the last end to the next
+    launch's first start (kernel end, fences, next dispatch)."""
+    import numpy as np
+    words = ANATOMY_LAUNCHES * 8192 * 2
+    stamps = hip.malloc(words * 8)
+    try:
+        grid = [0]
+        ms = timed(hip, lambda: grid.__setitem__(0, lib.nvshmemh_copy_stamped(
+            dst, src, ANATOMY_BYTES, NBYTES, ANATOMY_LAUNCHES, stamps, words, None)), 1, 1) / ANATOMY_LAUNCHES
+        t = hip.dtoh(stamps, words * 8, np.uint64).reshape(ANATOMY_LAUNCHES, grid[0], 2).astype(np.float64)
+    finally:
+        hip.free(stamps)
+    ns = 1e6 / lib.nvshmemx_wallclock_khz()
+    first, last = t[:, :, 0].min(axis=1), t[:, :, 1].max(axis=1)
+    ends = np.sort(t[:, :, 1], axis=1)
+    span = (last - first) * ns
+    drain = (last - ends[:, int(grid[0] * 0.9)]) * ns
+    gap = (first[1:] - last[:-1]) * ns
+    per_us = ms * 1e3
+    return {"us_per_launch": round(per_us, 2), "gbps": round(2 * ANATOMY_BYTES / (ms * 1e-3) / 1e9, 1),
+            "workgroup_span_us": round(float(np.median(span)) / 1e3, 2),
+            "drain_last_tenth_us": round(float(np.median(drain)) / 1e3, 2),
+            "gap_between_launches_us": round(float(np.median(gap)) / 1e3, 2),
+            "launches": ANATOMY_LAUNCHES, "workgroups": grid[0]}
+
+
 def timed(hip, fn, steps, warmup):
     e0, e1 = hip.event(), hip.event()
     for _ in range(warmup):
@@ -722,6 +756,7 @@ def main():
         # faster one is the part's write ceiling
         write_ms = [timed(hip, lambda pol=pol: lib.nvshmemh_stream_write(back, NBYTES, pol, None), 20, 3)
                     for pol in (0, 1)]
+        anatomy = launch_anatomy(lib, hip, back, src)
         lib.nvshmem_barrier_all()                                        # 2
         _log("puts/gets done; ping-pong")
         lib.nvshmem_barrier_all()                                        # 3
@@ -777,7 +812,7 @@ def main():
                              "sc1": round(NBYTES / (write_ms[1] * 1e-3) / 1e9, 1)},
             rtt_p50=float(np.percentile(rtt_ns, 50)), rtt_p99=float(np.percentile(rtt_ns, 99)),
             bar_p50=float(np.percentile(bar_ns, 50)), bar_p99=float(np.percentile(bar_ns, 99)), putflag=putflag,
-            streams_by_size=streams_by_size)
+            streams_by_size=streams_by_size, anatomy=anatomy)
     finally:
         if child is not None and child.poll() is None:
             child.kill()
@@ -832,7 +867,8 @@ def main():
                        "so its bound is the two streams back to back"},
         "bulk_streams_gbps": {"sizes": r["streams_by_size"], "frac_of_peak_min": round(
             min(min(v["put_gbps"], v["get_gbps"]) for v in r["streams_by_size"].values()) / PEAK_HBM_GBPS, 4),
-            "pattern": "back-to-back float put (get) launches, PE 0 -> PE 1, buffers rotated over 1 GiB"},
+            "pattern": "back-to-back float put (get) launches, PE 0 -> PE 1, buffers rotated over 1 GiB",
+            "launch_anatomy_64MiB": r["anatomy"]},
         "p_wait_rtt_ns": {"p50": round(r["rtt_p50"], 1), "p99": round(r["rtt_p99"], 1),
                           "iters": PINGPONG_ITERS, "between": "2 processes" if args.layout == "process" else
                           "2 workgroup PEs"},

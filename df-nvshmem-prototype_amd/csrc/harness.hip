@@ -446,6 +446,24 @@ __global__ void __launch_bounds__(256) k_stream_write_sc1(nvshmemi::u32x4 *a, si
     }
 }
 
+/* The library's interleaved copy as it runs for 32-128 MiB puts (every lane
+ * of a 8192 x 512 grid, 16-byte accesses, non-temporal loads, sc1 stores),
+ * with each workgroup's start and end stamped from the constant clock, to see
+ * where the time of back-to-back launches goes. */
+constexpr unsigned kStampGrid = 8192, kStampBlock = 512;
+__global__ void __launch_bounds__(512) k_copy_stamped(char *dst, const char *src, size_t nbytes,
+                                                      unsigned long long *stamps) {
+    const unsigned long long t0 = wall_clock64();
+    nvshmemi::copy<16, 2, nvshmemi::NT_LOAD | nvshmemi::ST_SC1>(dst, src, nbytes,
+                                                                (size_t)blockIdx.x * blockDim.x + threadIdx.x,
+                                                                (size_t)gridDim.x * blockDim.x);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        stamps[2 * (size_t)blockIdx.x] = t0;
+        stamps[2 * (size_t)blockIdx.x + 1] = wall_clock64();
+    }
+}
+
 } // namespace
 
 /* --------------------------------------------------------- C entry points -- */
@@ -493,6 +511,27 @@ void nvshmemh_stream_write(void *p, size_t nbytes, int policy, hipStream_t s) {
     if (policy == 0) hipLaunchKernelGGL(k_stream_write, grid, dim3(256), 0, s, (nvshmemi::u32x4 *)p, n);
     else hipLaunchKernelGGL(k_stream_write_sc1, grid, dim3(256), 0, s, (nvshmemi::u32x4 *)p, n);
     NVSHMEMI_HIP(hipGetLastError());
+}
+
+/* `launches` back-to-back stamped copies of `nbytes` on stream s; launch k
+ * copies src + o to dst + o, o = (k * nbytes) mod span (both buffers hold
+ * `span` bytes, a multiple of nbytes).  Workgroup b of launch k writes its
+ * start and end (constant-clock ticks) to stamps[(k * grid + b) * 2 + {0, 1}];
+ * `stamp_words` is the capacity.  Returns the grid (workgroups per launch). */
+int nvshmemh_copy_stamped(void *dst, const void *src, size_t nbytes, size_t span, int launches,
+                          unsigned long long *stamps, size_t stamp_words, hipStream_t s) {
+    if (!nbytes || span < nbytes || span % nbytes || launches < 1)
+        NVSHMEMI_FATAL("nvshmemh_copy_stamped: %zu bytes over a %zu-byte span, %d launches", nbytes, span, launches);
+    if ((size_t)launches * kStampGrid * 2 > stamp_words)
+        NVSHMEMI_FATAL("nvshmemh_copy_stamped: %d launches need %zu stamp words, %zu given", launches,
+                       (size_t)launches * kStampGrid * 2, stamp_words);
+    for (int k = 0; k < launches; ++k) {
+        const size_t o = ((size_t)k * nbytes) % span;
+        hipLaunchKernelGGL(k_copy_stamped, dim3(kStampGrid), dim3(kStampBlock), 0, s, (char *)dst + o,
+                           (const char *)src + o, nbytes, stamps + (size_t)k * kStampGrid * 2);
+    }
+    NVSHMEMI_HIP(hipGetLastError());
+    return (int)kStampGrid;
 }
 
 void nvshmemh_fill_pcg32(void *dst, size_t nwords, unsigned long long seed, unsigned long long seq, hipStream_t s) {

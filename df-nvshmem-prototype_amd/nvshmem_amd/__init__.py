@@ -97,6 +97,7 @@ SIGNATURES = {
     "nvshmemh_fill_pcg32": (None, [_P, _SZ, _U64, _U64, _P]),
     "nvshmemh_stream_read": (None, [_P, _SZ, _P, _P]),
     "nvshmemh_stream_write": (None, [_P, _SZ, _I, _P]),
+    "nvshmemh_copy_stamped": (_I, [_P, _P, _SZ, _SZ, _I, _P, _SZ, _P]),
     "nvshmemh_pcg32_host": (None, [_P, _SZ, _U64, _U64]),
     "nvshmemh_thread_putget": (None, [_I, _I, _P, _P, _I, _I, _I, _P]),
     "nvshmemh_coop_putget": (None, [_I, _I, _P, _P, _SZ, _SZ, _I, _I, _P]),

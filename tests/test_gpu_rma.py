@@ -48,6 +48,35 @@ def test_device_pcg_matches_oracle(local_pes, hip, oracle):
         np.testing.assert_array_equal(got, oracle.pcg32(n, 42, 54))
 
 
+def test_nvshmalloc_offsets_match_oracle(local_pes, oracle):
+    """the bump allocator (reference nvshmem.cu:775-819, here with 256-byte
+    alignment) hands out the oracle's offsets for the same call sequence,
+    symmetric on every PE; an exact fit of the remaining heap succeeds, and
+    nvshmcleanup starts again at offset 0"""
+    lib = local_pes.start(4, heap="16M")
+    heap = int(lib.nvshmemx_heap_size())
+    rng = np.random.default_rng(11)
+    sizes = [0, 1, 255, 256, 257, 4096, 12, 0, (1 << 20) + 3] + [int(s) for s in rng.integers(0, 200000, 40)]
+    for rnd in range(2):
+        lib.nvshmcleanup()
+        h = oracle.heap(heap, 256)
+        base = None
+        for s in sizes:
+            want = oracle.nvshmalloc(h, s)
+            assert want >= 0, "sequence sized to fit the heap"
+            p = lib.nvshmalloc(s)
+            base = p if base is None else base
+            assert p - base == want, f"round {rnd}: size {s} at {p - base}, oracle {want}"
+            for pe in range(1, 4):
+                assert lib.nvshmem_ptr(p, pe) - lib.nvshmem_ptr(base, pe) == want
+        # the rest of the heap fits exactly
+        rest = heap - (oracle.nvshmalloc(h, 0))
+        want = oracle.nvshmalloc(h, rest)
+        assert want >= 0 and want + rest == heap
+        assert lib.nvshmalloc(rest) - base == want
+        assert oracle.nvshmalloc(h, 1) == -1  # the library exits here (misuse test heap_exhausted)
+
+
 def _check_region(hip, oracle, lib, dst_dev_region, image, nbytes_region):
     got = hip.dtoh(dst_dev_region, nbytes_region)
     if not np.array_equal(got, image):
