@@ -1,6 +1,6 @@
 /*
- * persistent.hip -- a batch of puts executed by one persistent kernel whose
- * workgroups are the PEs that issued the puts.
+ * persistent.hip -- a batch of puts planned on the device by the PEs that
+ * issued them and executed by one persistent kernel.
  *
  * No counterpart in the reference, whose puts are one call per thread
  * (nvshmem_device.cu:487-512).  The fused plans of batch.hip are cut into
@@ -8,9 +8,9 @@
  * every launch, from the request list as it is:
  *
  *  - the grid holds only as many workgroups as the device runs at once, so
- *    every workgroup is resident for the whole launch.  Workgroup b belongs to
- *    PE b mod P (P = the local PEs, or 1 in process mode), so workgroups
- *    0 .. P-1 -- dispatched first -- are the PEs' leaders;
+ *    every workgroup is resident for the whole launch.  Workgroups 0 .. P-1
+ *    (P = the local PEs, or 1 in process mode) -- dispatched first -- are
+ *    the PEs' leaders;
  *  - each leader reads the whole request list and takes the requests its PE
  *    issued (those whose source lies in its heap; a private source counts for
  *    PE i mod P): it lays them end to end in a virtual byte stream, each
@@ -19,34 +19,39 @@
  *    starts in; then publishes its queue with a release flag.  Leaders agree
  *    on where each PE's entries and chunks go by counting every PE's requests
  *    in the same pass, so no leader waits for another;
- *  - every workgroup claims 128 KiB chunks of its own PE's queue with one
- *    atomic fetch-add each (the next claim is issued before the current chunk
- *    is copied), and once that queue is drained, takes chunks from the queue
- *    with the most left.  Inside a chunk, a request segment of 16 KiB or more
- *    is copied by the whole workgroup, shorter ones by one wavefront each, in
- *    turn, with 16-byte accesses where the alignment allows, eight loads in
- *    flight per lane and non-temporal stores;
- *  - the last workgroup to finish clears the claim counters and flags, so
+ *  - once every queue is published, the workgroups claim 128 KiB chunks of
+ *    the queues laid end to end in PE order, one atomic fetch-add on one
+ *    counter each (the next claim is issued before the current chunk is
+ *    copied), so the workgroups running at once copy neighbouring chunks of
+ *    the same few PEs' queues.  Inside a chunk, a request segment of 16 KiB
+ *    or more is copied by the whole workgroup, shorter ones by one wavefront
+ *    each, in turn, with 16-byte accesses where the alignment allows, eight
+ *    loads in flight per lane and non-temporal stores;
+ *  - the last workgroup to finish clears the claim counter and the flags, so
  *    the next launch (or graph replay) starts from zero with no memset.
  *
- * Contention and coherence: each PE's claim counter, ready flag and queue
- * record sit in a 4 KiB slot of their own, so the claims of different PEs do
- * not queue on one cache line.  Readers never take an agent-scope acquire
- * (which invalidates the XCD's L2 under the streams the other workgroups are
+ * Contention and coherence: each PE's ready flag and queue record sit in a
+ * 4 KiB slot of their own.  Readers never take an agent-scope acquire (which
+ * invalidates the XCD's L2 under the streams the other workgroups are
  * running): the leader publishes with one release, and everything it
  * published is read with coherent (agent-scope relaxed) loads after the flag
  * was seen set.
  *
  * Launch shape, from measurements of the 4096-put step of 64 PEs (1.57 GB
- * payload): two workgroups of 512 threads per CU with eight loads in flight
- * per lane ran the step in 0.64 ms; three or four per CU, or four loads in
- * flight, 0.68-0.72 ms; every wavefront working alone on 32 KiB chunks, 1.07
- * ms (each of ~8000 concurrent streams waited ~12 us per access: the DRAM
- * pages thrash); claim counters sharing one cache line and an acquire per
- * poll, 1.49 ms.  The leaders take about 24 us before the first chunk is
- * claimed.  The host-planned fused launch of batch.hip (source-ordered
- * pieces, no device-side planning) stays the faster path for a static
- * exchange: 0.51 ms.
+ * payload, ms per step): one workgroup of 1024 threads per CU, chunks of 128
+ * KiB, eight loads in flight per lane, 0.528; chunks of 64, 256 or 512 KiB
+ * 0.584, 0.540, 0.560; sixteen loads in flight 0.562; two workgroups of 512
+ * threads per CU 0.591, three of 512 threads with four loads in flight 0.659.
+ * Fewer, wider workgroups keep fewer address streams open at once.  Each
+ * workgroup first draining its own PE's queue and then stealing from the
+ * longest (the PE-owned order, 64 source windows at once) took 0.636 at two
+ * workgroups of 512 threads per CU against 0.591 for the PE-ordered claims;
+ * every wavefront working alone on 32 KiB chunks 1.07 (each of ~8000
+ * concurrent streams waited ~12 us per access: the DRAM pages thrash); claim
+ * counters sharing one cache line and an acquire per poll, 1.49.  The leaders
+ * take about 24 us before the first chunk is claimed.  The host-planned fused
+ * launch of batch.hip (source-ordered pieces, no device-side planning) stays
+ * slightly faster for a static exchange: 0.51.
  *
  * Scratch (per launch, device memory): the control slots, the chunk table
  * (65,536 entries; the chunk grows from 128 KiB so that a batch of any size
@@ -72,7 +77,7 @@ namespace nvshmemi {
 #ifndef NVSHMEMI_PUNROLL
 #define NVSHMEMI_PUNROLL 8
 #endif
-constexpr int kPBlock = 512;
+constexpr int kPBlock = 1024;
 constexpr int kPWaves = kPBlock / 64;
 constexpr unsigned kUnit = 4096;                           /* granule of the virtual stream, bytes */
 constexpr unsigned kChunkUnitsMin = NVSHMEMI_PCHUNK_UNITS; /* chunk unless the table needs larger */
@@ -81,7 +86,7 @@ constexpr int kPUnroll = NVSHMEMI_PUNROLL;                 /* loads in flight pe
 constexpr unsigned kTableChunks = 65536;                   /* chunk-table entries */
 constexpr unsigned kLeadCache = 4096;                      /* requests a leader keeps in LDS between passes */
 constexpr int kMaxReqsNoalloc = 65536;                     /* requests per call without allocation */
-constexpr int kPPerCU = 2;                                 /* workgroups per CU (see the launch shape above) */
+constexpr int kPPerCU = 1;                                                /* workgroups per CU (see the launch shape above) */
 constexpr int kMaxDevices = 64;
 
 struct pinfo {            /* PE p's queue, published by its leader */
@@ -97,11 +102,12 @@ struct pentry {
     unsigned vstart; /* start in the PE's virtual stream, granules */
 };
 
-/* scratch layout: per-PE 4 KiB slots (claim, ready, queue record), the done
- * counter, the chunk table, the entries */
+/* scratch layout: per-PE 4 KiB slots (ready flag, queue record), the done
+ * counter and the claim counter, the chunk table, the entries */
 constexpr size_t kSlot = 4096;
-constexpr size_t kClaimOff = 0, kReadyOff = 128, kInfoOff = 256;
+constexpr size_t kReadyOff = 128, kInfoOff = 256;
 constexpr size_t kDoneOff = kSlot * NVSHMEMI_MAX_PES;
+constexpr size_t kClaimOff = kDoneOff + 128;
 constexpr size_t kCtlBytes = kDoneOff + 256; /* zero between launches */
 constexpr size_t kTableOff = kCtlBytes;
 constexpr size_t kEntryOff = kTableOff + sizeof(unsigned) * kTableChunks;
@@ -114,10 +120,10 @@ static __device__ __align__(256) unsigned char scratch[scratch_bytes(kMaxReqsNoa
 
 struct pscr {
     unsigned char *base;
-    __device__ __forceinline__ unsigned *claim(int q) const { return (unsigned *)(base + q * kSlot + kClaimOff); }
     __device__ __forceinline__ unsigned *ready(int q) const { return (unsigned *)(base + q * kSlot + kReadyOff); }
     __device__ __forceinline__ pinfo *info(int q) const { return (pinfo *)(base + q * kSlot + kInfoOff); }
     __device__ __forceinline__ unsigned *done() const { return (unsigned *)(base + kDoneOff); }
+    __device__ __forceinline__ unsigned *claim() const { return (unsigned *)(base + kClaimOff); }
     __device__ __forceinline__ unsigned *table() const { return (unsigned *)(base + kTableOff); }
     __device__ __forceinline__ pentry *entries() const { return (pentry *)(base + kEntryOff); }
 };
@@ -317,89 +323,81 @@ __device__ __forceinline__ void run_chunk(const nvshmemx_put_request_t *__restri
     }
 }
 
-/* a workgroup drains queue q: thread 0 claims chunks, the next claim issued
- * before the current chunk is copied; claims pass through two LDS slots, one
- * barrier per chunk */
-__device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__ reqs, const pscr &S, const pinfo &inf,
-                                      int q, unsigned *s_k) {
-    if (threadIdx.x == 0) s_k[0] = atomicAdd(S.claim(q), 1u);
+/* every workgroup drains every queue in PE order: chunks are claimed from one
+ * counter over the queues laid end to end (the chunk table already holds
+ * them in PE order), so the workgroups running at once copy neighbouring
+ * chunks of the same few PEs' queues */
+__device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__ reqs, const pscr &S, int P) {
+    __shared__ pinfo s_inf[NVSHMEMI_MAX_PES];
+    __shared__ unsigned s_total, s_k[2];
+    __shared__ int s_qk[2];
+    if (threadIdx.x == 0) s_total = 0;
+    __syncthreads();
+    if ((int)threadIdx.x < P) {
+        const int q = (int)threadIdx.x;
+        pinfo inf;
+        if (wait_ready(S.ready(q))) {
+            inf = coherent_info(S.info(q));
+        } else { /* a queue that never appeared: its chunks are skipped */
+            inf.entry0 = inf.nown = inf.chunk0 = inf.nchunks = inf.chunk_units = 0u;
+        }
+        s_inf[q] = inf;
+        if (inf.nchunks) atomicMax(&s_total, inf.chunk0 + inf.nchunks);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) /* keep first chunks non-decreasing over a missing queue */
+        for (int q = 1; q < P; ++q)
+            if (!s_inf[q].chunk_units) s_inf[q].chunk0 = s_inf[q - 1].chunk0 + s_inf[q - 1].nchunks;
+    __syncthreads();
+    const unsigned total = s_total;
+    /* queue of global chunk k: the last PE whose first chunk is at or before
+     * k (PEs with no chunks never match: the range test below skips them) */
+    auto owner = [&](unsigned k) -> int {
+        int lo = 0, hi = P - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_inf[mid].chunk0 <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    };
+    if (threadIdx.x == 0) {
+        const unsigned k = atomicAdd(S.claim(), 1u);
+        s_k[0] = k;
+        s_qk[0] = k < total ? owner(k) : -1;
+    }
     __syncthreads();
     unsigned it = 0;
-    unsigned k = __builtin_amdgcn_readfirstlane(s_k[0]);
-    while (k < inf.nchunks) {
-        unsigned next = 0;
-        if (threadIdx.x == 0) next = atomicAdd(S.claim(q), 1u);
-        run_chunk(reqs, inf, k, S);
-        if (threadIdx.x == 0) s_k[(it + 1) & 1] = next;
+    for (;;) {
+        const unsigned k = __builtin_amdgcn_readfirstlane(s_k[it & 1]);
+        const int q = __builtin_amdgcn_readfirstlane(s_qk[it & 1]);
+        if (k >= total) break;
+        if (threadIdx.x == 0) {
+            const unsigned nk = atomicAdd(S.claim(), 1u);
+            s_k[(it + 1) & 1] = nk;
+            s_qk[(it + 1) & 1] = nk < total ? owner(nk) : -1;
+        }
+        const pinfo &inf = s_inf[q];
+        if (k - inf.chunk0 < inf.nchunks) run_chunk(reqs, inf, k - inf.chunk0, S);
         __syncthreads();
         ++it;
-        k = __builtin_amdgcn_readfirstlane(s_k[it & 1]);
     }
 }
 
 __global__ void __launch_bounds__(kPBlock) k_batch_persistent(const nvshmemx_put_request_t *__restrict__ reqs,
                                                               unsigned nreq, int P, unsigned char *scr) {
     const pscr S{scr};
-    __shared__ unsigned s_k[2];
-    __shared__ int s_q[2];
     const int p = (int)(blockIdx.x % (unsigned)P);
     if ((int)blockIdx.x < P) lead(reqs, nreq, P, p, S);
+    drain(reqs, S, P);
 
-    /* own queue first (a wait that timed out leaves the flag 0: the record
-     * then still holds an earlier launch's queue, and is not used) */
-    if (threadIdx.x == 0) s_q[0] = wait_ready(S.ready(p)) ? p : -1;
-    __syncthreads();
-    if (s_q[0] >= 0) {
-        const pinfo inf = coherent_info(S.info(p));
-        drain(reqs, S, inf, p, s_k);
-    }
-    /* then the queue with the most chunks left (chosen by wavefront 0); a
-     * queue not yet published (its leader still reading the list) is waited
-     * for only when no other queue has work */
-    for (unsigned round = 0;; ++round) {
-        if (threadIdx.x < 64) {
-            unsigned best = 0;
-            int best_q = -1, pending = -1;
-            for (int q = (int)threadIdx.x; q < P; q += 64) {
-                if (coherent_u32(S.ready(q)) == 0u) {
-                    pending = q;
-                    continue;
-                }
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                const unsigned nc = coherent_u32(&S.info(q)->nchunks), cl = coherent_u32(S.claim(q));
-                const unsigned rem = nc > cl ? nc - cl : 0u;
-                if (rem > best) best = rem, best_q = q;
-            }
-#pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) {
-                const unsigned ob = __shfl_xor(best, d, 64);
-                const int oq = __shfl_xor(best_q, d, 64), op = __shfl_xor(pending, d, 64);
-                if (ob > best || (ob == best && oq > best_q)) best = ob, best_q = oq;
-                pending = op > pending ? op : pending;
-            }
-            if (threadIdx.x == 0) {
-                int v = best_q >= 0 ? best_q : -1;
-                if (best_q < 0 && pending >= 0) v = wait_ready(S.ready(pending)) ? -2 : -1; /* -2: look again */
-                s_q[(round + 1) & 1] = v;
-            }
-        }
-        __syncthreads();
-        const int v = s_q[(round + 1) & 1];
-        if (v == -1) break;
-        if (v >= 0) {
-            const pinfo inf = coherent_info(S.info(v));
-            drain(reqs, S, inf, v, s_k);
-        }
-    }
     /* the last workgroup out clears the control slots for the next launch */
     if (threadIdx.x == 0) {
         fence_release();
         const unsigned old = __hip_atomic_fetch_add(S.done(), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (old == gridDim.x - 1) {
-            for (int q = 0; q < P; ++q) {
-                store_relaxed(S.claim(q), 0u);
-                store_relaxed(S.ready(q), 0u);
-            }
+            for (int q = 0; q < P; ++q) store_relaxed(S.ready(q), 0u);
+            store_relaxed(S.claim(), 0u);
             store_relaxed(S.done(), 0u);
         }
     }

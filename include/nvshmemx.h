@@ -181,16 +181,16 @@ void nvshmemx_batch_default_choice(size_t nbytes, int elem_bytes, nvshmemx_batch
 /* The per-call path: one grid put launch per request. */
 void nvshmemx_put_requests_on_stream(const nvshmemx_put_request_t *reqs, int nreq, hipStream_t stream);
 
-/* ---- batched puts in one persistent kernel of the issuing PEs ---- */
+/* ---- batched puts planned on the device by the issuing PEs ---- */
 
 /* Execute `nreq` puts (the request array in DEVICE memory) with one
- * persistent kernel on `stream`.  The kernel's workgroups are the PEs that
- * issued the puts -- in local mode workgroup b belongs to PE b mod npes, and a
- * request belongs to the PE whose heap holds its source (a private source
- * counts for PE i mod npes); in process mode every workgroup is this PE.
- * Each PE cuts its own requests into chunks on the device and its workgroups
- * copy them; a PE whose chunks are done takes chunks from the PE with the
- * most left.  Destinations must not overlap (puts are unordered, as in the
+ * persistent kernel on `stream`.  A request belongs to the PE that issued it:
+ * in local mode the PE whose heap holds its source (a private source counts
+ * for PE i mod npes), in process mode this PE.  Workgroup p < npes (local
+ * mode; workgroup 0 in process mode) is PE p's leader and cuts that PE's
+ * requests into chunks on the device; then every workgroup copies chunks of
+ * all PEs' queues in PE order, claimed one at a time from one counter.
+ * Destinations must not overlap (puts are unordered, as in the
  * fused plans).  A request naming a bad PE or a target range outside the heap
  * is skipped and sets NVSHMEMX_ERR_BAD_PE / NVSHMEMX_ERR_BAD_ADDRESS.
  * The library allocates the kernel's scratch (a control block, a 256 KiB

@@ -1,6 +1,6 @@
 """GPU parity of the persistent batch kernel (nvshmemx_batch_put_persistent_on_stream)
 against the oracle: the PEs that issued the puts cut them into chunks on the
-device and copy them, stealing from each other's queues.
+device, and every workgroup copies chunks of all PEs' queues in PE order.
 
 8 workgroup-partitioned PEs in one process; each PE's heap holds a 32 MiB
 source region (PCG32 words, seed 100 + pe) and a 64 MiB destination region.
@@ -9,7 +9,7 @@ application of the same requests.  Covered: a batch of the uneven-put
 workload, byte-granular (every alignment class) requests mixed with
 zero-byte ones, sources outside the heap, thousands of tiny requests (many
 per chunk, the wavefront path), one request larger than many chunks (the
-workgroup path, every PE's workgroups stealing from one queue), repeated
+workgroup path, every workgroup copying chunks of one queue), repeated
 calls (the kernel clears its own counters), an empty list, requests naming a
 bad PE or leaving the heap (skipped, error bit set), and two processes each
 issuing puts into the other's IPC-mapped heap."""
