@@ -96,8 +96,9 @@ def batch_worker(counter_pass=False):
     workgroups run the puts it initiated), the persistent kernel whose
     workgroup PEs plan their own puts on the device (library-allocated and
     static scratch) and the per-call path (one launch per put).  Prints one
-    JSON line.  counter_pass: only five fused launches and one 1 GiB
-    device-to-device copy (the scale of a counter pass)."""
+    JSON line.  counter_pass: only five fused launches, five persistent
+    launches and one 1 GiB device-to-device copy (the scale of a counter
+    pass)."""
     import ctypes as C
     import numpy as np
     import nvshmem_amd as na
@@ -126,6 +127,11 @@ def batch_worker(counter_pass=False):
     if counter_pass:
         for _ in range(5):
             lib.nvshmemx_batch_plan_launch(plan, None)
+        raw = np.frombuffer(bytes(arr), np.uint8)
+        d_reqs = hip.malloc(raw.size)
+        hip.htod(d_reqs, raw)
+        for _ in range(5):
+            lib.nvshmemx_batch_put_persistent_on_stream(d_reqs, n, None)
         ref = hip.malloc(2 << 30)
         hip.check(hip.lib.hipMemcpyAsync(ref + (1 << 30), ref, 1 << 30, hip.D2D, None))
         hip.sync()
