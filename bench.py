@@ -840,6 +840,7 @@ def main():
     sweep = leg("sweep", args.no_sweep, run_sweep_leg)
     cpu = leg("cpu baseline", args.no_cpu_baseline, cpu_baseline)
     r = result
+    copy_bound = 2 / (1 / r["read_gbps"] + 1 / r["write_gbps"])  # read and write streams back to back
     line = {
         "metric": "put/get GB/s & HBM-roofline frac; p50 p+wait ns; batched vs per-call puts",
         "value": round(r["put_gbps"], 1),
@@ -866,13 +867,15 @@ def main():
         "hbm_ceilings_gbps": {
             "read": round(r["read_gbps"], 1), "write": round(r["write_gbps"], 1),
             "write_by_store_policy": r["write_by_policy"],
-            "copy_bound": round(2 / (1 / r["read_gbps"] + 1 / r["write_gbps"]), 1),
+            "copy_bound": round(copy_bound, 1),
             "put_frac_of_copy_bound": round(r["put_gbps"] * (1 / r["read_gbps"] + 1 / r["write_gbps"]) / 2, 4),
             "pattern": "1 GiB read-only stream of 16-byte non-temporal loads and write-only streams of 16-byte "
                        "stores (non-temporal and sc1; the faster counts); a copy moves every byte once each way, "
                        "so its bound is the two streams back to back"},
         "bulk_streams_gbps": {"sizes": r["streams_by_size"], "frac_of_peak_min": round(
             min(min(v["put_gbps"], v["get_gbps"]) for v in r["streams_by_size"].values()) / PEAK_HBM_GBPS, 4),
+            "frac_of_copy_bound_min": round(
+                min(min(v["put_gbps"], v["get_gbps"]) for v in r["streams_by_size"].values()) / copy_bound, 4),
             "pattern": "back-to-back float put (get) launches, PE 0 -> PE 1, buffers rotated over 1 GiB",
             "launch_anatomy_64MiB": r["anatomy"]},
         "p_wait_rtt_ns": {"p50": round(r["rtt_p50"], 1), "p99": round(r["rtt_p99"], 1),
