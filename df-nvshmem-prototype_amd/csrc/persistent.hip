@@ -38,8 +38,9 @@
  * was seen set.
  *
  * Launch shape, from measurements of the 4096-put step of 64 PEs (1.57 GB
- * payload, ms per step): one workgroup of 1024 threads per CU, chunks of 128
- * KiB, eight loads in flight per lane, 0.528; chunks of 64, 256 or 512 KiB
+ * payload, ms per step, before the leader and chunk changes below): one
+ * workgroup of 1024 threads per CU, chunks of 128 KiB, eight loads in flight
+ * per lane, 0.528; chunks of 64, 256 or 512 KiB
  * 0.584, 0.540, 0.560; sixteen loads in flight 0.562; two workgroups of 512
  * threads per CU 0.591, three of 512 threads with four loads in flight 0.659.
  * Fewer, wider workgroups keep fewer address streams open at once.  Each
@@ -48,10 +49,15 @@
  * workgroups of 512 threads per CU against 0.591 for the PE-ordered claims;
  * every wavefront working alone on 32 KiB chunks 1.07 (each of ~8000
  * concurrent streams waited ~12 us per access: the DRAM pages thrash); claim
- * counters sharing one cache line and an acquire per poll, 1.49.  The leaders
- * take about 24 us before the first chunk is claimed.  The host-planned fused
- * launch of batch.hip (source-ordered pieces, no device-side planning) stays
- * slightly faster for a static exchange: 0.51.
+ * counters sharing one cache line and an acquire per poll, 1.49.  Stamped
+ * from the constant clock, the leaders took 6-8 us for 16 requests and 15-18
+ * us for 4096 while thread 0 summed every PE's chunks with a 64-bit division
+ * each; wavefront 0 now does that with shuffles (4 and 13-17 us), and a chunk
+ * reads its entries 64 at a time instead of one dependent pair of loads per
+ * entry: the step went from 0.528 to 0.512, a batch of 16 requests of 64 B
+ * from 23.5 to 17.1 us.  The host-planned fused launch of batch.hip
+ * (source-ordered pieces, no device-side planning) takes 0.509 for the same
+ * step and about 3.5 us for a tiny batch.
  *
  * Scratch (per launch, device memory): the control slots, the chunk table
  * (65,536 entries; the chunk grows from 128 KiB so that a batch of any size
@@ -245,19 +251,28 @@ __device__ void lead(const nvshmemx_put_request_t *__restrict__ reqs, unsigned n
         if (i < kLeadCache) s_req[i] = u < (1u << 25) ? ((unsigned)q << 25) | u : 0xFFFFFFFFu;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long total = 0;
-        for (int q = 0; q < P; ++q) total += s_units[q];
+    if (threadIdx.x < 64) { /* wavefront 0, two PEs per lane (P <= 128) */
+        const unsigned lane = threadIdx.x, qa = lane, qb = lane + 64;
+        const unsigned long long ua = qa < (unsigned)P ? s_units[qa] : 0ull, ub = qb < (unsigned)P ? s_units[qb] : 0ull;
+        unsigned long long total = ua + ub;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) total += __shfl_xor(total, d, 64);
         /* sum over PEs of ceil(units / cu) <= total / cu + P <= table size */
         const unsigned long long need = (total + (kTableChunks - P) - 1) / (kTableChunks - P);
         const unsigned cu = (unsigned)(need > kChunkUnitsMin ? need : kChunkUnitsMin);
-        unsigned e0 = 0, c0 = 0;
-        for (int q = 0; q < p; ++q) e0 += s_cnt[q], c0 += (unsigned)((s_units[q] + cu - 1) / cu);
-        s_me.entry0 = e0;
-        s_me.nown = s_cnt[p];
-        s_me.chunk0 = c0;
-        s_me.nchunks = (unsigned)((s_units[p] + cu - 1) / cu);
-        s_me.chunk_units = cu;
+        /* entries and chunks of the PEs before p */
+        unsigned e0 = (qa < (unsigned)p ? s_cnt[qa] : 0u) + (qb < (unsigned)p ? s_cnt[qb] : 0u);
+        unsigned c0 = (qa < (unsigned)p ? (unsigned)((ua + cu - 1) / cu) : 0u) +
+                      (qb < (unsigned)p ? (unsigned)((ub + cu - 1) / cu) : 0u);
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) e0 += __shfl_xor(e0, d, 64), c0 += __shfl_xor(c0, d, 64);
+        if (lane == 0) {
+            s_me.entry0 = e0;
+            s_me.nown = s_cnt[p];
+            s_me.chunk0 = c0;
+            s_me.nchunks = (unsigned)((s_units[p] + cu - 1) / cu);
+            s_me.chunk_units = cu;
+        }
     }
     __syncthreads();
     const unsigned cu = s_me.chunk_units, e0 = s_me.entry0, c0 = s_me.chunk0;
@@ -293,33 +308,52 @@ __device__ void lead(const nvshmemx_put_request_t *__restrict__ reqs, unsigned n
     }
 }
 
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, unsigned j) {
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, j), hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), j);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 /* a workgroup copies chunk k of a queue: a request segment of kWaveSegMax
  * bytes or more with all its threads, shorter ones one wavefront each, in
- * turn */
+ * turn.  Entries are read 64 at a time, one per lane (the entry, then its
+ * request), and then taken in order from the lanes, so a chunk of many small
+ * requests waits for two memory round trips per 64 entries, not per entry. */
 __device__ __forceinline__ void run_chunk(const nvshmemx_put_request_t *__restrict__ reqs, const pinfo &inf, unsigned k,
                                           const pscr &S) {
     const uint64_t c_lo = (uint64_t)k * inf.chunk_units * kUnit, c_hi = c_lo + (uint64_t)inf.chunk_units * kUnit;
     const unsigned e_end = inf.entry0 + inf.nown;
     const unsigned wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     unsigned small = 0;
-    for (unsigned e = coherent_u32(S.table() + inf.chunk0 + k); e < e_end; ++e) {
-        const pentry ent = coherent_entry(S.entries() + e);
-        const uint64_t vs = (uint64_t)ent.vstart * kUnit;
-        if (vs >= c_hi) break;
-        const nvshmemx_put_request_t r = reqs[ent.req];
-        const uint64_t lo = (c_lo > vs ? c_lo : vs) - vs;
-        const uint64_t end = vs + r.nbytes;
-        const uint64_t hi = (c_hi < end ? c_hi : end) - vs;
-        if (hi <= lo) continue; /* the chunk starts in the request's padding */
-        char *dst = translate(r.target, r.pe) + lo;
-        const char *src = (const char *)r.source + lo;
-        const size_t n = (size_t)(hi - lo);
-        if (n >= kWaveSegMax) {
-            copy<16, kPUnroll, NT_STORE>(dst, src, n, threadIdx.x, kPBlock);
-        } else {
-            if (small % kPWaves == wave) copy<16, kPUnroll, NT_STORE>(dst, src, n, lane, 64);
-            ++small;
+    for (unsigned e = coherent_u32(S.table() + inf.chunk0 + k); e < e_end; e += 64) {
+        uint64_t vs = ~0ull, tgt = 0, src = 0, nb = 0;
+        unsigned pe = 0;
+        if (e + lane < e_end) {
+            const pentry ent = coherent_entry(S.entries() + e + lane);
+            vs = (uint64_t)ent.vstart * kUnit;
+            if (vs < c_hi) {
+                const nvshmemx_put_request_t r = reqs[ent.req];
+                tgt = (uint64_t)r.target, src = (uint64_t)r.source, nb = (uint64_t)r.nbytes, pe = (unsigned)r.pe;
+            }
         }
+        /* entries start in increasing order, so the lanes in the chunk are a prefix */
+        const unsigned cnt = (unsigned)__builtin_popcountll(__ballot(vs < c_hi));
+        for (unsigned j = 0; j < cnt; ++j) {
+            const uint64_t jvs = lane_u64(vs, j), jnb = lane_u64(nb, j);
+            const uint64_t lo = (c_lo > jvs ? c_lo : jvs) - jvs;
+            const uint64_t end = jvs + jnb;
+            const uint64_t hi = (c_hi < end ? c_hi : end) - jvs;
+            if (hi <= lo) continue; /* the chunk starts in the request's padding */
+            char *dst = translate((const void *)lane_u64(tgt, j), (int)__builtin_amdgcn_readlane(pe, j)) + lo;
+            const char *srcp = (const char *)lane_u64(src, j) + lo;
+            const size_t n = (size_t)(hi - lo);
+            if (n >= kWaveSegMax) {
+                copy<16, kPUnroll, NT_STORE>(dst, srcp, n, threadIdx.x, kPBlock);
+            } else {
+                if (small % kPWaves == wave) copy<16, kPUnroll, NT_STORE>(dst, srcp, n, lane, 64);
+                ++small;
+            }
+        }
+        if (cnt < 64) break;
     }
 }
 
@@ -330,8 +364,8 @@ __device__ __forceinline__ void run_chunk(const nvshmemx_put_request_t *__restri
 __device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__ reqs, const pscr &S, int P) {
     __shared__ pinfo s_inf[NVSHMEMI_MAX_PES];
     __shared__ unsigned s_total, s_k[2];
-    __shared__ int s_qk[2];
-    if (threadIdx.x == 0) s_total = 0;
+    __shared__ int s_qk[2], s_missing;
+    if (threadIdx.x == 0) s_total = 0, s_missing = 0;
     __syncthreads();
     if ((int)threadIdx.x < P) {
         const int q = (int)threadIdx.x;
@@ -340,12 +374,13 @@ __device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__
             inf = coherent_info(S.info(q));
         } else { /* a queue that never appeared: its chunks are skipped */
             inf.entry0 = inf.nown = inf.chunk0 = inf.nchunks = inf.chunk_units = 0u;
+            s_missing = 1;
         }
         s_inf[q] = inf;
         if (inf.nchunks) atomicMax(&s_total, inf.chunk0 + inf.nchunks);
     }
     __syncthreads();
-    if (threadIdx.x == 0) /* keep first chunks non-decreasing over a missing queue */
+    if (threadIdx.x == 0 && s_missing) /* keep first chunks non-decreasing over a missing queue */
         for (int q = 1; q < P; ++q)
             if (!s_inf[q].chunk_units) s_inf[q].chunk0 = s_inf[q - 1].chunk0 + s_inf[q - 1].nchunks;
     __syncthreads();
