@@ -179,3 +179,31 @@ def test_persistent_between_processes():
     outs = run_job(2, "persistent", 300)
     for o in outs:
         assert o["errors"] == 0 and o["batch_ok"], o
+
+
+def test_persistent_128_pes(local_pes, hip, oracle):
+    """the most local PEs the library holds (128): leaders 64..127 take the
+    second half of each lane's PEs when the queues' offsets are summed.  Runs
+    last in this file: it restarts the library the module fixture set up."""
+    npes, src, dst = 128, 1 << 20, 1 << 20
+    lib = local_pes.start(npes, wgs_per_pe=1, heap="2M")
+    base = lib.nvshmalloc(src + dst)
+    images = []
+    for pe in range(npes):
+        p = lib.nvshmem_ptr(base, pe)
+        lib.nvshmemh_fill_pcg32(p, src // 4, 500 + pe, 54, None)
+        hip.memset(p + src, 0, dst)
+        img = np.zeros(src + dst, np.uint8)
+        img[:src] = oracle.pcg32(src // 4, 500 + pe).view(np.uint8)
+        images.append(img)
+    hip.sync()
+    reqs = na.gen_requests(lib, 53, 1024, npes, src, dst, 6, 14)
+    arr = na.make_requests(lib, reqs, base, 0, src)
+    oracle.batch_apply(images, reqs, 0, src)
+    _run(lib, hip, arr, len(arr), times=2)
+    assert lib.nvshmemx_get_last_error() == 0
+    for pe in range(npes):
+        got = hip.dtoh(lib.nvshmem_ptr(base, pe) + src, dst)
+        if not np.array_equal(got, images[pe][src:]):
+            bad = np.flatnonzero(got != images[pe][src:])
+            raise AssertionError(f"PE {pe}: {bad.size} bytes differ, first at dst offset {bad[0]}")
