@@ -540,7 +540,7 @@ def launch_anatomy(lib, hip, dst, src):
     workgroup ends (the drain), and the gap from the last end to the next
     launch's first start (kernel end, fences, next dispatch)."""
     import numpy as np
-    words = ANATOMY_LAUNCHES * 8192 * 2
+    words = ANATOMY_LAUNCHES * 16384 * 2
     stamps = hip.malloc(words * 8)
     try:
         grids = []
@@ -662,7 +662,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--layout", choices=["process", "local"], default="process")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")

@@ -446,17 +446,16 @@ __global__ void __launch_bounds__(256) k_stream_write_sc1(nvshmemi::u32x4 *a, si
     }
 }
 
-/* The library's interleaved copy as it runs for 32-128 MiB puts (every lane
- * of a 8192 x 512 grid, 16-byte accesses, non-temporal loads, sc1 stores),
- * with each workgroup's start and end stamped from the constant clock, to see
- * where the time of back-to-back launches goes. */
-constexpr unsigned kStampGrid = 8192, kStampBlock = 512;
-__global__ void __launch_bounds__(512) k_copy_stamped(char *dst, const char *src, size_t nbytes,
+/* The library's copy as it runs for puts from 32 MiB up (tiles of 256
+ * 16-byte vectors, one access per lane, non-temporal loads, sc1 stores; 16384
+ * tiles make 64 MiB), with each workgroup's start and end stamped from the
+ * constant clock, to see where the time of back-to-back launches goes. */
+constexpr unsigned kStampGrid = 16384, kStampBlock = 256;
+__global__ void __launch_bounds__(256) k_copy_stamped(char *dst, const char *src, size_t nbytes,
                                                       unsigned long long *stamps) {
     const unsigned long long t0 = wall_clock64();
-    nvshmemi::copy<16, 2, nvshmemi::NT_LOAD | nvshmemi::ST_SC1>(dst, src, nbytes,
-                                                                (size_t)blockIdx.x * blockDim.x + threadIdx.x,
-                                                                (size_t)gridDim.x * blockDim.x);
+    nvshmemi::copy_tiled<16, 1, nvshmemi::NT_LOAD | nvshmemi::ST_SC1>(dst, src, nbytes, 1, blockIdx.x, gridDim.x,
+                                                                      threadIdx.x, blockDim.x);
     __syncthreads();
     if (threadIdx.x == 0) {
         stamps[2 * (size_t)blockIdx.x] = t0;

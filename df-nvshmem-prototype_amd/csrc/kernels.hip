@@ -74,39 +74,38 @@ static copy_kernel_t pick_copy_kernel(const CopyConfig &c) {
     return c.mode == 1 ? pick_width<1>(c) : pick_width<0>(c);
 }
 
-/* Launch shape by size, from sweeps on MI355X of {interleaved, chunked,
- * tiled} x block {256,512,1024} x grid x unroll {1,2,4,8} x streams x store
- * policy (copies rotated over a 1 GiB footprint so the 256 MB MALL cannot
- * serve them; streams of back-to-back launches):
- *   >= 256 MiB        tiled, 256 threads x 2 accesses (8 KiB tiles):    6.5 TB/s at 256 MiB, 6.6 at 512 MiB - 1 GiB,
- *                     also into a peer process's IPC-mapped heap
- *   128 .. 256 MiB    tiled, 512 threads x 2 accesses (16 KiB tiles):   6.26 TB/s at 128 MiB
- *   32 .. 128 MiB     interleaved, 8192 x 512, 2 in flight:            6.04 TB/s at 64 MiB
- *   below             interleaved, grid sized to the copy (<= 4096)
- * From 32 MiB up, loads are non-temporal and stores are written through the
- * L2 (sc1): a kernel then leaves no dirty lines for its end-of-kernel
- * release, worth 1.5-3% over non-temporal stores at 64 MiB - 1 GiB (64 MiB
- * 5.93 -> 6.04, 128 MiB 6.13 -> 6.26, 1 GiB 6.5 -> 6.6 TB/s, same box).
- * No shape or policy brought 64 MiB to 6.4 TB/s (80% of 8 TB/s): each launch
- * pays about 1.5 us of ramp, tail and release, 7% of a 21 us copy.  Puts
- * alternated over two or more streams (they may overlap: puts are unordered
- * until quiet) reached 6.12 TB/s at 64 MiB with two and lost at 256 MiB, so
- * the library keeps one stream. */
+/* Launch shape by size, from sweeps on MI355X through this library's put
+ * into a peer process's IPC-mapped heap (copies rotated over a 1 GiB
+ * footprint so the 256 MB MALL cannot serve them; streams of back-to-back
+ * launches).  From 32 MiB up the best shape is a tiled one with ONE 16-byte
+ * access per lane: each workgroup of 256 threads copies one contiguous 4 KiB
+ * tile and exits, so the dispatcher streams short-lived workgroups through a
+ * narrow moving window.  Two runs, TB/s:
+ *
+ *   size      tiled 256x1  tiled 128x1  tiled 512x1  tiled 256x2  tiled 512x2  interleaved 8192x512x2
+ *   64 MiB    5.51-6.03    5.51-5.88    5.49-5.83    5.78-5.82    5.73-5.76    5.98-6.02
+ *   128 MiB   6.41-6.42    6.24-6.32    6.21-6.24    6.24-6.29    6.15-6.20    5.95-5.96
+ *   256 MiB   6.59-6.60    6.44-6.49    6.43-6.44    6.45-6.48    6.43-6.45    6.03-6.06
+ *   512 MiB   6.67-6.68    6.58-6.59    6.52         6.61         6.48-6.53    5.69-5.71
+ *   1 GiB     6.70-6.72    6.60-6.65    6.57         6.61         6.35-6.39    5.42-5.45
+ *
+ * The same ranking held with both PEs in one process.  Two accesses per lane
+ * cost 1-4% wherever they were measured; the interleaved shape with one
+ * access per lane (8192 x 512) was 6.09 TB/s at 64 MiB and 5.5 at 1 GiB.
+ * Below 32 MiB a grid sized to the copy (interleaved, several accesses in
+ * flight per lane).  From 32 MiB up loads are non-temporal and stores are
+ * written through the L2 (sc1): a kernel then leaves no dirty lines for its
+ * end-of-kernel release, worth 1.5-3% over non-temporal stores.  No shape or
+ * policy brings 64 MiB to 6.4 TB/s (80% of 8 TB/s): each launch pays about
+ * 1.2 us between launches and 1 us of drain, and copies alternated over
+ * several streams ran slower, not faster. */
 CopyConfig default_copy_config(size_t nbytes) {
     CopyConfig c;
     c.streams = 1;
     c.vec_bytes = 16;
     c.nontemporal = NT_LOAD | ST_SC1;
-    if (nbytes >= ((size_t)256 << 20)) {
-        c.mode = 2, c.block = 256, c.workgroups = 0, c.unroll = 2, c.streams = 1;
-        return c;
-    }
-    if (nbytes >= ((size_t)128 << 20)) {
-        c.mode = 2, c.block = 512, c.workgroups = 0, c.unroll = 2, c.streams = 1;
-        return c;
-    }
     if (nbytes >= ((size_t)32 << 20)) {
-        c.mode = 0, c.block = 512, c.workgroups = 8192, c.unroll = 2;
+        c.mode = 2, c.block = 256, c.workgroups = 0, c.unroll = 1, c.streams = 1;
         return c;
     }
     c.nontemporal = NT_BOTH;
