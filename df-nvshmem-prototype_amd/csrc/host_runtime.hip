@@ -434,9 +434,11 @@ extern "C" void nvshmemx_putmem_config_on_stream(void *target, const void *sourc
         if (cfg->block > 0) c.block = cfg->block;
         if (cfg->vec_bytes > 0) c.vec_bytes = cfg->vec_bytes;
         if (cfg->unroll > 0) c.unroll = cfg->unroll;
-        c.nontemporal = cfg->nontemporal;
-        if (cfg->mode == 2 && c.mode != 2 && cfg->workgroups <= 0) c.workgroups = 0; /* one tile per workgroup */
-        c.mode = cfg->mode;
+        if (cfg->nontemporal >= 0) c.nontemporal = cfg->nontemporal;
+        if (cfg->mode >= 0) {
+            if (cfg->mode == 2 && c.mode != 2 && cfg->workgroups <= 0) c.workgroups = 0; /* one tile per workgroup */
+            c.mode = cfg->mode;
+        }
         if (cfg->streams > 0) c.streams = cfg->streams;
     }
     launch_copy(nvshmem_ptr(target, pe), source, nbytes, c, s);

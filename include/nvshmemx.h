@@ -75,8 +75,9 @@ void nvshmemx_int_get_on_stream(int *target, const int *source, size_t len, int 
 void nvshmemx_float_get_on_stream(float *target, const float *source, size_t len, int pe, hipStream_t stream);
 void nvshmemx_double_get_on_stream(double *target, const double *source, size_t len, int pe, hipStream_t stream);
 
-/* Launch shape of a host-launched copy.  Zero / negative fields keep the
- * library's choice for that field. */
+/* Launch shape of a host-launched copy.  Negative fields keep the library's
+ * choice for that field, and so do zero workgroups, block, vec_bytes, unroll
+ * and streams (zero is a valid nontemporal policy and mode). */
 typedef struct nvshmemx_copy_config {
     int workgroups;  /* grid size */
     int block;       /* threads per workgroup (64..1024, multiple of 64) */

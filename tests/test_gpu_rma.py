@@ -190,6 +190,20 @@ def test_copy_configs(local_pes, hip, oracle, mode, block, unroll, streams, work
                          seed=_seed("config", mode, block, dtype, nelem, toff, soff))
 
 
+@pytest.mark.parametrize("fields", [(-1, -1, -1, -1, -1, -1, -1), (0, 512, 0, 0, -1, -1, 0), (-1, -1, 8, 2, 0, -1, -1)])
+def test_copy_config_keeps_library_choices(local_pes, hip, oracle, fields):
+    """negative fields (and zero sizes) of nvshmemx_copy_config_t keep the
+    library's choice: all of them, only the block size overridden, only the
+    access width, accesses in flight and store policy overridden; sizes on
+    both sides of the 32 MiB shape boundary"""
+    lib = local_pes.start(2, heap="256M")
+    cfg = na.CopyConfig(*fields)
+    for nelem in [0, 17, 4099, (3 << 20) // 4 + 5, (40 << 20) // 4 + 3]:
+        lib.nvshmcleanup()
+        _run_put(lib, hip, oracle, path="config", dtype="int", nelem=nelem, toff=4, soff=0, cfg=cfg,
+                 seed=_seed("keep", fields, nelem))
+
+
 def test_large_put_default_shape(local_pes, hip, oracle):
     """a 512 MiB + 3-element float put, misaligned by 4 bytes, on the library's
     default shape for large copies"""
