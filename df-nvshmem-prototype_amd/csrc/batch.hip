@@ -124,7 +124,7 @@ void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, co
     p->nsets = nsets;
     p->wgs_per_set = wgs_per_set;
     p->block = block;
-    p->nontemporal = nontemporal & 3;
+    p->nontemporal = nontemporal & 7;
     p->npieces = all.size();
     p->total_bytes = total;
 }
@@ -142,6 +142,15 @@ void plan_launch(nvshmemx_batch_plan *p, hipStream_t s) {
         break;
     case NT_STORE:
         hipLaunchKernelGGL(k_batch<NT_STORE>, dim3(grid), dim3(p->block), 0, s, p->d_pieces, p->d_wg_off,
+                           p->d_wave_off);
+        break;
+    case ST_SC1:
+    case ST_SC1 | NT_STORE: /* write-through takes precedence over non-temporal stores */
+        hipLaunchKernelGGL(k_batch<ST_SC1>, dim3(grid), dim3(p->block), 0, s, p->d_pieces, p->d_wg_off, p->d_wave_off);
+        break;
+    case ST_SC1 | NT_LOAD:
+    case ST_SC1 | NT_BOTH:
+        hipLaunchKernelGGL(k_batch<ST_SC1 | NT_LOAD>, dim3(grid), dim3(p->block), 0, s, p->d_pieces, p->d_wg_off,
                            p->d_wave_off);
         break;
     default:

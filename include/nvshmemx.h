@@ -144,7 +144,8 @@ typedef struct nvshmemx_batch_choice {
 typedef struct nvshmemx_batch_launch {
     int workgroups;  /* fused grid size (0: one workgroup per piece, at most 8192) */
     int block;       /* threads per workgroup, multiple of 64 (0: 1024) */
-    int nontemporal; /* bit 0 loads, bit 1 stores (NULL launch: stores only) */
+    int nontemporal; /* bit 0: non-temporal loads, bit 1: non-temporal stores, bit 2: stores written through the
+                        L2 (sc1) instead (NULL launch: non-temporal stores only) */
     int discipline;  /* NVSHMEMX_BATCH_SHARED or NVSHMEMX_BATCH_OWNER */
 } nvshmemx_batch_launch_t;
 

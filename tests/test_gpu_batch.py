@@ -66,7 +66,8 @@ def _run_plan(lib, hip, arr, n, choices=None, launch=None):
     return plan
 
 
-@pytest.mark.parametrize("variant", ["default", "random_choices", "owner", "per_call"])
+@pytest.mark.parametrize("variant", ["default", "random_choices", "owner", "per_call", "sc1_stores",
+                                     "sc1_stores_nt_loads"])
 def test_batch_matches_oracle(setup, hip, oracle, variant):
     lib, base, images = setup
     reqs = na.gen_requests(lib, 8, 512, NPES, SRC, DST)
@@ -89,6 +90,14 @@ def test_batch_matches_oracle(setup, hip, oracle, variant):
             launch = na.BatchLaunch(300, 512, 0, na.BATCH_SHARED)
         if variant == "owner":
             launch = na.BatchLaunch(0, 256, 3, na.BATCH_OWNER)
+        if variant.startswith("sc1_stores"):  # stores written through the L2 (bit 2), random access widths
+            rng = np.random.default_rng(9)
+            choices = (na.BatchChoice * n)()
+            for i in range(n):
+                choices[i].pieces = int(rng.integers(1, 65))
+                choices[i].vec_bytes = int(rng.choice([4, 8, 16]))
+                choices[i].mode = int(rng.integers(0, 2))
+            launch = na.BatchLaunch(0, 0, 5 if variant.endswith("nt_loads") else 4, na.BATCH_SOURCE_ORDER)
         plan = _run_plan(lib, hip, arr, n, choices, launch)
         assert lib.nvshmemx_batch_plan_bytes(plan) == int(reqs["nbytes"].sum())
         lib.nvshmemx_batch_plan_destroy(plan)
