@@ -87,14 +87,16 @@ PUTFLAG_SIZES = [4, 64, 1024, 4096, 16384, 65536]
 STREAM_SIZES_LOG2 = [26, 27, 28, 29]  # 64..512 MiB bulk streams beside the 1 GiB one
 PUTFLAG_ITERS, PUTFLAG_SKIP = 220, 20
 BATCH_NPES, BATCH_WGS, BATCH_SRC, BATCH_DST, BATCH_REQS, BATCH_SEED = 64, 4, 32 << 20, 64 << 20, 4096, 7
+POLICY_ONE_PIECE, POLICY_TILES = 11, 12  # batch policies: one workgroup piece per request; every request as tiles
 
 
 def batch_worker(counter_pass=False):
     """Uneven-put workload on 64 workgroup-partitioned PEs in one process:
     4096 puts per step, log-uniform 64 B..4 MiB, seed 7.  Times one fused
     launch, the same under a captured hipGraph, the owner discipline (each PE's
-    workgroups run the puts it initiated), the persistent kernel whose
-    workgroup PEs plan their own puts on the device (library-allocated and
+    workgroups run the puts it initiated), every request cut into tiles (one
+    workgroup a tile, launched and replayed from a hipGraph), the persistent
+    kernel planned on the device by the issuing PEs (library-allocated and
     static scratch) and the per-call path (one launch per put).  Prints one
     JSON line.  counter_pass: only five fused launches, five persistent
     launches and one 1 GiB device-to-device copy (the scale of a counter
@@ -147,6 +149,20 @@ def batch_worker(counter_pass=False):
     lib.nvshmemx_batch_plan_capture(plan, 10)
     out["graph_ms"] = timed(hip, lambda: lib.nvshmemx_batch_plan_graph_launch(plan, None), 3, 1) / 10
     lib.nvshmemx_batch_plan_destroy(plan)
+    # every request cut into tiles, one workgroup a tile, in source order
+    tile_choices = (na.BatchChoice * n)()
+    for i in range(n):
+        tile_choices[i].pieces, tile_choices[i].vec_bytes, tile_choices[i].mode = 1, 16, 2
+    tiles = plan_of(None, tile_choices)
+    for pe in range(BATCH_NPES):  # the check below then sees only what the tiles wrote
+        hip.memset(lib.nvshmem_ptr(base, pe) + BATCH_SRC, 0, BATCH_DST)
+    hip.sync()
+    out["tiles_ms"] = timed(hip, lambda: lib.nvshmemx_batch_plan_launch(tiles, None), 20, 3)
+    out["tiles"] = int(lib.nvshmemx_batch_plan_pieces(tiles))
+    out["bitexact"] = out["bitexact"] and int(lib.nvshmemh_verify_requests(arr, n)) == 0
+    lib.nvshmemx_batch_plan_capture(tiles, 10)
+    out["tiles_graph_ms"] = timed(hip, lambda: lib.nvshmemx_batch_plan_graph_launch(tiles, None), 3, 1) / 10
+    lib.nvshmemx_batch_plan_destroy(tiles)
     # each PE's puts on its own 128 workgroups (the default grid split by PE)
     owner = plan_of(na.BatchLaunch(8192, 0, 2, na.BATCH_OWNER))
     out["owner_ms"] = timed(hip, lambda: lib.nvshmemx_batch_plan_launch(owner, None), 10, 2)
@@ -176,9 +192,10 @@ def learned_worker(n_batches, train_steps):
     1. per-request autotuner: every one of the 54 candidates measured at every
        point of a 1782-point feature grid (96,228 samples), and the
        per-request MLP trained on them on the GPU;
-    2. batch-level autotuner: the 12 batch policies (the rule, the
+    2. batch-level autotuner: the 13 batch policies (the rule, the
        per-request network's choices, uniform wavefront / workgroup pieces of
-       4 KiB .. 1 MiB, one piece per request) of self-generated batches of 8
+       4 KiB .. 1 MiB, one piece per request, every request as tiles) of
+       self-generated batches of 8
        to 8192 puts timed on the device, and the batch-level MLP trained on
        them (inputs: log2 batch bytes, log2 request count, mean and largest
        log2 request size);
@@ -262,7 +279,7 @@ def learned_worker(n_batches, train_steps):
     per_call, bad = [], 0
     t0 = time.time()
     for nreq, nb, seed0 in ((BATCH_REQS, n_batches, 1000), (128, 100, 5128000), (16, 100, 5016000)):
-        r = {"judge": [], "batch_net": [], "request_net": [], "rule": [], "one_piece": []}
+        r = {"judge": [], "batch_net": [], "request_net": [], "rule": [], "one_piece": [], "tiles": []}
         picks = np.zeros(npol, np.int64)
         for b in range(nb):
             arr = batch_of(seed0 + b, nreq, 22)
@@ -274,7 +291,8 @@ def learned_worker(n_batches, train_steps):
             r["batch_net"].append(float(ms[pol]))
             r["request_net"].append(float(ms[1]))
             r["rule"].append(float(ms[0]))
-            r["one_piece"].append(float(ms[npol - 1]))
+            r["one_piece"].append(float(ms[POLICY_ONE_PIECE]))
+            r["tiles"].append(float(ms[POLICY_TILES]))
             if nreq == BATCH_REQS and b < 10:
                 per_call.append(timed(hip, lambda: lib.nvshmemx_put_requests_on_stream(arr, n, None), 1, 0))
             if b < 2:
@@ -520,9 +538,13 @@ def run_batch_leg():
     best = min(rec["fused_ms"], rec["graph_ms"])
     rec["fused_gbps"] = round(2 * rec["payload_bytes"] / (best * 1e-3) / 1e9, 1)
     rec["speedup_vs_per_call"] = round(rec["per_call_ms"] / best, 2)
-    for k in ("fused_ms", "graph_ms", "owner_ms", "per_call_ms", "persistent_ms", "persistent_noalloc_ms"):
+    for k in ("fused_ms", "graph_ms", "owner_ms", "per_call_ms", "persistent_ms", "persistent_noalloc_ms", "tiles_ms",
+              "tiles_graph_ms"):
         if k in rec:
             rec[k] = round(rec[k], 4)
+    if "tiles_ms" in rec:
+        rec["tiles_gbps"] = round(2 * rec["payload_bytes"] / (rec["tiles_ms"] * 1e-3) / 1e9, 1)
+        rec["tiles_speedup_vs_per_call"] = round(rec["per_call_ms"] / rec["tiles_ms"], 2)
     if "persistent_ms" in rec:
         rec["persistent_gbps"] = round(2 * rec["payload_bytes"] / (rec["persistent_ms"] * 1e-3) / 1e9, 1)
         rec["persistent_speedup_vs_per_call"] = round(rec["per_call_ms"] / rec["persistent_ms"], 2)

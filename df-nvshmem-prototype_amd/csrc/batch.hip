@@ -63,6 +63,32 @@ __global__ void __launch_bounds__(1024) k_batch(const piece_t *__restrict__ piec
     for (unsigned int j = wave_off[w]; j < wave_end; ++j) copy_piece<NT>(pieces[j], threadIdx.x & 63u, 64);
 }
 
+/* One tile per workgroup (choice mode 2): lane l copies accesses l, l +
+ * blockDim, ... of the tile (two for a whole tile of kTileAccesses with
+ * kTileBlock threads), each load stored before the next is issued; width-1
+ * tiles (unaligned heads and tails) go byte by byte. */
+template <typename V, int NT>
+__device__ __forceinline__ void tile_copy(const piece_t &p) {
+    const size_t step = (size_t)blockDim.x * sizeof(V);
+    for (size_t o = (size_t)threadIdx.x * sizeof(V); o < p.nbytes; o += step)
+        vstore_p<NT>((V *)(p.dst + o), vload<(NT & NT_LOAD) != 0>((const V *)(p.src + o)));
+}
+template <int NT>
+__global__ void __launch_bounds__(kTileBlock) k_batch_tiles(const piece_t *__restrict__ tiles, size_t ntiles) {
+    for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) { /* one pass unless the grid was capped */
+        const piece_t p = tiles[t];
+        switch (p.vec) {
+        case 16: tile_copy<u32x4, NT>(p); break;
+        case 8: tile_copy<u32x2, NT>(p); break;
+        case 4: tile_copy<unsigned int, NT>(p); break;
+        case 2: tile_copy<unsigned short, NT>(p); break;
+        default:
+            for (size_t i = threadIdx.x; i < p.nbytes; i += blockDim.x) p.dst[i] = p.src[i];
+            break;
+        }
+    }
+}
+
 } // namespace nvshmemi
 
 using namespace nvshmemi;
@@ -72,7 +98,9 @@ struct nvshmemx_batch_plan {
     int wgs_per_set = 1;
     int block = 256;
     int nontemporal = NT_BOTH;
-    size_t npieces = 0;
+    size_t npieces = 0;     /* pieces of the lists and tiles */
+    size_t tile0 = 0;       /* first tile in d_pieces; pieces before it belong to the lists */
+    size_t ntiles = 0;
     size_t total_bytes = 0;
     size_t piece_capacity = 0;
     size_t wg_capacity = 0;
@@ -126,12 +154,35 @@ void plan_fill(nvshmemx_batch_plan *p, const std::vector<resolved_req> &reqs, co
     p->block = block;
     p->nontemporal = nontemporal & 7;
     p->npieces = all.size();
+    p->tile0 = L.tile0;
+    p->ntiles = L.ntiles;
     p->total_bytes = total;
 }
 
 nvshmemx_batch_plan *plan_new() { return new nvshmemx_batch_plan(); }
 
+/* one workgroup per tile, the grid's threads within 32 bits */
+template <int NT> static void launch_tiles(const nvshmemx_batch_plan *p, hipStream_t s) {
+    const size_t grid = std::min<size_t>(p->ntiles, 0xFFFFFFFFu / kTileBlock);
+    hipLaunchKernelGGL(k_batch_tiles<NT>, dim3((unsigned)grid), dim3(kTileBlock), 0, s, p->d_pieces + p->tile0,
+                       p->ntiles);
+}
+
 void plan_launch(nvshmemx_batch_plan *p, hipStream_t s) {
+    if (p->ntiles) { /* the tiles' launch; a plan of tiles only is this one launch */
+        switch (p->nontemporal) {
+        case NT_BOTH: launch_tiles<NT_BOTH>(p, s); break;
+        case NT_LOAD: launch_tiles<NT_LOAD>(p, s); break;
+        case NT_STORE: launch_tiles<NT_STORE>(p, s); break;
+        case ST_SC1:
+        case ST_SC1 | NT_STORE: launch_tiles<ST_SC1>(p, s); break;
+        case ST_SC1 | NT_LOAD:
+        case ST_SC1 | NT_BOTH: launch_tiles<ST_SC1 | NT_LOAD>(p, s); break;
+        default: launch_tiles<NT_NONE>(p, s); break;
+        }
+        NVSHMEMI_HIP(hipGetLastError());
+        if (p->tile0 == 0) return;
+    }
     const unsigned grid = (unsigned)(p->nsets * p->wgs_per_set);
     switch (p->nontemporal) {
     case NT_BOTH:

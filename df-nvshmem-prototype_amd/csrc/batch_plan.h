@@ -60,6 +60,29 @@ inline void split_request(const char *src, char *dst, size_t nbytes, const nvshm
     }
 }
 
+/* Tiles (choice mode 2): a request cut into tiles of kTileAccesses accesses
+ * of the widest width <= maxw that the relative alignment of its source and
+ * target allows, each tile copied by one workgroup of kTileBlock threads (two
+ * accesses a lane, one after the other).  The bytes before the target's first
+ * boundary of that width, and those after the last whole access, go as byte
+ * tiles (width 1). */
+constexpr unsigned kTileAccesses = 512;
+constexpr int kTileBlock = 256;
+inline void tile_request(const char *src, char *dst, size_t nbytes, int maxw, std::vector<piece_t> &out) {
+    if (nbytes == 0) return;
+    const uintptr_t rel = (uintptr_t)src ^ (uintptr_t)dst;
+    size_t w = maxw >= 16 ? 16 : (maxw >= 8 ? 8 : 4);
+    while (w > 1 && (rel & (w - 1))) w >>= 1;
+    const size_t head = std::min(nbytes, (size_t)((w - ((uintptr_t)dst & (w - 1))) & (w - 1)));
+    if (head) out.push_back(piece_t{src, dst, (unsigned long long)head, 1, 0});
+    const size_t body = (nbytes - head) / w * w, step = (size_t)kTileAccesses * w;
+    for (size_t off = 0; off < body; off += step)
+        out.push_back(piece_t{src + head + off, dst + head + off, (unsigned long long)std::min(step, body - off),
+                              (unsigned)w, 0});
+    const size_t tail = nbytes - head - body;
+    if (tail) out.push_back(piece_t{src + head + body, dst + head + body, (unsigned long long)tail, 1, 0});
+}
+
 /* LPT: pieces (any order) onto `workers` bins; returns per-worker lists.
  * `base_load[w]` is load a worker already carries; `share` divides a piece's
  * bytes among the lanes of a workgroup worker's wavefronts. */
@@ -121,12 +144,15 @@ inline void scatter(std::vector<piece_t> &v) {
 
 /* A plan's host arrays: the pieces of every worker list back to back, the
  * offset of each workgroup's list (G + 1 entries) and of each wavefront's
- * list (G * block / 64 + 1 entries), G = nsets * wgs_per_set. */
+ * list (G * block / 64 + 1 entries), G = nsets * wgs_per_set; then the tiles
+ * of the requests cut into tiles, sorted by source address, from `tile0` on
+ * (one workgroup each, whatever set their request is in). */
 struct plan_layout {
     std::vector<piece_t> pieces;
     std::vector<unsigned int> wg_off, wave_off;
     int wgs_per_set = 0;
     size_t total_bytes = 0;
+    size_t tile0 = 0, ntiles = 0;
 };
 
 inline void build_layout(const std::vector<resolved_req> &reqs, const nvshmemx_batch_choice_t *choices,
@@ -135,6 +161,7 @@ inline void build_layout(const std::vector<resolved_req> &reqs, const nvshmemx_b
     if (block < 64 || block > 1024 || block % 64) NVSHMEMI_FATAL("batch block size %d invalid", block);
     const int waves_per_wg = block / 64;
     std::vector<std::vector<piece_t>> wg(nsets), wave(nsets);
+    std::vector<piece_t> tiles;
     size_t total = 0;
     for (size_t i = 0; i < reqs.size(); ++i) {
         const resolved_req &r = reqs[i];
@@ -142,7 +169,8 @@ inline void build_layout(const std::vector<resolved_req> &reqs, const nvshmemx_b
         if (one_choice) ch = *one_choice;
         else if (choices) ch = choices[i];
         else ch = default_choice(r.nbytes);
-        split_request(r.src, r.dst, r.nbytes, ch, ch.mode ? wg[r.set] : wave[r.set]);
+        if (ch.mode == 2) tile_request(r.src, r.dst, r.nbytes, ch.vec_bytes, tiles);
+        else split_request(r.src, r.dst, r.nbytes, ch, ch.mode ? wg[r.set] : wave[r.set]);
         total += r.nbytes;
     }
     if (wgs_per_set <= 0) {
@@ -207,6 +235,12 @@ inline void build_layout(const std::vector<resolved_req> &reqs, const nvshmemx_b
             }
         vo[W] = (unsigned)all.size();
     }
+    /* tiles in source order: the workgroups running at any moment read
+     * neighbouring sources (see source_runs) */
+    std::stable_sort(tiles.begin(), tiles.end(), [](const piece_t &a, const piece_t &b) { return a.src < b.src; });
+    L.tile0 = all.size();
+    L.ntiles = tiles.size();
+    all.insert(all.end(), tiles.begin(), tiles.end());
     L.wgs_per_set = wgs_per_set;
     L.total_bytes = total;
 }

@@ -9,9 +9,11 @@
  * also depends on the rest of the batch: a short launch of 128 mixed puts is
  * bound by how evenly its pieces fill the device, a long one of 4096 by how
  * the pieces walk memory.  Measured on the device (64 PEs, held-out batches
- * of the uneven-put workload), cutting every request into 64 KiB workgroup
- * pieces was the best plan of 128-put and 4096-put batches, while the
- * per-request network's own choices were the best of 16-put batches.
+ * of the uneven-put workload), among the piece policies cutting every request
+ * into 64 KiB workgroup pieces was the best plan of 128-put and 4096-put
+ * batches, and the per-request network's own choices the best of 16-put
+ * batches; every request cut into tiles (one short-lived workgroup a tile,
+ * policy 12) beat them all at 4096 puts and matched the best at 16 and 128.
  *
  * So the batch is planned by one of a family of policies:
  *   0       the library's rule (per-request default_choice);
@@ -19,10 +21,12 @@
  *   2 .. 5  every request cut into wavefront pieces of about 4, 8, 16, 32 KiB;
  *   6 .. 10 into workgroup pieces of about 64, 128, 256, 512 KiB and 1 MiB;
  *   11      one workgroup piece per request;
+ *   12      every request cut into tiles of 512 accesses, one workgroup of
+ *           256 threads a tile, all tiles in source order (choice mode 2);
  * (16-byte accesses where the alignment allows; at most 256 pieces a
  * request; pieces of at least 4 KiB, as every plan).  Timing every policy's
  * plan on the device is the exhaustive batch-level judge.  A second network
- * of the same shape (4 -> 32 -> 32 ReLU -> 12, the same HIP kernels) reads
+ * of the same shape (4 -> 32 -> 32 ReLU -> 13, the same HIP kernels) reads
  * four batch-level inputs -- log2 of the batch's bytes, of its request count,
  * the mean and the largest log2 request size -- and predicts the log-time of
  * every policy; it is trained on batches the library generates and times
@@ -41,14 +45,15 @@ using namespace nvshmemi;
 
 namespace {
 struct policy {
-    int kind;       /* 0 rule, 1 per-request network, 2 uniform pieces */
+    int kind;       /* 0 rule, 1 per-request network, 2 uniform pieces, 3 tiles */
     size_t gran;    /* uniform: bytes per piece (0: one piece per request) */
-    int mode;       /* uniform: 0 wavefront pieces, 1 workgroup pieces */
+    int mode;       /* uniform: 0 wavefront pieces, 1 workgroup pieces; tiles: 2 */
 };
 const policy kPolicies[] = {
     {0, 0, 1},         {1, 0, 1},         {2, 4u << 10, 0},   {2, 8u << 10, 0},
     {2, 16u << 10, 0}, {2, 32u << 10, 0}, {2, 64u << 10, 1},  {2, 128u << 10, 1},
     {2, 256u << 10, 1}, {2, 512u << 10, 1}, {2, 1u << 20, 1}, {2, 0, 1},
+    {3, 0, 2},
 };
 constexpr int kNumPolicies = (int)(sizeof(kPolicies) / sizeof(kPolicies[0]));
 
@@ -90,6 +95,12 @@ extern "C" int nvshmemx_batch_policy_choices(const nvshmemx_put_request_t *reqs,
     for (int i = 0; i < nreq; ++i) {
         if (pol.kind == 0) {
             out[i] = default_choice(reqs[i].nbytes);
+            continue;
+        }
+        if (pol.kind == 3) {
+            out[i].pieces = 1;
+            out[i].vec_bytes = 16;
+            out[i].mode = 2;
             continue;
         }
         size_t pieces = 1;

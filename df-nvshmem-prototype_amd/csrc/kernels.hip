@@ -133,7 +133,8 @@ static size_t tiled_grid(const void *dst, const void *src, size_t nbytes, const 
     const size_t streams = (size_t)c.streams;
     size_t g = (tiles + streams - 1) / streams * streams;
     if (g < 1) g = 1;
-    if (g > ((size_t)1 << 30)) g = (size_t)1 << 30;
+    const size_t most = 0xFFFFFFFFu / (size_t)c.block; /* the grid's threads within 32 bits */
+    if (g > most) g = most / streams * streams;
     return g;
 }
 

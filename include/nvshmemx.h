@@ -134,7 +134,9 @@ typedef struct nvshmemx_put_request {
 typedef struct nvshmemx_batch_choice {
     int pieces;    /* 1..256 parts; a part is one workgroup's work (mode 1) or four wavefronts' (mode 0) */
     int vec_bytes; /* widest access: 4, 8 or 16 */
-    int mode;      /* 0: wavefront pieces, 1: workgroup pieces */
+    int mode;      /* 0: wavefront pieces, 1: workgroup pieces, 2: tiles of 512 accesses of the widest width
+                      <= vec_bytes the alignment allows, one workgroup of 256 threads a tile, all of a
+                      plan's tiles in source order in a launch of their own (pieces is ignored) */
 } nvshmemx_batch_choice_t;
 
 #define NVSHMEMX_BATCH_SHARED 0 /* every workgroup of the launch serves every request */

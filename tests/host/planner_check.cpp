@@ -15,8 +15,15 @@
  *  - longest-processing-time lists within Graham's bound (max load <= mean
  *    load + largest piece); source-order lists sorted and in order across the
  *    workgroups of a set;
+ *  - tiles (choice mode 2) after the lists, sorted by source address, exactly
+ *    the tiled requests' bytes: each at most 512 accesses of the widest width
+ *    (<= the choice's) the relative alignment allows, source and target
+ *    aligned to it, whole accesses only; byte tiles only before the first and
+ *    after the last whole access of their request;
  *  - total bytes.
- * Exit status 0 when every plan passes.
+ * Planted mistakes (a dropped piece, a piece off its boundary, a dropped
+ * tile, tiles out of source order, a tile narrower than its alignment
+ * allows) must each fail.  Exit status 0 when every plan passes.
  */
 #include <stdarg.h>
 #include <stdio.h>
@@ -64,7 +71,8 @@ static void check_plan(const std::vector<resolved_req> &reqs, const std::vector<
     CHECK(L.wgs_per_set >= 1, "%s", tag);
     CHECK(L.wg_off.size() == G + 1 && L.wave_off.size() == W + 1, "%s sizes", tag);
     if (L.wg_off.size() != G + 1 || L.wave_off.size() != W + 1) return;
-    CHECK(L.wg_off[0] == 0 && L.wg_off[G] == L.wave_off[0] && L.wave_off[W] == L.pieces.size(), "%s ends", tag);
+    CHECK(L.wg_off[0] == 0 && L.wg_off[G] == L.wave_off[0] && L.wave_off[W] == L.tile0 &&
+              L.tile0 + L.ntiles == L.pieces.size(), "%s ends", tag);
     for (size_t i = 0; i < G; ++i) CHECK(L.wg_off[i] <= L.wg_off[i + 1], "%s wg_off %zu", tag, i);
     for (size_t i = 0; i < W; ++i) CHECK(L.wave_off[i] <= L.wave_off[i + 1], "%s wave_off %zu", tag, i);
     /* each piece back to its request (sources are disjoint) */
@@ -76,7 +84,7 @@ static void check_plan(const std::vector<resolved_req> &reqs, const std::vector<
     }
     CHECK(L.total_bytes == total, "%s total %zu vs %zu", tag, L.total_bytes, total);
     std::vector<std::vector<std::pair<size_t, size_t>>> cover(reqs.size());
-    std::vector<size_t> worker_of(L.pieces.size());
+    std::vector<size_t> worker_of(L.pieces.size(), 0);
     for (size_t w = 0; w < G; ++w)
         for (unsigned j = L.wg_off[w]; j < L.wg_off[w + 1]; ++j) worker_of[j] = w;
     for (size_t w = 0; w < W; ++w)
@@ -91,6 +99,29 @@ static void check_plan(const std::vector<resolved_req> &reqs, const std::vector<
         const size_t off = (size_t)(p.src - r.src);
         CHECK(off + p.nbytes <= r.nbytes, "%s piece %zu past its request", tag, j);
         CHECK((size_t)(p.dst - r.dst) == off, "%s piece %zu target offset", tag, j);
+        if (j >= L.tile0) { /* a tile */
+            CHECK(ch[it->second].mode == 2, "%s tile %zu of a request that is not tiled", tag, j);
+            CHECK(j == L.tile0 || L.pieces[j - 1].src <= p.src, "%s tile %zu out of source order", tag, j);
+            const uintptr_t rel = (uintptr_t)r.src ^ (uintptr_t)r.dst;
+            unsigned w = vec_of(ch[it->second].vec_bytes);
+            while (w > 1 && (rel & (w - 1))) w >>= 1;
+            CHECK(p.nbytes > 0, "%s empty tile %zu", tag, j);
+            if (p.vec > 1 || w == 1) { /* a body tile (all of them byte tiles when the widest width is 1) */
+                CHECK(p.vec == w, "%s tile %zu width %u, widest allowed %u", tag, j, p.vec, w);
+                CHECK((uintptr_t)p.src % w == 0 && (uintptr_t)p.dst % w == 0 && p.nbytes % w == 0 &&
+                          p.nbytes <= (size_t)kTileAccesses * w, "%s tile %zu shape", tag, j);
+            } else {
+                CHECK(p.vec == 1 && p.nbytes < w, "%s byte tile %zu of %llu bytes, width %u", tag, j, p.nbytes, w);
+                const size_t head = ((size_t)w - ((uintptr_t)r.dst & (w - 1))) & (w - 1);
+                /* the head (up to the target's first width boundary) or the tail (after the last whole access) */
+                const bool is_head = off == 0 && head > 0 && p.nbytes == std::min(head, r.nbytes);
+                const bool is_tail = off + p.nbytes == r.nbytes && (r.nbytes - std::min(head, r.nbytes)) % w == p.nbytes;
+                CHECK(is_head || is_tail, "%s byte tile %zu at %zu of %llu bytes is neither head nor tail", tag, j, off,
+                      p.nbytes);
+            }
+            cover[it->second].push_back({off, (size_t)p.nbytes});
+            continue;
+        }
         CHECK(off % kPieceQuantum == 0, "%s piece %zu offset %zu", tag, j, off);
         CHECK(p.nbytes > 0, "%s empty piece %zu", tag, j);
         CHECK(p.vec == vec_of(ch[it->second].vec_bytes), "%s piece %zu width", tag, j);
@@ -113,7 +144,7 @@ static void check_plan(const std::vector<resolved_req> &reqs, const std::vector<
         size_t most = 1;
         for (int s = 0; s < nsets; ++s) {
             size_t nwg = 0, nwave = 0;
-            for (size_t j = 0; j < L.pieces.size(); ++j)
+            for (size_t j = 0; j < std::min(L.tile0, L.pieces.size()); ++j)
                 if ((int)(worker_of[j] / (size_t)L.wgs_per_set) == s) (j < L.wg_off[G] ? nwg : nwave) += 1;
             most = std::max(most, nwg + (nwave + (size_t)waves - 1) / (size_t)waves);
         }
@@ -178,7 +209,7 @@ int main(int argc, char **argv) {
                 else {
                     ch[(size_t)i].pieces = 1 + (int)(rng() % 300);
                     ch[(size_t)i].vec_bytes = 4 << (rng() % 3);
-                    ch[(size_t)i].mode = (int)(rng() % 2);
+                    ch[(size_t)i].mode = (int)(rng() % 3);
                 }
             }
             if (source == 2) {
@@ -212,6 +243,7 @@ int main(int argc, char **argv) {
         dropped.pieces.pop_back();
         for (auto &o : dropped.wave_off) o = std::min<unsigned>(o, (unsigned)dropped.pieces.size());
         for (auto &o : dropped.wg_off) o = std::min<unsigned>(o, (unsigned)dropped.pieces.size());
+        dropped.tile0 = dropped.pieces.size(); /* the plan has no tiles */
         failures = 0;
         check_plan(reqs, ch, 1, 0, 256, ASSIGN_LPT, dropped, "planted: dropped piece");
         const bool caught_drop = failures > 0;
@@ -222,9 +254,33 @@ int main(int argc, char **argv) {
         failures = 0;
         check_plan(reqs, ch, 1, 0, 256, ASSIGN_LPT, shifted, "planted: shifted piece");
         const bool caught_shift = failures > 0;
+        /* a tile plan: one tile dropped, two tiles out of source order, a
+         * tile narrower than the alignment allows */
+        std::vector<resolved_req> treqs = {resolved_req{addr((1ull << 32) + 16), addr((1ull << 40) + 16), (size_t)1 << 20, 0},
+                                           resolved_req{addr((1ull << 33) + 3), addr((1ull << 41) + 3), 70000, 0}};
+        std::vector<nvshmemx_batch_choice_t> tch(2, nvshmemx_batch_choice_t{1, 16, 2});
+        plan_layout T;
+        build_layout(treqs, tch.data(), nullptr, 1, 0, 256, ASSIGN_SOURCE_ORDER, T);
+        plan_layout tdrop = T;
+        tdrop.pieces.pop_back();
+        tdrop.ntiles -= 1;
+        failures = 0;
+        check_plan(treqs, tch, 1, 0, 256, ASSIGN_SOURCE_ORDER, tdrop, "planted: dropped tile");
+        const bool caught_tdrop = failures > 0;
+        plan_layout tswap = T;
+        std::swap(tswap.pieces[tswap.tile0], tswap.pieces[tswap.tile0 + 1]);
+        failures = 0;
+        check_plan(treqs, tch, 1, 0, 256, ASSIGN_SOURCE_ORDER, tswap, "planted: tiles out of order");
+        const bool caught_tswap = failures > 0;
+        plan_layout tnarrow = T;
+        tnarrow.pieces[tnarrow.tile0].vec = 8;
+        failures = 0;
+        check_plan(treqs, tch, 1, 0, 256, ASSIGN_SOURCE_ORDER, tnarrow, "planted: narrow tile");
+        const bool caught_tnarrow = failures > 0;
         report = true;
-        printf("planted mistakes caught: %s\n", caught_drop && caught_shift ? "yes" : "NO");
-        failures = real_failures + (caught_drop && caught_shift ? 0 : 1);
+        const bool all_caught = caught_drop && caught_shift && caught_tdrop && caught_tswap && caught_tnarrow;
+        printf("planted mistakes caught: %s\n", all_caught ? "yes" : "NO");
+        failures = real_failures + (all_caught ? 0 : 1);
     }
     printf("%s: %d plans, %d failed checks\n", failures ? "FAILED" : "ok", plans, failures);
     return failures ? 1 : 0;

@@ -105,8 +105,9 @@ def test_batch_policy_choices(lib):
     """the rule's choices equal nvshmemx_batch_default_choice; a uniform
     policy cuts a request into ceil(bytes / granularity) pieces (a wavefront
     policy counts four wavefront pieces per piece), 1..256, 16-byte accesses;
-    the last policy is one workgroup piece per request"""
-    assert lib.nvshmemx_batch_policy_count() == 12
+    policy 11 is one workgroup piece per request, policy 12 every request as
+    tiles (mode 2)"""
+    assert lib.nvshmemx_batch_policy_count() == 13
     sizes = [1, 4095, 4096, 65536, 65537, 1 << 20, (1 << 22) - 4, 1 << 30]
     arr, n = _reqs(sizes), len(sizes)
     out = (na.BatchChoice * n)()
@@ -124,3 +125,5 @@ def test_batch_policy_choices(lib):
             assert (out[i].pieces, out[i].vec_bytes, out[i].mode) == (max(1, min(256, -(-s // per))), 16, mode), (p, s)
     lib.nvshmemx_batch_policy_choices(arr, n, 11, None, out)
     assert all((out[i].pieces, out[i].mode) == (1, 1) for i in range(n))
+    lib.nvshmemx_batch_policy_choices(arr, n, 12, None, out)
+    assert all((out[i].vec_bytes, out[i].mode) == (16, 2) for i in range(n))

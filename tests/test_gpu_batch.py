@@ -67,7 +67,7 @@ def _run_plan(lib, hip, arr, n, choices=None, launch=None):
 
 
 @pytest.mark.parametrize("variant", ["default", "random_choices", "owner", "per_call", "sc1_stores",
-                                     "sc1_stores_nt_loads"])
+                                     "sc1_stores_nt_loads", "tiles", "mixed_with_tiles"])
 def test_batch_matches_oracle(setup, hip, oracle, variant):
     lib, base, images = setup
     reqs = na.gen_requests(lib, 8, 512, NPES, SRC, DST)
@@ -90,6 +90,18 @@ def test_batch_matches_oracle(setup, hip, oracle, variant):
             launch = na.BatchLaunch(300, 512, 0, na.BATCH_SHARED)
         if variant == "owner":
             launch = na.BatchLaunch(0, 256, 3, na.BATCH_OWNER)
+        if variant == "tiles":  # every request cut into tiles, one workgroup a tile, in source order
+            choices = (na.BatchChoice * n)()
+            for i in range(n):
+                choices[i].pieces, choices[i].vec_bytes, choices[i].mode = 1, 16, 2
+        if variant == "mixed_with_tiles":  # tiles beside wavefront and workgroup pieces, random widths
+            rng = np.random.default_rng(13)
+            choices = (na.BatchChoice * n)()
+            for i in range(n):
+                choices[i].pieces = int(rng.integers(1, 257))
+                choices[i].vec_bytes = int(rng.choice([4, 8, 16]))
+                choices[i].mode = int(rng.integers(0, 3))
+            launch = na.BatchLaunch(0, 0, 2, na.BATCH_OWNER)
         if variant.startswith("sc1_stores"):  # stores written through the L2 (bit 2), random access widths
             rng = np.random.default_rng(9)
             choices = (na.BatchChoice * n)()
@@ -104,7 +116,8 @@ def test_batch_matches_oracle(setup, hip, oracle, variant):
     _check(lib, hip, base, want)
 
 
-def test_batch_graph_replays(setup, hip, oracle):
+@pytest.mark.parametrize("kind", ["pieces", "tiles"])
+def test_batch_graph_replays(setup, hip, oracle, kind):
     """a captured graph of 3 fused steps, replayed 3 times with the
     destinations cleared in between: every replay redoes all the work (the
     queue counters reset themselves; the graph holds no memset node)"""
@@ -112,8 +125,13 @@ def test_batch_graph_replays(setup, hip, oracle):
     reqs = na.gen_requests(lib, 9, 384, NPES, SRC, DST)
     arr = na.make_requests(lib, reqs, base, 0, SRC)
     want = _expected(oracle, images, reqs)
+    choices = None
+    if kind == "tiles":
+        choices = (na.BatchChoice * len(arr))()
+        for i in range(len(arr)):
+            choices[i].pieces, choices[i].vec_bytes, choices[i].mode = 1, 16, 2
     plan = C.c_void_p()
-    lib.nvshmemx_batch_plan_create(arr, len(arr), None, None, C.byref(plan))
+    lib.nvshmemx_batch_plan_create(arr, len(arr), choices, None, C.byref(plan))
     lib.nvshmemx_batch_plan_capture(plan, 3)
     try:
         for _ in range(3):
@@ -132,7 +150,8 @@ def test_batch_graph_replays(setup, hip, oracle):
 
 
 def test_batch_unaligned_ragged(setup, hip, oracle):
-    """byte-granular offsets and lengths: every relative alignment class"""
+    """byte-granular offsets and lengths: every relative alignment class, as
+    pieces and as tiles"""
     lib, base, images = setup
     rng = np.random.default_rng(17)
     n = 300
@@ -152,8 +171,8 @@ def test_batch_unaligned_ragged(setup, hip, oracle):
     want = _expected(oracle, images, reqs)
     _reset_dst(lib, hip, base)
     choices = (na.BatchChoice * n)()
-    for i in range(n):
-        choices[i].pieces, choices[i].vec_bytes, choices[i].mode = int(rng.integers(1, 9)), 16, int(i % 2)
+    for i in range(n):  # wavefront pieces, workgroup pieces and tiles (head, body and tail tiles of every width)
+        choices[i].pieces, choices[i].vec_bytes, choices[i].mode = int(rng.integers(1, 9)), 16, int(i % 3)
     plan = _run_plan(lib, hip, arr, n, choices, None)
     lib.nvshmemx_batch_plan_destroy(plan)
     _check(lib, hip, base, want)

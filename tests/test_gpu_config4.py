@@ -10,7 +10,8 @@ application of the same requests (oracle_batch_apply) on host images of all
 are checked.  Nothing on the device takes part in the comparison.
 
 Paths: the fused plan (one launch), its captured hipGraph replayed, the
-owner discipline (each PE's workgroups run the puts it issued), the per-call
+owner discipline (each PE's workgroups run the puts it issued), every request
+cut into tiles (one workgroup a tile; launched and replayed), the per-call
 path (one grid put launch per request), and the persistent kernel in which
 each PE cuts its own puts into pieces on the device (library-allocated
 scratch)."""
@@ -76,7 +77,8 @@ def device_requests(hip, arr):
     return d
 
 
-@pytest.mark.parametrize("path", ["fused", "graph", "owner", "per_call", "persistent", "persistent_graph"])
+@pytest.mark.parametrize("path", ["fused", "graph", "owner", "per_call", "persistent", "persistent_graph", "tiles",
+                                  "tiles_graph"])
 def test_config4_full_size(full, hip, path):
     lib, base, arr, want = full
     n = len(arr)
@@ -117,10 +119,16 @@ def test_config4_full_size(full, hip, path):
         hip.sync()
     else:
         launch = na.BatchLaunch(8192, 0, 2, na.BATCH_OWNER) if path == "owner" else None
+        choices = None
+        if path.startswith("tiles"):  # every request cut into tiles
+            choices = (na.BatchChoice * n)()
+            for i in range(n):
+                choices[i].pieces, choices[i].vec_bytes, choices[i].mode = 1, 16, 2
         plan = C.c_void_p()
-        assert lib.nvshmemx_batch_plan_create(arr, n, None, C.byref(launch) if launch else None, C.byref(plan)) == 0
+        assert lib.nvshmemx_batch_plan_create(arr, n, choices, C.byref(launch) if launch else None,
+                                              C.byref(plan)) == 0
         try:
-            if path == "graph":
+            if path in ("graph", "tiles_graph"):
                 lib.nvshmemx_batch_plan_capture(plan, 2)
                 for _ in range(2):  # replays, the destinations cleared in between
                     _reset(lib, hip, base)

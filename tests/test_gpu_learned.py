@@ -20,7 +20,7 @@ The assertions are ones a box-to-box spread of tens of per cent cannot flip:
 How much faster than the rule it is goes into the benchmark's report, not into
 a pass/fail gate.
 
-The batch-level policy model (4 batch inputs -> 12 policies, the same
+The batch-level policy model (4 batch inputs -> 13 policies, the same
 kernels) is trained on batches of 8 to 256 puts that the library times itself
 under every policy (the exhaustive batch-level judge), then asked to pick the
 policy of held-out 16- and 128-put batches.  Asserted: its fit is 10x better
