@@ -99,8 +99,8 @@ def batch_worker(counter_pass=False):
     kernel planned on the device by the issuing PEs (library-allocated and
     static scratch) and the per-call path (one launch per put).  Prints one
     JSON line.  counter_pass: only five fused launches, five persistent
-    launches and one 1 GiB device-to-device copy (the scale of a counter
-    pass)."""
+    launches, five tile launches and one 1 GiB device-to-device copy (the
+    scale of a counter pass)."""
     import ctypes as C
     import numpy as np
     import nvshmem_amd as na
@@ -134,6 +134,12 @@ def batch_worker(counter_pass=False):
         hip.htod(d_reqs, raw)
         for _ in range(5):
             lib.nvshmemx_batch_put_persistent_on_stream(d_reqs, n, None)
+        tile_choices = (na.BatchChoice * n)()
+        for i in range(n):
+            tile_choices[i].pieces, tile_choices[i].vec_bytes, tile_choices[i].mode = 1, 16, 2
+        tiles = plan_of(None, tile_choices)
+        for _ in range(5):
+            lib.nvshmemx_batch_plan_launch(tiles, None)
         ref = hip.malloc(2 << 30)
         hip.check(hip.lib.hipMemcpyAsync(ref + (1 << 30), ref, 1 << 30, hip.D2D, None))
         hip.sync()
