@@ -107,6 +107,7 @@ void oracle_batch_apply(uint8_t **heaps, int nreq, const int *src_pe, const int 
 void oracle_jacobi_init(double *u, int G, uint64_t seed);
 void oracle_jacobi_step(const double *u, double *v, int G);
 int oracle_jacobi_run(double *u, double *v, int G, int steps);
+int oracle_jacobi_run_mt(double *u, double *v, int G, int steps, int threads);
 
 /* ---- cost model (oracle_mlp.c): the 4 -> 32 ReLU -> 32 ReLU -> C network's
  * forward pass and mean-squared-error gradient in double precision ---- */

@@ -8,6 +8,8 @@
  * Initial value of cell g = (x * G + y) * G + z: PCG32 (seed, stream 54)
  * words 2g and 2g+1 as (w0 << 32 | w1) >> 11, times 2^-53.
  */
+#define _POSIX_C_SOURCE 200809L
+#include <pthread.h>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -23,9 +25,10 @@ void oracle_jacobi_init(double *u, int G, uint64_t seed) {
     }
 }
 
-void oracle_jacobi_step(const double *u, double *v, int G) {
+/* the step for the planes x0 <= x < x1 */
+static void step_planes(const double *u, double *v, int G, int x0, int x1) {
     const size_t sx = (size_t)G * G, sy = (size_t)G;
-    for (int x = 0; x < G; ++x) {
+    for (int x = x0; x < x1; ++x) {
         const int xm = (x + G - 1) % G, xp = (x + 1) % G;
         for (int y = 0; y < G; ++y) {
             const int ym = (y + G - 1) % G, yp = (y + 1) % G;
@@ -42,11 +45,53 @@ void oracle_jacobi_step(const double *u, double *v, int G) {
     }
 }
 
+void oracle_jacobi_step(const double *u, double *v, int G) { step_planes(u, v, G, 0, G); }
+
 /* `steps` steps; returns 0 if the result is in u, 1 if in v */
 int oracle_jacobi_run(double *u, double *v, int G, int steps) {
     for (int t = 0; t < steps; ++t) {
         if (t & 1) oracle_jacobi_step(v, u, G);
         else oracle_jacobi_step(u, v, G);
+    }
+    return steps & 1;
+}
+
+/* The same run with the planes of every step split between `threads`
+ * threads (at most 64, at most G), joined before the next step starts.  Every
+ * cell is computed by the expression of oracle_jacobi_step from the same
+ * inputs, so the result is bit-identical to oracle_jacobi_run; only the wall
+ * time changes.  A slab whose thread cannot start is computed by the caller's
+ * thread.  Returns as oracle_jacobi_run. */
+struct mt_slab {
+    const double *u;
+    double *v;
+    int G, x0, x1;
+};
+
+static void *mt_slab_step(void *p) {
+    const struct mt_slab *a = (const struct mt_slab *)p;
+    step_planes(a->u, a->v, a->G, a->x0, a->x1);
+    return NULL;
+}
+
+int oracle_jacobi_run_mt(double *u, double *v, int G, int steps, int threads) {
+    if (threads > G) threads = G;
+    if (threads > 64) threads = 64;
+    if (threads < 1) threads = 1;
+    pthread_t tid[64];
+    int live[64];
+    struct mt_slab slab[64];
+    for (int t = 0; t < steps; ++t) {
+        const double *src = (t & 1) ? v : u;
+        double *dst = (t & 1) ? u : v;
+        for (int k = 0; k < threads; ++k) {
+            slab[k] = (struct mt_slab){src, dst, G, (int)((long)G * k / threads), (int)((long)G * (k + 1) / threads)};
+            live[k] = k > 0 && pthread_create(&tid[k], NULL, mt_slab_step, &slab[k]) == 0;
+            if (k > 0 && !live[k]) mt_slab_step(&slab[k]);
+        }
+        mt_slab_step(&slab[0]);
+        for (int k = 1; k < threads; ++k)
+            if (live[k]) pthread_join(tid[k], NULL);
     }
     return steps & 1;
 }

@@ -150,6 +150,17 @@ class Oracle:
         which = run(u.ctypes.data, v.ctypes.data, G, steps)
         return (v if which else u).reshape(G, G, G)
 
+    def jacobi_mt(self, G, steps, seed, threads):
+        """the same field, every step's planes split between `threads` threads"""
+        init, run = self.lib.oracle_jacobi_init, self.lib.oracle_jacobi_run_mt
+        init.restype, init.argtypes = None, [_P, _I, _U64]
+        run.restype, run.argtypes = _I, [_P, _P, _I, _I, _I]
+        u = np.zeros(G ** 3, np.float64)
+        v = np.zeros(G ** 3, np.float64)
+        init(u.ctypes.data, G, seed)
+        which = run(u.ctypes.data, v.ctypes.data, G, steps, threads)
+        return (v if which else u).reshape(G, G, G)
+
     def iput(self, heaps, target_off, tst, source, sst, nelems, elem_bytes, pe):
         f = self.lib.oracle_iput
         f.restype, f.argtypes = None, [_P, _SZ, C.c_ssize_t, _P, C.c_ssize_t, _SZ, _SZ, _I]

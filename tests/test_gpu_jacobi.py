@@ -2,10 +2,10 @@
 a periodic 2x2x2 grid, bit-exact against the oracle's host replay of the whole
 undecomposed domain.  Local mode (all 8 PEs in one process, kernel boundaries
 as the barrier) at 16^3 per PE for 100 steps and at the full 256^3 per PE for
-3 steps; process mode (8 processes, nvshmem_barrier_all_offload per step) at
-16^3 for 40 steps.  A step is normally one fused kernel (halo read from the
-receive slots, faces written into the neighbours' slots); the separate
-pack / unpack / stencil kernels are checked as well."""
+3 and 100 steps; process mode (8 processes, nvshmem_barrier_all_offload per
+step) at 16^3 for 40 steps and at 256^3 for 100.  A step is normally one fused
+kernel (halo read from the receive slots, faces written into the neighbours'
+slots); the separate pack / unpack / stencil kernels are checked as well."""
 import json
 import os
 import subprocess
@@ -55,6 +55,20 @@ def test_jacobi_local_bitexact(local_pes, hip, oracle, n, steps, chunk):
                                  f"{blk[tuple(bad[0])]!r}")
 
 
+def test_jacobi_full_domain_100_steps(local_pes, hip, oracle):
+    """the benchmarked workload run to the end: 8 PEs x 256^3 (a 512^3 global
+    domain), 100 fused steps at the library's default chunk of planes, every
+    cell bit-exact against the host replay of the whole domain (the replay's
+    planes split over up to 16 host threads, same arithmetic per cell)"""
+    n, steps = 256, 100
+    lib = local_pes.start(8, heap="320M")
+    out = _run_local(lib, hip, n, steps, None)
+    want = oracle.jacobi_mt(2 * n, steps, SEED, max(1, min(16, os.cpu_count() or 1)))
+    for pe in range(8):
+        got = na.jacobi_interior(hip, lib.nvshmem_ptr(out, pe), n, lib)
+        np.testing.assert_array_equal(got.view(np.uint64), na.jacobi_block(want, pe, n).view(np.uint64))
+
+
 @pytest.mark.parametrize("n,steps,chunk", [(128, 10, None), (128, 5, 2), (128, 4, 5), (128, 3, 127)])
 def test_jacobi_chunks_bitexact(local_pes, hip, oracle, n, steps, chunk):
     """whole tiles in j and k with chunks of planes from 2 to 127 (a ragged
@@ -94,16 +108,19 @@ def test_jacobi_separate_kernels_bitexact(local_pes, hip, oracle, n, pack):
         np.testing.assert_array_equal(got.view(np.uint64), na.jacobi_block(want, pe, n).view(np.uint64))
 
 
-@pytest.mark.parametrize("n,steps,how", [(16, 40, "fused"), (128, 6, "fused"), (16, 12, "api"), (48, 5, "api")])
+@pytest.mark.parametrize("n,steps,how", [(16, 40, "fused"), (128, 6, "fused"), (16, 12, "api"), (48, 5, "api"),
+                                         (256, 100, "fused")])
 def test_jacobi_processes_bitexact(tmp_path, oracle, n, steps, how):
     """8 processes, one PE each: the fused step (faces stored straight into
     the neighbours' receive slots) and the unfused step whose faces go through
     the library's contiguous and strided device puts, the offload barrier
-    between"""
+    between; the last case is the benchmarked workload, 256^3 per PE for 100
+    steps"""
     name = f"nvshmem_jacobi_{os.getpid()}_{uuid.uuid4().hex[:8]}"
+    heap = "16M" if n <= 16 else ("64M" if n <= 128 else "320M")
     procs = [subprocess.Popen([sys.executable, str(HERE / "pe_worker.py"), "jacobi", str(n), str(steps),
                                str(tmp_path), how],
-                              env=na.shm_job_env(r, 8, name, {"NVSHMEM_HEAP_SIZE": "16M" if n <= 16 else "64M",
+                              env=na.shm_job_env(r, 8, name, {"NVSHMEM_HEAP_SIZE": heap,
                                                               "NVSHMEM_WAIT_TIMEOUT_MS": "5000",
                                                               "NVSHMEM_BOOTSTRAP_TIMEOUT_S": "60"}),
                               stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for r in range(8)]
@@ -119,7 +136,7 @@ def test_jacobi_processes_bitexact(tmp_path, oracle, n, steps, how):
         shm = Path("/dev/shm") / name
         if shm.exists():
             shm.unlink()
-    want = oracle.jacobi(2 * n, steps, SEED)
+    want = oracle.jacobi_mt(2 * n, steps, SEED, max(1, min(16, os.cpu_count() or 1)))
     for pe in range(8):
         got = np.load(tmp_path / f"jacobi_pe{pe}.npy")
         np.testing.assert_array_equal(got.view(np.uint64), na.jacobi_block(want, pe, n).view(np.uint64))

@@ -437,6 +437,20 @@ def test_strided_and_2d_closed_forms(oracle):
         assert not heaps[1].any()
 
 
+def test_jacobi_threaded_run_equals_the_serial_run(oracle):
+    """the threaded replay gives the serial run's field bit for bit: slabs of
+    unequal size, more threads than planes, one thread, zero and odd step
+    counts (which buffer holds the result)"""
+    for G, steps, threads in [(24, 5, 3), (24, 4, 7), (9, 3, 16), (5, 2, 8), (12, 1, 1), (12, 0, 4), (16, 6, 5)]:
+        want = oracle.jacobi(G, steps, 42)
+        got = oracle.jacobi_mt(G, steps, 42, threads)
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), (G, steps, threads)
+    # the serial run it is checked against moves: a threaded run that skipped a
+    # step would equal neither neighbour
+    a, b = oracle.jacobi_mt(16, 5, 42, 4), oracle.jacobi(16, 4, 42)
+    assert not np.array_equal(a, b)
+
+
 # ---------------------------------------------------------------- coverage
 
 # Each function oracle.h declares, and the CPU test that pins it.
@@ -472,6 +486,7 @@ PINNED_BY = {
     "oracle_jacobi_init": ("test_oracle", "test_jacobi_run_conserves_sum_and_stays_in_range"),
     "oracle_jacobi_step": ("test_oracle", "test_jacobi_plane_waves_are_eigenvectors"),
     "oracle_jacobi_run": ("test_oracle", "test_jacobi_run_conserves_sum_and_stays_in_range"),
+    "oracle_jacobi_run_mt": ("test_oracle", "test_jacobi_threaded_run_equals_the_serial_run"),
     "oracle_batch_generate": ("test_batch_cpu", "test_generator_matches_oracle"),
     "oracle_batch_apply": ("test_batch_cpu", "test_batch_apply_matches_sequential_puts"),
     "oracle_mlp_num_params": ("test_oracle", "test_mlp_forward_closed_forms"),
