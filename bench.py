@@ -253,7 +253,7 @@ def learned_worker(n_batches, train_steps):
     X, Y = [], []
     for nreq in (8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192):
         for hi in (16, 18, 20, 22):
-            for sd in range(6):
+            for sd in range(16):
                 arr = batch_of(20000 + 1000 * nreq + 10 * hi + sd, nreq, hi)
                 if arr is None:
                     continue
