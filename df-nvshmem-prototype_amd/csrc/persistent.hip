@@ -19,16 +19,23 @@
  *    starts in; then publishes its queue with a release flag.  Leaders agree
  *    on where each PE's entries and chunks go by counting every PE's requests
  *    in the same pass, so no leader waits for another;
- *  - once every queue is published, the workgroups claim 128 KiB chunks of
- *    the queues laid end to end in PE order, one atomic fetch-add on one
- *    counter each (the next claim is issued before the current chunk is
- *    copied), so the workgroups running at once copy neighbouring chunks of
- *    the same few PEs' queues.  Inside a chunk, a request segment of 16 KiB
- *    or more is copied by the whole workgroup, shorter ones by one wavefront
- *    each, in turn, with 16-byte accesses where the alignment allows, eight
- *    loads in flight per lane and non-temporal stores;
- *  - the last workgroup to finish clears the claim counter and the flags, so
- *    the next launch (or graph replay) starts from zero with no memset.
+ *  - a PE with at most 64 requests and 128 KiB of payload gets no queue: its
+ *    leader publishes an empty one and copies the requests itself, so a
+ *    small batch is moved by the PEs that issued it, with no chunk to claim;
+ *  - once every queue is published, the workgroups copy 128 KiB chunks of
+ *    the queues laid end to end in PE order: workgroup b takes chunk b, then
+ *    claims more with one atomic fetch-add on one counter each (the next
+ *    claim is issued before the current chunk is copied), so the workgroups
+ *    running at once copy neighbouring chunks of the same few PEs' queues.
+ *    Inside a chunk, a request segment of 16 KiB or more is copied by the
+ *    whole workgroup, shorter ones by one wavefront each, in turn, with
+ *    16-byte accesses where the alignment allows, eight loads in flight per
+ *    lane and non-temporal stores;
+ *  - nothing is cleared between launches: each workgroup takes a launch
+ *    ticket, flags carry the launch's number, and the claim counters
+ *    alternate between launches, each launch zeroing the next one's; so the
+ *    next launch (or graph replay) starts clean with no memset and no
+ *    workgroup waits for the others to finish.
  *
  * Contention and coherence: each PE's ready flag and queue record sit in a
  * 4 KiB slot of their own.  Readers never take an agent-scope acquire (which
@@ -55,9 +62,13 @@
  * each; wavefront 0 now does that with shuffles (4 and 13-17 us), and a chunk
  * reads its entries 64 at a time instead of one dependent pair of loads per
  * entry: the step went from 0.528 to 0.512, a batch of 16 requests of 64 B
- * from 23.5 to 17.1 us.  The host-planned fused launch of batch.hip
- * (source-ordered pieces, no device-side planning) takes 0.509 for the same
- * step and about 3.5 us for a tiny batch.
+ * from 23.5 to 17.1 us.  Launch tickets instead of a last-workgroup
+ * clean-up, a first chunk taken without an atomic, and leaders copying
+ * small PEs' requests themselves then took the step to 0.491 and 16
+ * requests of 64 B to 10.4 us (128 such requests 10.6, 1024 12.1, 4096
+ * 38.9).  The host-planned fused launch of batch.hip (source-ordered pieces,
+ * no device-side planning) takes 0.509 for the same step and about 3.5 us
+ * for a tiny batch.
  *
  * Scratch (per launch, device memory): the control slots, the chunk table
  * (65,536 entries; the chunk grows from 128 KiB so that a batch of any size
@@ -93,6 +104,9 @@ constexpr unsigned kTableChunks = 65536;                   /* chunk-table entrie
 constexpr unsigned kLeadCache = 4096;                      /* requests a leader keeps in LDS between passes */
 constexpr int kMaxReqsNoalloc = 65536;                     /* requests per call without allocation */
 constexpr int kPPerCU = 1;                                                /* workgroups per CU (see the launch shape above) */
+constexpr unsigned kSelfCount = 64;      /* a PE with at most this many requests ... */
+constexpr size_t kSelfBytes = 128 << 10; /* ... and payload bytes is copied by its own leader */
+static_assert(kSelfCount <= 64, "a leader reads its own requests one per lane");
 constexpr int kMaxDevices = 64;
 
 struct pinfo {            /* PE p's queue, published by its leader */
@@ -108,13 +122,14 @@ struct pentry {
     unsigned vstart; /* start in the PE's virtual stream, granules */
 };
 
-/* scratch layout: per-PE 4 KiB slots (ready flag, queue record), the done
- * counter and the claim counter, the chunk table, the entries */
+/* scratch layout: per-PE 4 KiB slots (ready flag, queue record), the
+ * launch ticket counter and the two claim counters, the chunk table, the
+ * entries */
 constexpr size_t kSlot = 4096;
 constexpr size_t kReadyOff = 128, kInfoOff = 256;
-constexpr size_t kDoneOff = kSlot * NVSHMEMI_MAX_PES;
-constexpr size_t kClaimOff = kDoneOff + 128;
-constexpr size_t kCtlBytes = kDoneOff + 256; /* zero between launches */
+constexpr size_t kTicketOff = kSlot * NVSHMEMI_MAX_PES;
+constexpr size_t kClaimOff = kTicketOff + 128; /* claim counter of even launches; odd ones 128 B further */
+constexpr size_t kCtlBytes = kTicketOff + 384; /* zeroed when the scratch is set up */
 constexpr size_t kTableOff = kCtlBytes;
 constexpr size_t kEntryOff = kTableOff + sizeof(unsigned) * kTableChunks;
 constexpr size_t scratch_bytes(size_t nreq) { return kEntryOff + sizeof(pentry) * (nreq ? nreq : 1); }
@@ -128,8 +143,8 @@ struct pscr {
     unsigned char *base;
     __device__ __forceinline__ unsigned *ready(int q) const { return (unsigned *)(base + q * kSlot + kReadyOff); }
     __device__ __forceinline__ pinfo *info(int q) const { return (pinfo *)(base + q * kSlot + kInfoOff); }
-    __device__ __forceinline__ unsigned *done() const { return (unsigned *)(base + kDoneOff); }
-    __device__ __forceinline__ unsigned *claim() const { return (unsigned *)(base + kClaimOff); }
+    __device__ __forceinline__ unsigned long long *ticket() const { return (unsigned long long *)(base + kTicketOff); }
+    __device__ __forceinline__ unsigned *claim(unsigned parity) const { return (unsigned *)(base + kClaimOff + 128 * parity); }
     __device__ __forceinline__ unsigned *table() const { return (unsigned *)(base + kTableOff); }
     __device__ __forceinline__ pentry *entries() const { return (pentry *)(base + kEntryOff); }
 };
@@ -152,16 +167,17 @@ __device__ __forceinline__ pentry coherent_entry(const pentry *p) {
 }
 
 /* poll a ready flag with relaxed loads (back to back, then the sleep
- * ladder) until it is set or the wait budget runs out; no acquire: what the
- * flag publishes is read with coherent loads.  Returns the flag. */
-__device__ __forceinline__ unsigned wait_ready(const unsigned *flag) {
-    unsigned v = load_relaxed(flag);
-    for (int i = 0; i < NVSHMEMI_SPIN_POLLS && !v; ++i) v = load_relaxed(flag);
-    if (!v) {
+ * ladder) until it holds this launch's tag or the wait budget runs out; no
+ * acquire: what the flag publishes is read with coherent loads.  Returns
+ * whether the tag was seen. */
+__device__ __forceinline__ bool wait_tag(const unsigned *flag, unsigned tag) {
+    bool seen = load_relaxed(flag) == tag;
+    for (int i = 0; i < NVSHMEMI_SPIN_POLLS && !seen; ++i) seen = load_relaxed(flag) == tag;
+    if (!seen) {
         const uint64_t budget = nvshmemi_dstate.wait_timeout_ticks;
         const uint64_t t0 = wall_clock64();
         sleep_ladder pace;
-        while (!(v = load_relaxed(flag))) {
+        while (!(seen = load_relaxed(flag) == tag)) {
             pace.pause();
             if ((uint64_t)(wall_clock64() - t0) > budget) {
                 set_error(NVSHMEMI_ERR_WAIT_TIMEOUT);
@@ -170,7 +186,7 @@ __device__ __forceinline__ unsigned wait_ready(const unsigned *flag) {
         }
     }
     __atomic_signal_fence(__ATOMIC_SEQ_CST); /* no later load moves above the poll */
-    return v;
+    return seen;
 }
 
 __device__ __forceinline__ int req_issuer(const nvshmemx_put_request_t &r, unsigned i, int P) {
@@ -225,18 +241,36 @@ __device__ __forceinline__ void block_scan2(unsigned a, unsigned b, unsigned &xa
     __syncthreads();
 }
 
-/* leader of PE p: build and publish the PE's queue */
-__device__ void lead(const nvshmemx_put_request_t *__restrict__ reqs, unsigned nreq, int P, int p, const pscr &S) {
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, unsigned j) {
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, j), hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), j);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+/* a PE whose requests are few and small enough is served by its own leader:
+ * it gets no queue, so the leaders and workers must all decide alike, from
+ * the per-PE counts every leader takes */
+__device__ __forceinline__ bool leader_served(unsigned cnt, unsigned long long bytes) {
+    return cnt <= kSelfCount && bytes <= kSelfBytes;
+}
+
+/* leader of PE p: build and publish the PE's queue, or, for a PE the leader
+ * serves itself, publish an empty queue and copy the PE's requests.
+ * `ticket` (the workgroup's launch ticket) is read by thread 0 only, when it
+ * publishes, so the ticket's atomic completes while the leader works. */
+__device__ void lead(const nvshmemx_put_request_t *__restrict__ reqs, unsigned nreq, int P, int p, const pscr &S,
+                     unsigned long long ticket) {
     /* issuer (high 7 bits) and granules (low 25 bits) of the first requests,
      * kept from the first pass so the second reads no request again */
     __shared__ unsigned s_req[kLeadCache];
     __shared__ unsigned s_cnt[NVSHMEMI_MAX_PES];
-    __shared__ unsigned long long s_units[NVSHMEMI_MAX_PES];
+    __shared__ unsigned long long s_units[NVSHMEMI_MAX_PES], s_bytes[NVSHMEMI_MAX_PES];
     __shared__ unsigned s_w[2][kPWaves];
+    __shared__ unsigned s_own[kSelfCount]; /* a self-served PE's requests, in request order */
     __shared__ pinfo s_me;
+    __shared__ int s_self;
     unsigned *table = S.table();
     pentry *entries = S.entries();
-    for (int q = threadIdx.x; q < P; q += kPBlock) s_cnt[q] = 0, s_units[q] = 0;
+    for (int q = threadIdx.x; q < P; q += kPBlock) s_cnt[q] = 0, s_units[q] = 0, s_bytes[q] = 0;
     __syncthreads();
     /* a PE's entries are its requests that move bytes: zero-byte and invalid
      * requests get none, so no chunk can reach them */
@@ -247,13 +281,16 @@ __device__ void lead(const nvshmemx_put_request_t *__restrict__ reqs, unsigned n
         if (u) {
             atomicAdd(&s_cnt[q], 1u);
             atomicAdd(&s_units[q], (unsigned long long)u);
+            atomicAdd(&s_bytes[q], (unsigned long long)r.nbytes);
         }
         if (i < kLeadCache) s_req[i] = u < (1u << 25) ? ((unsigned)q << 25) | u : 0xFFFFFFFFu;
     }
     __syncthreads();
-    if (threadIdx.x < 64) { /* wavefront 0, two PEs per lane (P <= 128) */
+    if (threadIdx.x < 64) { /* wavefront 0, two PEs per lane (P <= 128); self-served PEs count for nothing */
         const unsigned lane = threadIdx.x, qa = lane, qb = lane + 64;
-        const unsigned long long ua = qa < (unsigned)P ? s_units[qa] : 0ull, ub = qb < (unsigned)P ? s_units[qb] : 0ull;
+        const bool ina = qa < (unsigned)P && !leader_served(s_cnt[qa], s_bytes[qa]);
+        const bool inb = qb < (unsigned)P && !leader_served(s_cnt[qb], s_bytes[qb]);
+        const unsigned long long ua = ina ? s_units[qa] : 0ull, ub = inb ? s_units[qb] : 0ull;
         unsigned long long total = ua + ub;
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) total += __shfl_xor(total, d, 64);
@@ -261,20 +298,23 @@ __device__ void lead(const nvshmemx_put_request_t *__restrict__ reqs, unsigned n
         const unsigned long long need = (total + (kTableChunks - P) - 1) / (kTableChunks - P);
         const unsigned cu = (unsigned)(need > kChunkUnitsMin ? need : kChunkUnitsMin);
         /* entries and chunks of the PEs before p */
-        unsigned e0 = (qa < (unsigned)p ? s_cnt[qa] : 0u) + (qb < (unsigned)p ? s_cnt[qb] : 0u);
+        unsigned e0 = (qa < (unsigned)p && ina ? s_cnt[qa] : 0u) + (qb < (unsigned)p && inb ? s_cnt[qb] : 0u);
         unsigned c0 = (qa < (unsigned)p ? (unsigned)((ua + cu - 1) / cu) : 0u) +
                       (qb < (unsigned)p ? (unsigned)((ub + cu - 1) / cu) : 0u);
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) e0 += __shfl_xor(e0, d, 64), c0 += __shfl_xor(c0, d, 64);
         if (lane == 0) {
+            const bool self = leader_served(s_cnt[p], s_bytes[p]);
+            s_self = self;
             s_me.entry0 = e0;
-            s_me.nown = s_cnt[p];
+            s_me.nown = self ? 0u : s_cnt[p];
             s_me.chunk0 = c0;
-            s_me.nchunks = (unsigned)((s_units[p] + cu - 1) / cu);
+            s_me.nchunks = self ? 0u : (unsigned)((s_units[p] + cu - 1) / cu);
             s_me.chunk_units = cu;
         }
     }
     __syncthreads();
+    const bool self = s_self;
     const unsigned cu = s_me.chunk_units, e0 = s_me.entry0, c0 = s_me.chunk0;
     unsigned rc = 0, ru = 0; /* own entries and granules of earlier tiles */
     for (unsigned base = 0; base < nreq; base += kPBlock) {
@@ -293,10 +333,14 @@ __device__ void lead(const nvshmemx_put_request_t *__restrict__ reqs, unsigned n
         unsigned xo, xu, to, tu;
         block_scan2(own ? 1u : 0u, u, xo, xu, to, tu, s_w);
         if (own) {
-            const unsigned e = e0 + rc + xo, vs = ru + xu;
-            entries[e] = pentry{i, vs};
-            /* every chunk that starts inside [vs, vs + u) starts in entry e */
-            for (unsigned k = (vs + cu - 1) / cu; k * cu < vs + u; ++k) table[c0 + k] = e;
+            if (self) {
+                s_own[rc + xo] = i;
+            } else {
+                const unsigned e = e0 + rc + xo, vs = ru + xu;
+                entries[e] = pentry{i, vs};
+                /* every chunk that starts inside [vs, vs + u) starts in entry e */
+                for (unsigned k = (vs + cu - 1) / cu; k * cu < vs + u; ++k) table[c0 + k] = e;
+            }
         }
         rc += to;
         ru += tu;
@@ -304,13 +348,40 @@ __device__ void lead(const nvshmemx_put_request_t *__restrict__ reqs, unsigned n
     __syncthreads();
     if (threadIdx.x == 0) {
         *S.info(p) = s_me;
-        store_release(S.ready(p), 1u); /* writes back this XCD's L2 first */
+        store_release(S.ready(p), (unsigned)(ticket / gridDim.x) + 1u); /* writes back this XCD's L2 first */
     }
-}
-
-__device__ __forceinline__ uint64_t lane_u64(uint64_t v, unsigned j) {
-    const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, j), hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), j);
-    return ((uint64_t)hi << 32) | lo;
+    if (!self) return;
+    /* the PE's own requests, one per lane of every wavefront: those of
+     * 16 KiB or more by the whole workgroup, in turn; the shorter ones at
+     * once, one wavefront each when there are at most 16 of them, else 16
+     * lanes each (at most 64 of them) */
+    const unsigned cnt = s_cnt[p], lane = threadIdx.x & 63u;
+    uint64_t tgt = 0, src = 0, nb = 0;
+    unsigned pe = 0;
+    if (lane < cnt) {
+        const nvshmemx_put_request_t r = reqs[s_own[lane]];
+        tgt = (uint64_t)r.target, src = (uint64_t)r.source, nb = (uint64_t)r.nbytes, pe = (unsigned)r.pe;
+    }
+    for (unsigned j = 0; j < cnt; ++j) {
+        const size_t n = (size_t)lane_u64(nb, j);
+        if (n < kWaveSegMax) continue;
+        char *dst = translate((const void *)lane_u64(tgt, j), (int)__builtin_amdgcn_readlane(pe, j));
+        copy<16, kPUnroll, NT_STORE>(dst, (const char *)lane_u64(src, j), n, threadIdx.x, kPBlock);
+    }
+    uint64_t smask = __ballot(lane < cnt && nb < kWaveSegMax);
+    const unsigned nsmall = (unsigned)__builtin_popcountll(smask);
+    const unsigned group = nsmall <= (unsigned)kPWaves ? 64u : 16u; /* lanes per short request */
+    const unsigned rank = threadIdx.x / group;
+    /* the rank-th short request sits in lane j; every lane takes part in the
+     * shuffles (a shuffle reads nothing from a lane that is switched off) */
+    for (unsigned k = 0; k < (rank < nsmall ? rank : 0u); ++k) smask &= smask - 1;
+    const int j = smask ? __builtin_ctzll(smask) : 0;
+    const uint64_t jt = ((uint64_t)(unsigned)__shfl((int)(tgt >> 32), j, 64) << 32) | (unsigned)__shfl((int)tgt, j, 64);
+    const uint64_t js = ((uint64_t)(unsigned)__shfl((int)(src >> 32), j, 64) << 32) | (unsigned)__shfl((int)src, j, 64);
+    const size_t n = (size_t)(unsigned)__shfl((int)nb, j, 64); /* below 16 KiB */
+    const int jpe = __shfl((int)pe, j, 64);
+    if (rank < nsmall) copy<16, kPUnroll, NT_STORE>(translate((const void *)jt, jpe), (const char *)js, n,
+                                                   threadIdx.x % group, group);
 }
 
 /* a workgroup copies chunk k of a queue: a request segment of kWaveSegMax
@@ -357,20 +428,32 @@ __device__ __forceinline__ void run_chunk(const nvshmemx_put_request_t *__restri
     }
 }
 
-/* every workgroup drains every queue in PE order: chunks are claimed from one
- * counter over the queues laid end to end (the chunk table already holds
- * them in PE order), so the workgroups running at once copy neighbouring
- * chunks of the same few PEs' queues */
-__device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__ reqs, const pscr &S, int P) {
+/* every workgroup drains every queue in PE order: chunks are numbered over
+ * the queues laid end to end (the chunk table already holds them in PE
+ * order); workgroup b takes chunk b first, without an atomic, and claims the
+ * rest from this launch's counter, so the workgroups running at once copy
+ * neighbouring chunks of the same few PEs' queues.  `ticket` (read by thread
+ * 0 only) is the workgroup's launch ticket. */
+__device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__ reqs, const pscr &S, int P,
+                                      unsigned long long ticket) {
     __shared__ pinfo s_inf[NVSHMEMI_MAX_PES];
-    __shared__ unsigned s_total, s_k[2];
+    __shared__ unsigned s_total, s_k[2], s_tag, s_parity;
     __shared__ int s_qk[2], s_missing;
-    if (threadIdx.x == 0) s_total = 0, s_missing = 0;
+    if (threadIdx.x == 0) {
+        const unsigned long long epoch = ticket / gridDim.x;
+        s_tag = (unsigned)epoch + 1u;
+        s_parity = (unsigned)epoch & 1u;
+        s_total = 0, s_missing = 0;
+        /* the next launch's claim counter: no workgroup of this launch uses it */
+        if (blockIdx.x == 0) store_relaxed(S.claim(s_parity ^ 1u), 0u);
+    }
     __syncthreads();
+    const unsigned tag = s_tag;
+    unsigned *claim = S.claim(s_parity);
     if ((int)threadIdx.x < P) {
         const int q = (int)threadIdx.x;
         pinfo inf;
-        if (wait_ready(S.ready(q))) {
+        if (wait_tag(S.ready(q), tag)) {
             inf = coherent_info(S.info(q));
         } else { /* a queue that never appeared: its chunks are skipped */
             inf.entry0 = inf.nown = inf.chunk0 = inf.nchunks = inf.chunk_units = 0u;
@@ -396,10 +479,10 @@ __device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__
         }
         return lo;
     };
+    if (blockIdx.x >= total) return; /* no chunk for this workgroup: it claims nothing */
     if (threadIdx.x == 0) {
-        const unsigned k = atomicAdd(S.claim(), 1u);
-        s_k[0] = k;
-        s_qk[0] = k < total ? owner(k) : -1;
+        s_k[0] = blockIdx.x;
+        s_qk[0] = owner(blockIdx.x);
     }
     __syncthreads();
     unsigned it = 0;
@@ -408,7 +491,7 @@ __device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__
         const int q = __builtin_amdgcn_readfirstlane(s_qk[it & 1]);
         if (k >= total) break;
         if (threadIdx.x == 0) {
-            const unsigned nk = atomicAdd(S.claim(), 1u);
+            const unsigned nk = gridDim.x + atomicAdd(claim, 1u);
             s_k[(it + 1) & 1] = nk;
             s_qk[(it + 1) & 1] = nk < total ? owner(nk) : -1;
         }
@@ -419,23 +502,21 @@ __device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__
     }
 }
 
+/* Launch ticket: every workgroup takes one, so launch E (counted from the
+ * scratch's set-up, with the grid fixed in between) holds tickets
+ * E * grid .. E * grid + grid - 1.  Ready flags carry E + 1 and are never
+ * cleared; the claim counters alternate between launches and each launch
+ * zeroes the next one's.  Nothing waits for the last workgroup, and a graph
+ * replay needs no reset. */
 __global__ void __launch_bounds__(kPBlock) k_batch_persistent(const nvshmemx_put_request_t *__restrict__ reqs,
                                                               unsigned nreq, int P, unsigned char *scr) {
     const pscr S{scr};
+    unsigned long long ticket = 0;
+    if (threadIdx.x == 0)
+        ticket = __hip_atomic_fetch_add(S.ticket(), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int p = (int)(blockIdx.x % (unsigned)P);
-    if ((int)blockIdx.x < P) lead(reqs, nreq, P, p, S);
-    drain(reqs, S, P);
-
-    /* the last workgroup out clears the control slots for the next launch */
-    if (threadIdx.x == 0) {
-        fence_release();
-        const unsigned old = __hip_atomic_fetch_add(S.done(), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == gridDim.x - 1) {
-            for (int q = 0; q < P; ++q) store_relaxed(S.ready(q), 0u);
-            store_relaxed(S.claim(), 0u);
-            store_relaxed(S.done(), 0u);
-        }
-    }
+    if ((int)blockIdx.x < P) lead(reqs, nreq, P, p, S, ticket);
+    drain(reqs, S, P, ticket);
 }
 
 /* zeroes the control slots (set-up only; a fill kernel, not a memset) */
@@ -464,10 +545,12 @@ static void clear_ctl(unsigned char *base) {
     NVSHMEMI_HIP(hipGetLastError());
 }
 
-/* Library set-up, once per device with that device current: the address of
- * this device's copy of `scratch` (its control slots cleared by a fill
- * kernel), and how many workgroups of the kernel the device runs at once (at
- * most kPPerCU per CU). */
+/* Library set-up, at every start with the device current: the address of
+ * this device's copy of `scratch`, and how many workgroups of the kernel the
+ * device runs at once (at most kPPerCU per CU).  The control slots of both
+ * scratches are cleared by a fill kernel here: launch tickets count from
+ * zero with the grid of this start (it changes only with the number of local
+ * PEs, i.e. between starts). */
 void persistent_init() {
     PersistDevice &d = pdev();
     if (!d.static_ptr) {
@@ -475,9 +558,10 @@ void persistent_init() {
         NVSHMEMI_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
         if (!ptr) NVSHMEMI_FATAL("hipGetSymbolAddress returned no address for the persistent kernel's scratch");
         d.static_ptr = (unsigned char *)ptr;
-        clear_ctl(d.static_ptr);
-        NVSHMEMI_HIP(hipDeviceSynchronize());
     }
+    clear_ctl(d.static_ptr);
+    if (d.alloc_ptr) clear_ctl(d.alloc_ptr);
+    NVSHMEMI_HIP(hipDeviceSynchronize());
     if (!d.capacity) {
         int per_cu = 0, cus = 0;
         NVSHMEMI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g.device));

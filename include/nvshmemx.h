@@ -192,14 +192,16 @@ void nvshmemx_put_requests_on_stream(const nvshmemx_put_request_t *reqs, int nre
  * in local mode the PE whose heap holds its source (a private source counts
  * for PE i mod npes), in process mode this PE.  Workgroup p < npes (local
  * mode; workgroup 0 in process mode) is PE p's leader and cuts that PE's
- * requests into chunks on the device; then every workgroup copies chunks of
- * all PEs' queues in PE order, claimed one at a time from one counter.
- * Destinations must not overlap (puts are unordered, as in the
+ * requests into chunks on the device (a PE with at most 64 requests and
+ * 128 KiB of payload is copied by its leader instead); then every workgroup
+ * copies chunks of all PEs' queues in PE order, claimed one at a time from
+ * one counter.  Destinations must not overlap (puts are unordered, as in the
  * fused plans).  A request naming a bad PE or a target range outside the heap
  * is skipped and sets NVSHMEMX_ERR_BAD_PE / NVSHMEMX_ERR_BAD_ADDRESS.
  * The library allocates the kernel's scratch (a control block, a 256 KiB
- * chunk table and 8 bytes per request), growing it on demand; the kernel
- * clears its counters itself, so a captured call replays without a memset.
+ * chunk table and 8 bytes per request), growing it on demand; launches tag
+ * their flags with a launch number the kernel counts itself, so nothing is
+ * cleared between launches and a captured call replays without a memset.
  * The first call (or one with more requests than any before) allocates and
  * so cannot be captured.  Calls that overlap on one GPU share the scratch
  * and race.  Returns 0. */
@@ -215,9 +217,9 @@ int nvshmemx_batch_persistent_grid(void);
  * allocates nothing and only launches the kernel, so even a first call can be
  * captured.  At most nvshmemx_batch_persistent_max_requests_noalloc()
  * (65,536) requests: a larger list is refused with a message and -1, nothing
- * launched.  Every call on a GPU shares that GPU's array (the kernel clears
- * its counters at the end of each launch); calls that overlap on one GPU
- * race.  Returns 0. */
+ * launched.  Every call on a GPU shares that GPU's array (each launch
+ * leaves it ready for the next; nothing is cleared between calls); calls
+ * that overlap on one GPU race.  Returns 0. */
 int nvshmemx_batch_put_persistent_noalloc_on_stream(const nvshmemx_put_request_t *reqs, int nreq, hipStream_t stream);
 int nvshmemx_batch_persistent_max_requests_noalloc(void);
 

@@ -89,11 +89,49 @@ def _ragged(rng, n, max_nb, sizes=None):
     return reqs
 
 
-@pytest.mark.parametrize("case", ["workload", "ragged", "tiny", "one_large", "repeated"])
+def _per_pe(rng, sizes):
+    """sizes[pe] = the byte counts PE pe issues (sources in its own heap),
+    each to a random other PE, destinations packed with gaps"""
+    rows = [(pe, nb) for pe in range(NPES) for nb in sizes[pe]]
+    rows = [rows[i] for i in rng.permutation(len(rows))]
+    n = len(rows)
+    reqs = {"src_pe": np.zeros(n, np.int32), "dst_pe": np.zeros(n, np.int32), "src_off": np.zeros(n, np.uint64),
+            "dst_off": np.zeros(n, np.uint64), "nbytes": np.zeros(n, np.uint64), "dtype": np.zeros(n, np.int32)}
+    cursor = np.zeros(NPES, np.int64)
+    for i, (pe, nb) in enumerate(rows):
+        d = (pe + 1 + int(rng.integers(0, NPES - 1))) % NPES
+        reqs["src_pe"][i], reqs["dst_pe"][i], reqs["nbytes"][i] = pe, d, nb
+        reqs["src_off"][i] = int(rng.integers(0, SRC - nb - 16))
+        reqs["dst_off"][i] = int(cursor[d]) + int(rng.integers(0, 16))
+        cursor[d] = int(reqs["dst_off"][i]) + nb + 1
+    assert cursor.max() <= DST
+    return reqs
+
+
+@pytest.mark.parametrize("case", ["workload", "ragged", "tiny", "one_large", "repeated", "leader_served",
+                                  "served_and_queued"])
 def test_persistent_matches_oracle(setup, hip, oracle, case):
+    """leader_served: every PE issues at most 64 requests and 128 KiB, so its
+    leader copies them itself (no queue): 1 to 64 requests, byte-granular
+    sizes below 1600 B (wavefront or 16-lane groups), requests of 16 KiB and
+    more among them (the whole workgroup), and a PE with none;
+    served_and_queued: half the PEs served by their leaders, half with
+    queues (more requests or more bytes than that)"""
     lib, base, images = setup
     rng = np.random.default_rng(31)
-    if case == "workload":
+    if case == "leader_served":
+        counts = [1, 5, 16, 17, 40, 64, 0, 3]
+        sizes = [[int(rng.integers(0, 1600)) for _ in range(c)] for c in counts]
+        sizes[1][0], sizes[3][5], sizes[5][63] = 16384, 70001, 20000
+        sizes[7] = [65536, 16385, 100]
+        assert all(len(z) <= 64 and sum(z) <= 128 << 10 for z in sizes)
+        reqs = _per_pe(rng, sizes)
+    elif case == "served_and_queued":
+        sizes = [[int(rng.integers(1, 4096)) for _ in range(c)] for c in [10, 65, 30, 200, 64, 1, 2, 50]]
+        sizes[6] = [131072, 1]  # one byte over the leader's limit: queued
+        sizes[7][0] = 1 << 20
+        reqs = _per_pe(rng, sizes)
+    elif case == "workload":
         reqs = na.gen_requests(lib, 41, 1024, NPES, SRC, DST)
     elif case == "ragged":
         reqs = _ragged(rng, 700, 0, sizes=[0, 1, 2, 3, 5, 15, 17, 255, 1023, 4095, 4097, 16383, 16385, 70001,
