@@ -222,7 +222,8 @@ def test_persistent_between_processes():
 def test_persistent_128_pes(local_pes, hip, oracle):
     """the most local PEs the library holds (128): leaders 64..127 take the
     second half of each lane's PEs when the queues' offsets are summed.  Runs
-    last in this file: it restarts the library the module fixture set up."""
+    after the tests of the module fixture: it restarts the library that
+    fixture set up (as the tests after it do)."""
     npes, src, dst = 128, 1 << 20, 1 << 20
     lib = local_pes.start(npes, wgs_per_pe=1, heap="2M")
     base = lib.nvshmalloc(src + dst)
@@ -239,6 +240,40 @@ def test_persistent_128_pes(local_pes, hip, oracle):
     arr = na.make_requests(lib, reqs, base, 0, src)
     oracle.batch_apply(images, reqs, 0, src)
     _run(lib, hip, arr, len(arr), times=2)
+    assert lib.nvshmemx_get_last_error() == 0
+    for pe in range(npes):
+        got = hip.dtoh(lib.nvshmem_ptr(base, pe) + src, dst)
+        if not np.array_equal(got, images[pe][src:]):
+            bad = np.flatnonzero(got != images[pe][src:])
+            raise AssertionError(f"PE {pe}: {bad.size} bytes differ, first at dst offset {bad[0]}")
+
+
+@pytest.mark.parametrize("npes", [6, 8])
+def test_persistent_after_restart_with_another_grid(local_pes, hip, oracle, npes):
+    """The kernel numbers its launches by tickets that count from the start of
+    the library with a fixed grid (the local PEs times the most workgroups
+    per PE that fit at once).  On 256 CUs, 6 PEs make a grid of 252
+    workgroups where 8 or 128 make 256: after each restart, launches (and
+    repeated ones) still find every queue, bytes exact, no wait timing
+    out"""
+    src, dst = 4 << 20, 4 << 20
+    lib = local_pes.start(npes, wgs_per_pe=1, heap="16M")
+    grid = lib.nvshmemx_batch_persistent_grid()
+    assert grid % npes == 0
+    base = lib.nvshmalloc(src + dst)
+    images = []
+    for pe in range(npes):
+        p = lib.nvshmem_ptr(base, pe)
+        lib.nvshmemh_fill_pcg32(p, src // 4, 700 + pe, 54, None)
+        hip.memset(p + src, 0, dst)
+        img = np.zeros(src + dst, np.uint8)
+        img[:src] = oracle.pcg32(src // 4, 700 + pe).view(np.uint8)
+        images.append(img)
+    hip.sync()
+    reqs = na.gen_requests(lib, 59 + npes, 300, npes, src, dst, 6, 18)
+    arr = na.make_requests(lib, reqs, base, 0, src)
+    oracle.batch_apply(images, reqs, 0, src)
+    _run(lib, hip, arr, len(arr), times=3)
     assert lib.nvshmemx_get_last_error() == 0
     for pe in range(npes):
         got = hip.dtoh(lib.nvshmem_ptr(base, pe) + src, dst)
