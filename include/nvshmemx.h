@@ -272,7 +272,8 @@ int nvshmemx_costmodel_num_outputs(nvshmemx_costmodel_t model);
 /* Policies: 0 the library's rule, 1 the per-request model's choices,
  * 2..5 every request cut into wavefront pieces of about 4/8/16/32 KiB,
  * 6..10 into workgroup pieces of about 64/128/256/512 KiB and 1 MiB,
- * 11 one workgroup piece per request. */
+ * 11 one workgroup piece per request, 12 every request as tiles (choice
+ * mode 2). */
 int nvshmemx_batch_policy_count(void);
 /* the four normalised batch-level inputs of the policy model: log2 of the
  * batch's bytes, of its request count, mean and largest log2 request size */
