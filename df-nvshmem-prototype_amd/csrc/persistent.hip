@@ -42,7 +42,7 @@
  * invalidates the XCD's L2 under the streams the other workgroups are
  * running): the leader publishes with one release, and everything it
  * published is read with coherent (agent-scope relaxed) loads after the flag
- * was seen set.
+ * was seen to carry this launch's number.
  *
  * Launch shape, from measurements of the 4096-put step of 64 PEs (1.57 GB
  * payload, ms per step, before the leader and chunk changes below): one
