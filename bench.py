@@ -97,10 +97,11 @@ def batch_worker(counter_pass=False):
     workgroups run the puts it initiated), every request cut into tiles (one
     workgroup a tile, launched and replayed from a hipGraph), the persistent
     kernel planned on the device by the issuing PEs (library-allocated and
-    static scratch) and the per-call path (one launch per put).  Prints one
-    JSON line.  counter_pass: only five fused launches, five persistent
-    launches, five tile launches and one 1 GiB device-to-device copy (the
-    scale of a counter pass)."""
+    static scratch), the same puts posted on the device by the PEs' own
+    workgroups and flushed, and the per-call path (one launch per put).
+    Prints one JSON line.  counter_pass: only five fused launches, five
+    persistent launches, five post-and-flush rounds, five tile launches and
+    one 1 GiB device-to-device copy (the scale of a counter pass)."""
     import ctypes as C
     import numpy as np
     import nvshmem_amd as na
@@ -134,6 +135,12 @@ def batch_worker(counter_pass=False):
         hip.htod(d_reqs, raw)
         for _ in range(5):
             lib.nvshmemx_batch_put_persistent_on_stream(d_reqs, n, None)
+        iss = np.ascontiguousarray(reqs["src_pe"], np.int32)
+        d_iss = hip.malloc(iss.nbytes)
+        hip.htod(d_iss, iss.view(np.uint8))
+        for _ in range(5):
+            lib.nvshmemh_post_nbi(d_reqs, d_iss, n, None)
+            lib.nvshmemx_nbi_flush_on_stream(None)
         tile_choices = (na.BatchChoice * n)()
         for i in range(n):
             tile_choices[i].pieces, tile_choices[i].vec_bytes, tile_choices[i].mode = 1, 16, 2
@@ -184,8 +191,36 @@ def batch_worker(counter_pass=False):
         d_reqs, n, None), 20, 3)
     out["persistent_grid"] = int(lib.nvshmemx_batch_persistent_grid())
     out["bitexact"] = out["bitexact"] and int(lib.nvshmemh_verify_requests(arr, n)) == 0
+    # the same puts posted on the device by the workgroup PEs that issue them
+    # (nvshmemx_putmem_nbi from each PE's 4 workgroups), then one flush; the
+    # posting kernel is timed alone as well
+    iss = np.ascontiguousarray(reqs["src_pe"], np.int32)
+    d_iss = hip.malloc(iss.nbytes)
+    hip.htod(d_iss, iss.view(np.uint8))
+    for pe in range(BATCH_NPES):  # the check below then sees only what the posts wrote
+        hip.memset(lib.nvshmem_ptr(base, pe) + BATCH_SRC, 0, BATCH_DST)
+    hip.sync()
+
+    def post_and_flush():
+        lib.nvshmemh_post_nbi(d_reqs, d_iss, n, None)
+        lib.nvshmemx_nbi_flush_on_stream(None)
+    out["posted_ms"] = timed(hip, post_and_flush, 20, 3)
+    out["bitexact"] = out["bitexact"] and int(lib.nvshmemh_verify_requests(arr, n)) == 0
+    e0, e1 = hip.event(), hip.event()
+    post_ms = []
+    for _ in range(20):
+        hip.record(e0, None)
+        lib.nvshmemh_post_nbi(d_reqs, d_iss, n, None)
+        hip.record(e1, None)
+        lib.nvshmemx_nbi_flush_on_stream(None)
+        hip.sync()
+        post_ms.append(hip.elapsed_ms(e0, e1))
+    hip.event_destroy(e0)
+    hip.event_destroy(e1)
+    out["posted_post_kernel_ms"] = float(np.median(post_ms))
+    hip.free(d_iss)
     hip.free(d_reqs)
-    _log("batch: fused, graph, owner and persistent timed; per-call path next")
+    _log("batch: fused, graph, owner, persistent and posted timed; per-call path next")
     out["per_call_ms"] = timed(hip, lambda: lib.nvshmemx_put_requests_on_stream(arr, n, None), 3, 1)
     out["bitexact"] = out["bitexact"] and int(lib.nvshmemh_verify_requests(arr, n)) == 0
     out["errors"] = lib.nvshmemx_get_last_error()
@@ -545,7 +580,7 @@ def run_batch_leg():
     rec["fused_gbps"] = round(2 * rec["payload_bytes"] / (best * 1e-3) / 1e9, 1)
     rec["speedup_vs_per_call"] = round(rec["per_call_ms"] / best, 2)
     for k in ("fused_ms", "graph_ms", "owner_ms", "per_call_ms", "persistent_ms", "persistent_noalloc_ms", "tiles_ms",
-              "tiles_graph_ms"):
+              "tiles_graph_ms", "posted_ms", "posted_post_kernel_ms"):
         if k in rec:
             rec[k] = round(rec[k], 4)
     if "tiles_ms" in rec:
@@ -554,6 +589,9 @@ def run_batch_leg():
     if "persistent_ms" in rec:
         rec["persistent_gbps"] = round(2 * rec["payload_bytes"] / (rec["persistent_ms"] * 1e-3) / 1e9, 1)
         rec["persistent_speedup_vs_per_call"] = round(rec["per_call_ms"] / rec["persistent_ms"], 2)
+    if "posted_ms" in rec:
+        rec["posted_gbps"] = round(2 * rec["payload_bytes"] / (rec["posted_ms"] * 1e-3) / 1e9, 1)
+        rec["posted_speedup_vs_per_call"] = round(rec["per_call_ms"] / rec["posted_ms"], 2)
     return rec
 
 

@@ -193,6 +193,36 @@ NVSHMEMI_DEF_GET(int, int, NVSHMEMI_OP_GET_INT)
 NVSHMEMI_DEF_GET(float, float, NVSHMEMI_OP_GET_FLOAT)
 NVSHMEMI_DEF_GET(double, double, NVSHMEMI_OP_GET_DOUBLE)
 
+/* ---------------------------------------------- non-blocking posts -- */
+
+/* Post a put for the next nvshmemx_nbi_flush_on_stream: one slot of the
+ * calling PE's queue (the workgroup's PE in local mode, the process's in
+ * process mode), claimed with one atomic.  A post that finds the queue's
+ * share full (or a workgroup of no PE's) is copied at once by the calling
+ * lane instead, which a non-blocking put may do.  Invalid targets are
+ * refused here, as for the other device calls; zero-byte posts do nothing. */
+extern "C" __device__ void nvshmemx_putmem_nbi(void *target, const void *source, size_t nbytes, int pe) {
+    if (bad_target(target, nbytes, pe) || nbytes == 0) return;
+    const nvshmemi_device_state_t &s = nvshmemi_dstate;
+    const bool local = s.mode == NVSHMEMI_MODE_LOCAL;
+    const unsigned queues = local ? (unsigned)s.npes : 1u, q = local ? (unsigned)my_pe() : 0u;
+    const unsigned share = kNbiSlots / queues;
+    /* a workgroup past the last PE's (a grid larger than the PEs') has no queue */
+    const unsigned slot = q < queues ? __hip_atomic_fetch_add(&nbi_count[q], 1u, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT)
+                                     : share;
+    if (slot < share) {
+        nvshmemx_put_request_t &r = nbi_queue[(size_t)q * share + slot];
+        r.target = target;
+        r.source = source;
+        r.nbytes = nbytes;
+        r.pe = pe;
+        r.elem_bytes = 1;
+    } else {
+        lane_copy(translate(target, pe), source, nbytes);
+    }
+}
+
 /* --------------------------------------------------------------- waits -- */
 
 extern "C" __device__ void nvshmem_int_wait(int *ivar, int cmp_value) {

@@ -463,6 +463,20 @@ __global__ void __launch_bounds__(256) k_copy_stamped(char *dst, const char *src
     }
 }
 
+/* Workgroup-partitioned PEs post puts with nvshmemx_putmem_nbi: workgroup w
+ * belongs to the PE nvshmemx_my_pe_d() names, and the `wgs` workgroups of a
+ * PE share the requests it issued (issuer[i] == its PE; a null issuer array
+ * gives every request to the caller's PE). */
+__global__ void k_post_nbi(const nvshmemx_put_request_t *reqs, const int *issuer, int n, int wgs) {
+    const int pe = nvshmemx_my_pe_d();
+    const int sub = (int)(blockIdx.x % (unsigned)wgs);
+    for (int i = sub * (int)blockDim.x + (int)threadIdx.x; i < n; i += wgs * (int)blockDim.x) {
+        if (issuer && issuer[i] != pe) continue;
+        const nvshmemx_put_request_t r = reqs[i];
+        nvshmemx_putmem_nbi(r.target, r.source, r.nbytes, r.pe);
+    }
+}
+
 } // namespace
 
 /* --------------------------------------------------------- C entry points -- */
@@ -716,6 +730,20 @@ void nvshmemh_publish_round(int *slots_sym, int r, hipStream_t s) {
 
 void nvshmemh_check_round(const int *slots_local, int r, unsigned long long *violations, hipStream_t s) {
     hipLaunchKernelGGL(k_check_round, dim3(1), dim3(128), 0, s, slots_local, nvshmem_n_pes(), r, violations);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+/* Post `n` requests (device array) from the PEs that issued them: in local
+ * mode issuer[i] (device array) names request i's PE and each PE posts from
+ * its own workgroups; in process mode this PE posts every request (issuer
+ * may be null).  256 threads a workgroup. */
+void nvshmemh_post_nbi(const nvshmemx_put_request_t *reqs, const int *issuer, int n, hipStream_t s) {
+    if (n <= 0) return;
+    const bool local = nvshmemx_mode() == NVSHMEMI_MODE_LOCAL;
+    if (local && !issuer) NVSHMEMI_FATAL("nvshmemh_post_nbi: local mode needs the issuing PE of every request");
+    const int wgs = local ? nvshmemx_wgs_per_pe() : 4;
+    const int grid = local ? nvshmem_n_pes() * wgs : wgs;
+    hipLaunchKernelGGL(k_post_nbi, dim3((unsigned)grid), dim3(256), 0, s, reqs, local ? issuer : nullptr, n, wgs);
     NVSHMEMI_HIP(hipGetLastError());
 }
 

@@ -22,12 +22,22 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "nvshmemi_state.h"
+#include "nvshmemx.h"
 
 extern __constant__ nvshmemi_device_state_t nvshmemi_dstate;
 
 namespace nvshmemi {
 
 /* ---------------------------------------------------------------- state -- */
+
+/* Puts posted with nvshmemx_putmem_nbi wait here for
+ * nvshmemx_nbi_flush_on_stream: kNbiSlots requests split evenly between the
+ * queues of the PEs this process holds (all of them in local mode, its own
+ * in process mode); nbi_count[q] counts queue q's posts since its last flush
+ * (posts past the queue's share were copied at once and are not in it). */
+constexpr unsigned kNbiSlots = 65536;
+extern __device__ nvshmemx_put_request_t nbi_queue[kNbiSlots];
+extern __device__ unsigned nbi_count[NVSHMEMI_MAX_PES];
 
 __device__ __forceinline__ char *translate(const void *addr, int pe) {
     const nvshmemi_device_state_t &s = nvshmemi_dstate;

@@ -139,6 +139,14 @@ constexpr size_t scratch_bytes(size_t nreq) { return kEntryOff + sizeof(pentry) 
  * hipGetSymbolAddress returned for the current device. */
 static __device__ __align__(256) unsigned char scratch[scratch_bytes(kMaxReqsNoalloc)];
 
+/* the posted puts (nvshmemx_putmem_nbi) and the scratch of their flush, kept
+ * apart from `scratch` so that a flush and a no-allocation batch do not share
+ * control slots; like `scratch`, host code reaches them only through
+ * hipGetSymbolAddress */
+__device__ nvshmemx_put_request_t nbi_queue[kNbiSlots];
+__device__ unsigned nbi_count[NVSHMEMI_MAX_PES];
+static __device__ __align__(256) unsigned char nbi_scratch[scratch_bytes(kNbiSlots)];
+
 struct pscr {
     unsigned char *base;
     __device__ __forceinline__ unsigned *ready(int q) const { return (unsigned *)(base + q * kSlot + kReadyOff); }
@@ -253,6 +261,41 @@ __device__ __forceinline__ bool leader_served(unsigned cnt, unsigned long long b
     return cnt <= kSelfCount && bytes <= kSelfBytes;
 }
 
+/* A leader copies its PE's own requests (cnt <= kSelfCount of them, indices
+ * in own[], all in LDS), one per lane of every wavefront: those of 16 KiB or
+ * more by the whole workgroup, in turn; the shorter ones at once, one
+ * wavefront each when there are at most 16 of them, else 16 lanes each (at
+ * most 64 of them).  Every thread of the workgroup calls. */
+__device__ __forceinline__ void copy_own(const nvshmemx_put_request_t *__restrict__ reqs, unsigned cnt, const unsigned *own) {
+    const unsigned lane = threadIdx.x & 63u;
+    uint64_t tgt = 0, src = 0, nb = 0;
+    unsigned pe = 0;
+    if (lane < cnt) {
+        const nvshmemx_put_request_t r = reqs[own[lane]];
+        tgt = (uint64_t)r.target, src = (uint64_t)r.source, nb = (uint64_t)r.nbytes, pe = (unsigned)r.pe;
+    }
+    for (unsigned j = 0; j < cnt; ++j) {
+        const size_t n = (size_t)lane_u64(nb, j);
+        if (n < kWaveSegMax) continue;
+        char *dst = translate((const void *)lane_u64(tgt, j), (int)__builtin_amdgcn_readlane(pe, j));
+        copy<16, kPUnroll, NT_STORE>(dst, (const char *)lane_u64(src, j), n, threadIdx.x, kPBlock);
+    }
+    uint64_t smask = __ballot(lane < cnt && nb < kWaveSegMax);
+    const unsigned nsmall = (unsigned)__builtin_popcountll(smask);
+    const unsigned group = nsmall <= (unsigned)kPWaves ? 64u : 16u; /* lanes per short request */
+    const unsigned rank = threadIdx.x / group;
+    /* the rank-th short request sits in lane j; every lane takes part in the
+     * shuffles (a shuffle reads nothing from a lane that is switched off) */
+    for (unsigned k = 0; k < (rank < nsmall ? rank : 0u); ++k) smask &= smask - 1;
+    const int j = smask ? __builtin_ctzll(smask) : 0;
+    const uint64_t jt = ((uint64_t)(unsigned)__shfl((int)(tgt >> 32), j, 64) << 32) | (unsigned)__shfl((int)tgt, j, 64);
+    const uint64_t js = ((uint64_t)(unsigned)__shfl((int)(src >> 32), j, 64) << 32) | (unsigned)__shfl((int)src, j, 64);
+    const size_t n = (size_t)(unsigned)__shfl((int)nb, j, 64); /* below 16 KiB */
+    const int jpe = __shfl((int)pe, j, 64);
+    if (rank < nsmall) copy<16, kPUnroll, NT_STORE>(translate((const void *)jt, jpe), (const char *)js, n,
+                                                   threadIdx.x % group, group);
+}
+
 /* leader of PE p: build and publish the PE's queue, or, for a PE the leader
  * serves itself, publish an empty queue and copy the PE's requests.
  * `ticket` (the workgroup's launch ticket) is read by thread 0 only, when it
@@ -350,38 +393,85 @@ __device__ void lead(const nvshmemx_put_request_t *__restrict__ reqs, unsigned n
         *S.info(p) = s_me;
         store_release(S.ready(p), (unsigned)(ticket / gridDim.x) + 1u); /* writes back this XCD's L2 first */
     }
-    if (!self) return;
-    /* the PE's own requests, one per lane of every wavefront: those of
-     * 16 KiB or more by the whole workgroup, in turn; the shorter ones at
-     * once, one wavefront each when there are at most 16 of them, else 16
-     * lanes each (at most 64 of them) */
-    const unsigned cnt = s_cnt[p], lane = threadIdx.x & 63u;
-    uint64_t tgt = 0, src = 0, nb = 0;
-    unsigned pe = 0;
-    if (lane < cnt) {
-        const nvshmemx_put_request_t r = reqs[s_own[lane]];
-        tgt = (uint64_t)r.target, src = (uint64_t)r.source, nb = (uint64_t)r.nbytes, pe = (unsigned)r.pe;
+    if (self) copy_own(reqs, s_cnt[p], s_own);
+}
+
+/* Leader of PE p in a flush of posted puts: its queue is slots
+ * p * share .. of nbi_queue, min(posted, share) of them.  It lays them out as
+ * lead() does, but in regions of its own (entries from p * share, chunk-table
+ * slots from p * (kTableChunks / P), its chunk size chosen so its chunks fit
+ * there), so it needs no other PE's counts; workers number the chunks of all
+ * queues from the published counts.  It zeroes the PE's post counter for the
+ * posts after this flush. */
+__device__ void lead_posted(int P, int p, const pscr &S, unsigned long long ticket) {
+    __shared__ unsigned s_w[2][kPWaves];
+    __shared__ unsigned s_own[kSelfCount];
+    __shared__ unsigned s_n, s_cnt;
+    __shared__ unsigned long long s_units, s_bytes;
+    __shared__ pinfo s_me;
+    __shared__ int s_self;
+    const unsigned share = kNbiSlots / (unsigned)P, tab = kTableChunks / (unsigned)P;
+    const unsigned base = (unsigned)p * share;
+    const nvshmemx_put_request_t *__restrict__ q = nbi_queue + base;
+    if (threadIdx.x == 0) {
+        const unsigned posted = load_relaxed(&nbi_count[p]);
+        s_n = posted < share ? posted : share;
+        s_cnt = 0, s_units = 0, s_bytes = 0;
+        store_relaxed(&nbi_count[p], 0u);
     }
-    for (unsigned j = 0; j < cnt; ++j) {
-        const size_t n = (size_t)lane_u64(nb, j);
-        if (n < kWaveSegMax) continue;
-        char *dst = translate((const void *)lane_u64(tgt, j), (int)__builtin_amdgcn_readlane(pe, j));
-        copy<16, kPUnroll, NT_STORE>(dst, (const char *)lane_u64(src, j), n, threadIdx.x, kPBlock);
+    __syncthreads();
+    const unsigned n = s_n;
+    for (unsigned i = threadIdx.x; i < n; i += kPBlock) {
+        const nvshmemx_put_request_t r = q[i];
+        const unsigned u = req_units(r, true);
+        if (u) {
+            atomicAdd(&s_cnt, 1u);
+            atomicAdd(&s_units, (unsigned long long)u);
+            atomicAdd(&s_bytes, (unsigned long long)r.nbytes);
+        }
     }
-    uint64_t smask = __ballot(lane < cnt && nb < kWaveSegMax);
-    const unsigned nsmall = (unsigned)__builtin_popcountll(smask);
-    const unsigned group = nsmall <= (unsigned)kPWaves ? 64u : 16u; /* lanes per short request */
-    const unsigned rank = threadIdx.x / group;
-    /* the rank-th short request sits in lane j; every lane takes part in the
-     * shuffles (a shuffle reads nothing from a lane that is switched off) */
-    for (unsigned k = 0; k < (rank < nsmall ? rank : 0u); ++k) smask &= smask - 1;
-    const int j = smask ? __builtin_ctzll(smask) : 0;
-    const uint64_t jt = ((uint64_t)(unsigned)__shfl((int)(tgt >> 32), j, 64) << 32) | (unsigned)__shfl((int)tgt, j, 64);
-    const uint64_t js = ((uint64_t)(unsigned)__shfl((int)(src >> 32), j, 64) << 32) | (unsigned)__shfl((int)src, j, 64);
-    const size_t n = (size_t)(unsigned)__shfl((int)nb, j, 64); /* below 16 KiB */
-    const int jpe = __shfl((int)pe, j, 64);
-    if (rank < nsmall) copy<16, kPUnroll, NT_STORE>(translate((const void *)jt, jpe), (const char *)js, n,
-                                                   threadIdx.x % group, group);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const bool self = leader_served(s_cnt, s_bytes);
+        const unsigned long long need = (s_units + tab - 1) / tab; /* so that the chunks fit the PE's slots */
+        const unsigned cu = (unsigned)(need > kChunkUnitsMin ? need : kChunkUnitsMin);
+        s_self = self;
+        s_me.entry0 = base;
+        s_me.nown = self ? 0u : s_cnt;
+        s_me.chunk0 = (unsigned)p * tab;
+        s_me.nchunks = self ? 0u : (unsigned)((s_units + cu - 1) / cu);
+        s_me.chunk_units = cu;
+    }
+    __syncthreads();
+    const bool self = s_self;
+    const unsigned cu = s_me.chunk_units, c0 = s_me.chunk0;
+    unsigned *table = S.table();
+    pentry *entries = S.entries();
+    unsigned rc = 0, ru = 0; /* entries and granules of earlier blocks */
+    for (unsigned b = 0; b < n; b += kPBlock) {
+        const unsigned i = b + threadIdx.x;
+        const unsigned u = i < n ? req_units(q[i], false) : 0u;
+        unsigned xo, xu, to, tu;
+        block_scan2(u ? 1u : 0u, u, xo, xu, to, tu, s_w);
+        if (u) {
+            if (self) {
+                s_own[rc + xo] = base + i;
+            } else {
+                const unsigned e = base + rc + xo, vs = ru + xu;
+                entries[e] = pentry{base + i, vs};
+                /* every chunk that starts inside [vs, vs + u) starts in entry e */
+                for (unsigned k = (vs + cu - 1) / cu; k * cu < vs + u; ++k) table[c0 + k] = e;
+            }
+        }
+        rc += to;
+        ru += tu;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *S.info(p) = s_me;
+        store_release(S.ready(p), (unsigned)(ticket / gridDim.x) + 1u); /* writes back this XCD's L2 first */
+    }
+    if (self) copy_own(nbi_queue, s_cnt, s_own);
 }
 
 /* a workgroup copies chunk k of a queue: a request segment of kWaveSegMax
@@ -434,9 +524,11 @@ __device__ __forceinline__ void run_chunk(const nvshmemx_put_request_t *__restri
  * rest from this launch's counter, so the workgroups running at once copy
  * neighbouring chunks of the same few PEs' queues.  `ticket` (read by thread
  * 0 only) is the workgroup's launch ticket. */
+template <bool POSTED>
 __device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__ reqs, const pscr &S, int P,
                                       unsigned long long ticket) {
     __shared__ pinfo s_inf[NVSHMEMI_MAX_PES];
+    __shared__ unsigned s_g0[NVSHMEMI_MAX_PES]; /* each queue's first chunk in the launch's numbering */
     __shared__ unsigned s_total, s_k[2], s_tag, s_parity;
     __shared__ int s_qk[2], s_missing;
     if (threadIdx.x == 0) {
@@ -460,21 +552,42 @@ __device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__
             s_missing = 1;
         }
         s_inf[q] = inf;
-        if (inf.nchunks) atomicMax(&s_total, inf.chunk0 + inf.nchunks);
+        if (!POSTED && inf.nchunks) atomicMax(&s_total, inf.chunk0 + inf.nchunks);
     }
     __syncthreads();
-    if (threadIdx.x == 0 && s_missing) /* keep first chunks non-decreasing over a missing queue */
-        for (int q = 1; q < P; ++q)
-            if (!s_inf[q].chunk_units) s_inf[q].chunk0 = s_inf[q - 1].chunk0 + s_inf[q - 1].nchunks;
+    if constexpr (POSTED) {
+        /* posted queues keep their chunks in regions of their own: number
+         * them end to end in PE order (wavefront 0, two PEs a lane) */
+        if (threadIdx.x < 64) {
+            const unsigned lane = threadIdx.x, qa = 2 * lane, qb = 2 * lane + 1;
+            const unsigned na = qa < (unsigned)P ? s_inf[qa].nchunks : 0u, nb = qb < (unsigned)P ? s_inf[qb].nchunks : 0u;
+            unsigned incl = na + nb;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const unsigned o = __shfl_up(incl, d, 64);
+                if (lane >= (unsigned)d) incl += o;
+            }
+            const unsigned excl = incl - na - nb;
+            if (qa < (unsigned)P) s_g0[qa] = excl;
+            if (qb < (unsigned)P) s_g0[qb] = excl + na;
+            if (lane == 63) s_total = incl;
+        }
+    } else {
+        if (threadIdx.x == 0 && s_missing) /* keep first chunks non-decreasing over a missing queue */
+            for (int q = 1; q < P; ++q)
+                if (!s_inf[q].chunk_units) s_inf[q].chunk0 = s_inf[q - 1].chunk0 + s_inf[q - 1].nchunks;
+        __syncthreads();
+        if ((int)threadIdx.x < P) s_g0[threadIdx.x] = s_inf[threadIdx.x].chunk0;
+    }
     __syncthreads();
     const unsigned total = s_total;
-    /* queue of global chunk k: the last PE whose first chunk is at or before
-     * k (PEs with no chunks never match: the range test below skips them) */
+    /* queue of chunk k: the last PE whose first chunk is at or before k (PEs
+     * with no chunks never match: the range test below skips them) */
     auto owner = [&](unsigned k) -> int {
         int lo = 0, hi = P - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (s_inf[mid].chunk0 <= k) lo = mid;
+            if (s_g0[mid] <= k) lo = mid;
             else hi = mid - 1;
         }
         return lo;
@@ -496,7 +609,7 @@ __device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__
             s_qk[(it + 1) & 1] = nk < total ? owner(nk) : -1;
         }
         const pinfo &inf = s_inf[q];
-        if (k - inf.chunk0 < inf.nchunks) run_chunk(reqs, inf, k - inf.chunk0, S);
+        if (k - s_g0[q] < inf.nchunks) run_chunk(reqs, inf, k - s_g0[q], S);
         __syncthreads();
         ++it;
     }
@@ -516,7 +629,19 @@ __global__ void __launch_bounds__(kPBlock) k_batch_persistent(const nvshmemx_put
         ticket = __hip_atomic_fetch_add(S.ticket(), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int p = (int)(blockIdx.x % (unsigned)P);
     if ((int)blockIdx.x < P) lead(reqs, nreq, P, p, S, ticket);
-    drain(reqs, S, P, ticket);
+    drain<false>(reqs, S, P, ticket);
+}
+
+/* The flush of posted puts: the same launch with each leader taking its
+ * PE's own queue (lead_posted) instead of picking its requests out of a
+ * shared list. */
+__global__ void __launch_bounds__(kPBlock) k_flush_posted(int P, unsigned char *scr) {
+    const pscr S{scr};
+    unsigned long long ticket = 0;
+    if (threadIdx.x == 0)
+        ticket = __hip_atomic_fetch_add(S.ticket(), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((int)blockIdx.x < P) lead_posted(P, (int)blockIdx.x, S, ticket);
+    drain<true>(nbi_queue, S, P, ticket);
 }
 
 /* zeroes the control slots (set-up only; a fill kernel, not a memset) */
@@ -529,6 +654,8 @@ __global__ void k_persist_clear(unsigned *ctl, size_t n) {
 
 struct PersistDevice {
     unsigned char *static_ptr = nullptr; /* this device's copy of `scratch` */
+    unsigned char *posted_ptr = nullptr; /* ... of `nbi_scratch` */
+    unsigned *count_ptr = nullptr;       /* ... of `nbi_count` */
     unsigned char *alloc_ptr = nullptr;  /* library-allocated scratch */
     size_t alloc_reqs = 0;
     int capacity = 0; /* workgroups of the persistent kernel resident at once */
@@ -554,12 +681,20 @@ static void clear_ctl(unsigned char *base) {
 void persistent_init() {
     PersistDevice &d = pdev();
     if (!d.static_ptr) {
-        void *ptr = nullptr;
+        void *ptr = nullptr, *posted = nullptr, *count = nullptr;
         NVSHMEMI_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
-        if (!ptr) NVSHMEMI_FATAL("hipGetSymbolAddress returned no address for the persistent kernel's scratch");
+        NVSHMEMI_HIP(hipGetSymbolAddress(&posted, HIP_SYMBOL(nbi_scratch)));
+        NVSHMEMI_HIP(hipGetSymbolAddress(&count, HIP_SYMBOL(nbi_count)));
+        if (!ptr || !posted || !count)
+            NVSHMEMI_FATAL("hipGetSymbolAddress returned no address for the persistent kernel's scratch");
         d.static_ptr = (unsigned char *)ptr;
+        d.posted_ptr = (unsigned char *)posted;
+        d.count_ptr = (unsigned *)count;
     }
     clear_ctl(d.static_ptr);
+    clear_ctl(d.posted_ptr);
+    hipLaunchKernelGGL(k_persist_clear, dim3(1), dim3(256), 0, 0, d.count_ptr, (size_t)NVSHMEMI_MAX_PES);
+    NVSHMEMI_HIP(hipGetLastError());
     if (d.alloc_ptr) clear_ctl(d.alloc_ptr);
     NVSHMEMI_HIP(hipDeviceSynchronize());
     if (!d.capacity) {
@@ -628,6 +763,25 @@ extern "C" int nvshmemx_batch_put_persistent_noalloc_on_stream(const nvshmemx_pu
 }
 
 extern "C" int nvshmemx_batch_persistent_max_requests_noalloc(void) { return kMaxReqsNoalloc; }
+
+extern "C" int nvshmemx_nbi_flush_on_stream(hipStream_t s) {
+    require_init("nvshmemx_nbi_flush_on_stream");
+    const PersistDevice &d = pdev();
+    if (!d.posted_ptr) NVSHMEMI_FATAL("nvshmemx_nbi_flush_on_stream: the library's set-up did not run");
+    const int P = local_issuers();
+    if (d.capacity < P)
+        NVSHMEMI_FATAL("the persistent batch kernel needs one resident workgroup per PE: %d PEs, %d resident", P,
+                       d.capacity);
+    const int grid = P * (d.capacity / P);
+    hipLaunchKernelGGL(k_flush_posted, dim3((unsigned)grid), dim3(kPBlock), 0, s, P, d.posted_ptr);
+    NVSHMEMI_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int nvshmemx_nbi_capacity(void) {
+    require_init("nvshmemx_nbi_capacity");
+    return (int)(kNbiSlots / (unsigned)local_issuers());
+}
 
 extern "C" int nvshmemx_batch_persistent_grid(void) {
     require_init("nvshmemx_batch_persistent_grid");

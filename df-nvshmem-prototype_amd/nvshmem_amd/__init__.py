@@ -98,6 +98,7 @@ SIGNATURES = {
     "nvshmemh_stream_read": (None, [_P, _SZ, _P, _P]),
     "nvshmemh_stream_write": (None, [_P, _SZ, _I, _P]),
     "nvshmemh_copy_stamped": (_I, [_P, _P, _SZ, _SZ, _I, _P, _SZ, _P]),
+    "nvshmemh_post_nbi": (None, [_P, _P, _I, _P]),
     "nvshmemh_pcg32_host": (None, [_P, _SZ, _U64, _U64]),
     "nvshmemh_thread_putget": (None, [_I, _I, _P, _P, _I, _I, _I, _P]),
     "nvshmemh_coop_putget": (None, [_I, _I, _P, _P, _SZ, _SZ, _I, _I, _P]),
@@ -149,6 +150,8 @@ SIGNATURES.update({
     "nvshmemx_batch_persistent_grid": (_I, []),
     "nvshmemx_batch_put_persistent_noalloc_on_stream": (_I, [_P, _I, _P]),
     "nvshmemx_batch_persistent_max_requests_noalloc": (_I, []),
+    "nvshmemx_nbi_flush_on_stream": (_I, [_P]),
+    "nvshmemx_nbi_capacity": (_I, []),
     "nvshmemx_tune_candidate_count": (_I, []),
     "nvshmemx_tune_candidate": (None, [_I, _P]),
     "nvshmemx_align_class": (_I, [_P, _P]),

@@ -222,6 +222,20 @@ int nvshmemx_batch_persistent_grid(void);
  * that overlap on one GPU race.  Returns 0. */
 int nvshmemx_batch_put_persistent_noalloc_on_stream(const nvshmemx_put_request_t *reqs, int nreq, hipStream_t stream);
 int nvshmemx_batch_persistent_max_requests_noalloc(void);
+/* No counterpart in the reference.  Move every put posted on the device with
+ * nvshmemx_putmem_nbi (nvshmemx_device.h) since the last flush: one launch
+ * of the persistent kernel on `stream`, in which each local PE's leader
+ * workgroup plans its own PE's queue (or copies it itself when it is small)
+ * and every workgroup copies chunks of all queues in PE order.  Stream-order
+ * it after the kernels that posted and before anything that reads the
+ * targets; kernels that post while a flush runs race with it.  The queues
+ * and the flush's scratch are static per-GPU arrays of the library (3.25 MiB
+ * together): a flush allocates nothing and can be captured in a hipGraph.
+ * Returns 0. */
+int nvshmemx_nbi_flush_on_stream(hipStream_t stream);
+/* posts a PE's queue holds between flushes (65,536 / local PEs in local
+ * mode, 65,536 in process mode); later posts are copied as they are made */
+int nvshmemx_nbi_capacity(void);
 
 /* ---- exhaustive autotuner of batch choices ---- */
 

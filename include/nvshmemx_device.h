@@ -47,6 +47,15 @@ __device__ void nvshmemx_int_get_wave(int *target, const int *source, int len, i
 __device__ void nvshmemx_float_get_wave(float *target, const float *source, int len, int pe);
 __device__ void nvshmemx_double_get_wave(double *target, const double *source, int len, int pe);
 
+/* Non-blocking put (per thread, like nvshmem_T_put): posts the put to the
+ * calling PE's device queue and returns; nvshmemx_nbi_flush_on_stream (host)
+ * moves every posted put in one persistent launch, stream-ordered after the
+ * kernels that posted.  A put is complete when that flush is; a post that
+ * finds its PE's queue full (nvshmemx_nbi_capacity() posts since the last
+ * flush) is copied at once.  Destinations of puts posted between two flushes
+ * must not overlap.  No counterpart in the reference. */
+__device__ void nvshmemx_putmem_nbi(void *target, const void *source, size_t nbytes, int pe);
+
 /* Workgroup-cooperative copies: every thread of the workgroup calls; the
  * call ends with each thread's stores released at agent scope and a
  * workgroup barrier, so any thread may signal completion afterwards. */

@@ -33,6 +33,7 @@ _GPU_ORDER = [
     "test_gpu_batch.py",
     "test_gpu_config4.py",
     "test_gpu_persistent.py",
+    "test_gpu_nbi.py",
     "test_gpu_jacobi.py",
     "test_gpu_costmodel.py",
     "test_gpu_learned.py",

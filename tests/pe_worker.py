@@ -19,6 +19,8 @@ Scenarios (every PE runs the same collective sequence):
            landing checked against the oracle with guard bytes
   persistent  like batch, through the persistent kernel (requests in device
            memory, each process's kernel the only PE of its grid)
+  nbi      like persistent, the puts posted with nvshmemx_putmem_nbi from a
+           kernel of the process and moved by nvshmemx_nbi_flush_on_stream
   barrier_timeout  PE 0 alone enters the offload barrier with a 50 ms budget;
            its next host barrier must end the process
 """
@@ -137,10 +139,11 @@ def main():
         out["batch_ok"] = bool(np.array_equal(hip.dtoh(base + SRC, DST), imgs[me][SRC:]))
         out["requests_issued"] = k
         lib.nvshmem_barrier_all()
-    elif scenario == "persistent":
+    elif scenario in ("persistent", "nbi"):
         # every PE issues its share of a shared uneven-put batch (sources in
         # its own heap, targets in the others' IPC-mapped heaps) through the
-        # persistent kernel; each PE checks its destinations against the oracle
+        # persistent kernel, or posts it from a kernel of its own and flushes
+        # (nbi); each PE checks its destinations against the oracle
         import oracle_py
         orc = oracle_py.Oracle()
         SRC, DST = 4 << 20, 24 << 20
@@ -157,7 +160,11 @@ def main():
         if raw.size:
             hip.htod(d_reqs, raw)
         lib.nvshmem_barrier_all()
-        assert lib.nvshmemx_batch_put_persistent_on_stream(d_reqs, len(arr), None) == 0
+        if scenario == "nbi":
+            lib.nvshmemh_post_nbi(d_reqs, None, len(arr), None)
+            assert lib.nvshmemx_nbi_flush_on_stream(None) == 0
+        else:
+            assert lib.nvshmemx_batch_put_persistent_on_stream(d_reqs, len(arr), None) == 0
         hip.sync()
         lib.nvshmem_barrier_all()
         imgs = [np.zeros(SRC + DST, np.uint8) for _ in range(n)]

@@ -130,6 +130,7 @@ _BEFORE_INIT = {
     "nvshmem_barrier_all_offload": "lib.nvshmem_barrier_all_offload()",
     "nvshmemx_putmem_on_stream": "lib.nvshmemx_putmem_on_stream(None, None, 4, 0, None)",
     "nvshmemx_batch_plan_create": "lib.nvshmemx_batch_plan_create(None, 0, None, None, None)",
+    "nvshmemx_nbi_flush_on_stream": "lib.nvshmemx_nbi_flush_on_stream(None)",
     "nvstop_pes": "lib.nvstop_pes()",
 }
 
