@@ -223,6 +223,14 @@ extern "C" __device__ void nvshmemx_putmem_nbi(void *target, const void *source,
     }
 }
 
+#define NVSHMEMI_DEF_PUT_NBI(Name, Type)                                                          \
+    extern "C" __device__ void nvshmemx_##Name##_put_nbi(Type *target, const Type *source, int len, int pe) { \
+        if (len > 0) nvshmemx_putmem_nbi(target, source, (size_t)len * sizeof(Type), pe);             \
+    }
+NVSHMEMI_DEF_PUT_NBI(int, int)
+NVSHMEMI_DEF_PUT_NBI(float, float)
+NVSHMEMI_DEF_PUT_NBI(double, double)
+
 /* --------------------------------------------------------------- waits -- */
 
 extern "C" __device__ void nvshmem_int_wait(int *ivar, int cmp_value) {

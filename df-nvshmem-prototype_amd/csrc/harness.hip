@@ -477,6 +477,15 @@ __global__ void k_post_nbi(const nvshmemx_put_request_t *reqs, const int *issuer
     }
 }
 
+/* one typed non-blocking put posted by thread 0 of workgroup 0 (PE 0 in
+ * local mode): dtype 0 int, 1 float, 2 double */
+__global__ void k_post_nbi_typed(int dtype, void *target, const void *source, int len, int pe) {
+    if (threadIdx.x || blockIdx.x) return;
+    if (dtype == 0) nvshmemx_int_put_nbi((int *)target, (const int *)source, len, pe);
+    else if (dtype == 1) nvshmemx_float_put_nbi((float *)target, (const float *)source, len, pe);
+    else nvshmemx_double_put_nbi((double *)target, (const double *)source, len, pe);
+}
+
 } // namespace
 
 /* --------------------------------------------------------- C entry points -- */
@@ -744,6 +753,11 @@ void nvshmemh_post_nbi(const nvshmemx_put_request_t *reqs, const int *issuer, in
     const int wgs = local ? nvshmemx_wgs_per_pe() : 4;
     const int grid = local ? nvshmem_n_pes() * wgs : wgs;
     hipLaunchKernelGGL(k_post_nbi, dim3((unsigned)grid), dim3(256), 0, s, reqs, local ? issuer : nullptr, n, wgs);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+void nvshmemh_post_nbi_typed(int dtype, void *target, const void *source, int len, int pe, hipStream_t s) {
+    hipLaunchKernelGGL(k_post_nbi_typed, dim3(1), dim3(64), 0, s, dtype, target, source, len, pe);
     NVSHMEMI_HIP(hipGetLastError());
 }
 

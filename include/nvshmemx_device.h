@@ -55,6 +55,10 @@ __device__ void nvshmemx_double_get_wave(double *target, const double *source, i
  * flush) is copied at once.  Destinations of puts posted between two flushes
  * must not overlap.  No counterpart in the reference. */
 __device__ void nvshmemx_putmem_nbi(void *target, const void *source, size_t nbytes, int pe);
+/* the same for `len` elements (nothing for len <= 0) */
+__device__ void nvshmemx_int_put_nbi(int *target, const int *source, int len, int pe);
+__device__ void nvshmemx_float_put_nbi(float *target, const float *source, int len, int pe);
+__device__ void nvshmemx_double_put_nbi(double *target, const double *source, int len, int pe);
 
 /* Workgroup-cooperative copies: every thread of the workgroup calls; the
  * call ends with each thread's stores released at agent scope and a

@@ -10,8 +10,8 @@ Covered: the uneven-put workload, byte-granular and zero-byte requests,
 queues small enough for their leaders to copy, a queue that overflows its
 share (the overflow is copied as it is posted), posts refused for a bad PE
 or a target outside the heap, repeated post/flush rounds, a flush with
-nothing posted, a captured post+flush replayed, and two processes posting
-into each other's IPC-mapped heaps."""
+nothing posted, the typed int/float/double posts, a captured post+flush
+replayed, and two processes posting into each other's IPC-mapped heaps."""
 import ctypes as C
 
 import numpy as np
@@ -167,6 +167,26 @@ def test_refused_posts_and_empty_flush(setup, hip, oracle):
     lib.nvshmemx_clear_errors()
     assert err & na.ERR_BAD_PE and err & na.ERR_BAD_ADDRESS, hex(err)
     _check(lib, hip, base, _expected(oracle, images, {k: v[2:] for k, v in reqs.items()}))
+
+
+@pytest.mark.parametrize("dtype,width", [(0, 4), (1, 4), (2, 8)])
+def test_typed_posts(setup, hip, oracle, dtype, width):
+    """nvshmemx_{int,float,double}_put_nbi: len elements of the type, posted
+    by PE 0 into PE 5's destination region (0, 1 and 777 elements, the last
+    at an odd element offset), land after the flush; len 0 posts nothing"""
+    lib, base, images = setup
+    _reset(lib, hip, base)
+    want = [im.copy() for im in images]
+    src_off, dst_off = 4096 + 3 * width, 100 * width
+    for k, length in enumerate((0, 1, 777)):
+        t = base + SRC + dst_off + k * 8192
+        lib.nvshmemh_post_nbi_typed(dtype, t, lib.nvshmem_ptr(base, 0) + src_off, length, 5, None)
+        want[5][SRC + dst_off + k * 8192:SRC + dst_off + k * 8192 + length * width] = \
+            images[0][src_off:src_off + length * width]
+    assert lib.nvshmemx_nbi_flush_on_stream(None) == 0
+    hip.sync()
+    assert lib.nvshmemx_get_last_error() == 0
+    _check(lib, hip, base, want)
 
 
 def test_captured_post_and_flush_replays(setup, hip, oracle):
