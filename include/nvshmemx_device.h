@@ -53,7 +53,8 @@ __device__ void nvshmemx_double_get_wave(double *target, const double *source, i
  * kernels that posted.  A put is complete when that flush is; a post that
  * finds its PE's queue full (nvshmemx_nbi_capacity() posts since the last
  * flush) is copied at once.  Destinations of puts posted between two flushes
- * must not overlap.  No counterpart in the reference. */
+ * must not overlap.  Posts still waiting when the library stops are dropped
+ * (a new start begins with empty queues).  No counterpart in the reference. */
 __device__ void nvshmemx_putmem_nbi(void *target, const void *source, size_t nbytes, int pe);
 /* the same for `len` elements (nothing for len <= 0) */
 __device__ void nvshmemx_int_put_nbi(int *target, const int *source, int len, int pe);
