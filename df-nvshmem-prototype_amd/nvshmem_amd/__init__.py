@@ -379,7 +379,7 @@ class CopyConfig(C.Structure):
 def load(trace: bool = False, variant: str = "") -> C.CDLL:
     path = lib_path(trace, variant)
     if not path.exists():
-        raise LibraryMissing(f"{path} is not built; run __graft_entry__.build() (make -C {PKG_DIR})")
+        raise LibraryMissing(f"{path} is not built; run make -C {PKG_DIR}")
     lib = C.CDLL(str(path), mode=C.RTLD_GLOBAL)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name, None)
