@@ -1,6 +1,7 @@
 """The reference's caller pattern as drop-in MPI programs (examples/*.hip,
 linked against libnvshmem.a with relocatable device code), run as 2 MPI
-processes on the one GPU."""
+processes on the one GPU; and the same build of a program whose own kernel
+posts non-blocking puts that the host then flushes."""
 import re
 import subprocess
 from pathlib import Path
@@ -43,3 +44,14 @@ def test_dropin_sync_latency():
 def test_dropin_wrong_pe_count_is_reported():
     p = mpirun("shmem_sync_latency", n=1)
     assert p.returncode != 0 and "exactly two processes" in p.stderr
+
+
+def test_dropin_posted_puts():
+    """a user kernel, linked against libnvshmem.a, posts 903 puts into the
+    other process's heap with nvshmemx_putmem_nbi; after the flush and the
+    barrier each process finds every byte, and zeros past the last put"""
+    p = mpirun("shmem_nbi_exchange")
+    assert p.returncode == 0, p.stdout + p.stderr[-2000:]
+    ok = [l for l in p.stdout.splitlines() if l.endswith(", ok")]
+    assert len(ok) == 2 and all("903 puts, 1048512 bytes received" in l for l in ok), p.stdout
+    print(p.stdout)
