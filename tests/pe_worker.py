@@ -161,6 +161,7 @@ def main():
             hip.htod(d_reqs, raw)
         lib.nvshmem_barrier_all()
         if scenario == "nbi":
+            out["nbi_capacity"] = lib.nvshmemx_nbi_capacity()
             lib.nvshmemh_post_nbi(d_reqs, None, len(arr), None)
             assert lib.nvshmemx_nbi_flush_on_stream(None) == 0
         else:

@@ -228,3 +228,4 @@ def test_posted_puts_between_processes():
     outs = run_job(2, "nbi", 300)
     for o in outs:
         assert o["errors"] == 0 and o["batch_ok"], o
+        assert o["nbi_capacity"] == 65536, o  # a process holds one PE: the whole queue is its
