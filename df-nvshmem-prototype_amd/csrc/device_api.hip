@@ -195,14 +195,16 @@ NVSHMEMI_DEF_GET(double, double, NVSHMEMI_OP_GET_DOUBLE)
 
 /* ---------------------------------------------- non-blocking posts -- */
 
-/* Post a put for the next nvshmemx_nbi_flush_on_stream: one slot of the
- * calling PE's queue (the workgroup's PE in local mode, the process's in
- * process mode), claimed with one atomic.  A post that finds the queue's
- * share full (or a workgroup of no PE's) is copied at once by the calling
- * lane instead, which a non-blocking put may do.  Invalid targets are
- * refused here, as for the other device calls; zero-byte posts do nothing. */
-extern "C" __device__ void nvshmemx_putmem_nbi(void *target, const void *source, size_t nbytes, int pe) {
-    if (bad_target(target, nbytes, pe) || nbytes == 0) return;
+/* Post a put (or get) for the next nvshmemx_nbi_flush_on_stream: one slot
+ * of the calling PE's queue (the workgroup's PE in local mode, the
+ * process's in process mode), claimed with one atomic.  A post that finds
+ * the queue's share full (or a workgroup of no PE's) is copied at once by
+ * the calling lane instead, which a non-blocking call may do.  Invalid
+ * symmetric ranges are refused here, as for the other device calls;
+ * zero-byte posts do nothing. */
+namespace {
+__device__ __forceinline__ void post_nbi(void *target, const void *source, size_t nbytes, int pe, bool get) {
+    if (bad_target(get ? source : target, nbytes, pe) || nbytes == 0) return;
     const nvshmemi_device_state_t &s = nvshmemi_dstate;
     const bool local = s.mode == NVSHMEMI_MODE_LOCAL;
     const unsigned queues = local ? (unsigned)s.npes : 1u, q = local ? (unsigned)my_pe() : 0u;
@@ -217,10 +219,20 @@ extern "C" __device__ void nvshmemx_putmem_nbi(void *target, const void *source,
         r.source = source;
         r.nbytes = nbytes;
         r.pe = pe;
-        r.elem_bytes = 1;
+        r.elem_bytes = get ? kNbiGet : 1;
+    } else if (get) {
+        lane_copy(target, translate(source, pe), nbytes);
     } else {
         lane_copy(translate(target, pe), source, nbytes);
     }
+}
+} // namespace
+
+extern "C" __device__ void nvshmemx_putmem_nbi(void *target, const void *source, size_t nbytes, int pe) {
+    post_nbi(target, source, nbytes, pe, false);
+}
+extern "C" __device__ void nvshmemx_getmem_nbi(void *target, const void *source, size_t nbytes, int pe) {
+    post_nbi(target, source, nbytes, pe, true);
 }
 
 #define NVSHMEMI_DEF_PUT_NBI(Name, Type)                                                          \
@@ -230,6 +242,13 @@ extern "C" __device__ void nvshmemx_putmem_nbi(void *target, const void *source,
 NVSHMEMI_DEF_PUT_NBI(int, int)
 NVSHMEMI_DEF_PUT_NBI(float, float)
 NVSHMEMI_DEF_PUT_NBI(double, double)
+#define NVSHMEMI_DEF_GET_NBI(Name, Type)                                                          \
+    extern "C" __device__ void nvshmemx_##Name##_get_nbi(Type *target, const Type *source, int len, int pe) { \
+        if (len > 0) nvshmemx_getmem_nbi(target, source, (size_t)len * sizeof(Type), pe);             \
+    }
+NVSHMEMI_DEF_GET_NBI(int, int)
+NVSHMEMI_DEF_GET_NBI(float, float)
+NVSHMEMI_DEF_GET_NBI(double, double)
 
 /* --------------------------------------------------------------- waits -- */
 

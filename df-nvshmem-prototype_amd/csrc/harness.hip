@@ -466,24 +466,35 @@ __global__ void __launch_bounds__(256) k_copy_stamped(char *dst, const char *src
 /* Workgroup-partitioned PEs post puts with nvshmemx_putmem_nbi: workgroup w
  * belongs to the PE nvshmemx_my_pe_d() names, and the `wgs` workgroups of a
  * PE share the requests it issued (issuer[i] == its PE; a null issuer array
- * gives every request to the caller's PE). */
-__global__ void k_post_nbi(const nvshmemx_put_request_t *reqs, const int *issuer, int n, int wgs) {
+ * gives every request to the caller's PE).  A request with kind[i] != 0 is
+ * posted instead as the get that moves the same bytes: by the target's PE,
+ * from the source in PE issuer[i]'s heap (local mode). */
+__global__ void k_post_nbi(const nvshmemx_put_request_t *reqs, const int *issuer, const int *kind, int n, int wgs) {
     const int pe = nvshmemx_my_pe_d();
     const int sub = (int)(blockIdx.x % (unsigned)wgs);
     for (int i = sub * (int)blockDim.x + (int)threadIdx.x; i < n; i += wgs * (int)blockDim.x) {
-        if (issuer && issuer[i] != pe) continue;
         const nvshmemx_put_request_t r = reqs[i];
-        nvshmemx_putmem_nbi(r.target, r.source, r.nbytes, r.pe);
+        if (kind && kind[i]) { /* the same bytes fetched by the target's PE from the issuer's source */
+            if (r.pe == pe) nvshmemx_getmem_nbi(nvshmemx_ptr_d(r.target, r.pe), r.source, r.nbytes, issuer[i]);
+        } else if (!issuer || issuer[i] == pe) {
+            nvshmemx_putmem_nbi(r.target, r.source, r.nbytes, r.pe);
+        }
     }
 }
 
-/* one typed non-blocking put posted by thread 0 of workgroup 0 (PE 0 in
- * local mode): dtype 0 int, 1 float, 2 double */
-__global__ void k_post_nbi_typed(int dtype, void *target, const void *source, int len, int pe) {
+/* one typed non-blocking put (get: a get from `pe`) posted by thread 0 of
+ * workgroup 0 (PE 0 in local mode): dtype 0 int, 1 float, 2 double */
+__global__ void k_post_nbi_typed(int dtype, int get, void *target, const void *source, int len, int pe) {
     if (threadIdx.x || blockIdx.x) return;
-    if (dtype == 0) nvshmemx_int_put_nbi((int *)target, (const int *)source, len, pe);
-    else if (dtype == 1) nvshmemx_float_put_nbi((float *)target, (const float *)source, len, pe);
-    else nvshmemx_double_put_nbi((double *)target, (const double *)source, len, pe);
+    if (get) {
+        if (dtype == 0) nvshmemx_int_get_nbi((int *)target, (const int *)source, len, pe);
+        else if (dtype == 1) nvshmemx_float_get_nbi((float *)target, (const float *)source, len, pe);
+        else nvshmemx_double_get_nbi((double *)target, (const double *)source, len, pe);
+    } else {
+        if (dtype == 0) nvshmemx_int_put_nbi((int *)target, (const int *)source, len, pe);
+        else if (dtype == 1) nvshmemx_float_put_nbi((float *)target, (const float *)source, len, pe);
+        else nvshmemx_double_put_nbi((double *)target, (const double *)source, len, pe);
+    }
 }
 
 } // namespace
@@ -752,12 +763,26 @@ void nvshmemh_post_nbi(const nvshmemx_put_request_t *reqs, const int *issuer, in
     if (local && !issuer) NVSHMEMI_FATAL("nvshmemh_post_nbi: local mode needs the issuing PE of every request");
     const int wgs = local ? nvshmemx_wgs_per_pe() : 4;
     const int grid = local ? nvshmem_n_pes() * wgs : wgs;
-    hipLaunchKernelGGL(k_post_nbi, dim3((unsigned)grid), dim3(256), 0, s, reqs, local ? issuer : nullptr, n, wgs);
+    hipLaunchKernelGGL(k_post_nbi, dim3((unsigned)grid), dim3(256), 0, s, reqs, local ? issuer : nullptr,
+                       (const int *)nullptr, n, wgs);
     NVSHMEMI_HIP(hipGetLastError());
 }
 
-void nvshmemh_post_nbi_typed(int dtype, void *target, const void *source, int len, int pe, hipStream_t s) {
-    hipLaunchKernelGGL(k_post_nbi_typed, dim3(1), dim3(64), 0, s, dtype, target, source, len, pe);
+/* The same in local mode, request i posted as a get by its target's PE where
+ * kind[i] != 0 (device array), as a put by issuer[i] otherwise. */
+void nvshmemh_post_nbi_mixed(const nvshmemx_put_request_t *reqs, const int *issuer, const int *kind, int n,
+                             hipStream_t s) {
+    if (n <= 0) return;
+    if (nvshmemx_mode() != NVSHMEMI_MODE_LOCAL || !issuer || !kind)
+        NVSHMEMI_FATAL("nvshmemh_post_nbi_mixed: local mode, with the issuing PE and kind of every request");
+    const int wgs = nvshmemx_wgs_per_pe();
+    hipLaunchKernelGGL(k_post_nbi, dim3((unsigned)(nvshmem_n_pes() * wgs)), dim3(256), 0, s, reqs, issuer, kind, n,
+                       wgs);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+void nvshmemh_post_nbi_typed(int dtype, int get, void *target, const void *source, int len, int pe, hipStream_t s) {
+    hipLaunchKernelGGL(k_post_nbi_typed, dim3(1), dim3(64), 0, s, dtype, get, target, source, len, pe);
     NVSHMEMI_HIP(hipGetLastError());
 }
 

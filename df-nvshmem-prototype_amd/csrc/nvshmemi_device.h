@@ -36,6 +36,9 @@ namespace nvshmemi {
  * in process mode); nbi_count[q] counts queue q's posts since its last flush
  * (posts past the queue's share were copied at once and are not in it). */
 constexpr unsigned kNbiSlots = 65536;
+/* a posted get is queued as a request whose target is the local address and
+ * whose source is the symmetric one, marked by this element width */
+constexpr int kNbiGet = -1;
 extern __device__ nvshmemx_put_request_t nbi_queue[kNbiSlots];
 extern __device__ unsigned nbi_count[NVSHMEMI_MAX_PES];
 

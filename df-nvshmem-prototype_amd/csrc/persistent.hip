@@ -207,15 +207,16 @@ __device__ __forceinline__ int req_issuer(const nvshmemx_put_request_t &r, unsig
 }
 
 /* granules a request occupies in its PE's stream; 0 (and an error bit, set
- * by leader 0) for a request naming a bad PE or a target range outside the
- * heap, which is then skipped */
-__device__ __forceinline__ unsigned req_units(const nvshmemx_put_request_t &r, bool report) {
+ * by leader 0) for a request naming a bad PE or a symmetric range outside
+ * the heap (the target of a put, the source of a posted get), which is then
+ * skipped */
+__device__ __forceinline__ unsigned req_units(const nvshmemx_put_request_t &r, bool report, bool get = false) {
     const nvshmemi_device_state_t &s = nvshmemi_dstate;
     if ((unsigned)r.pe >= (unsigned)s.npes) {
         if (report) set_error(NVSHMEMI_ERR_BAD_PE);
         return 0;
     }
-    const uint64_t rel = (uint64_t)((const char *)r.target - s.anchor);
+    const uint64_t rel = (uint64_t)((const char *)(get ? r.source : r.target) - s.anchor);
     const uint64_t off = rel & s.offset_mask;
     if (rel >= s.anchor_span || off >= s.heap_bytes || r.nbytes > s.heap_bytes - off) {
         if (report) set_error(NVSHMEMI_ERR_BAD_ADDRESS);
@@ -266,19 +267,24 @@ __device__ __forceinline__ bool leader_served(unsigned cnt, unsigned long long b
  * more by the whole workgroup, in turn; the shorter ones at once, one
  * wavefront each when there are at most 16 of them, else 16 lanes each (at
  * most 64 of them).  Every thread of the workgroup calls. */
+template <bool POSTED>
 __device__ __forceinline__ void copy_own(const nvshmemx_put_request_t *__restrict__ reqs, unsigned cnt, const unsigned *own) {
     const unsigned lane = threadIdx.x & 63u;
     uint64_t tgt = 0, src = 0, nb = 0;
-    unsigned pe = 0;
+    unsigned pe = 0, get = 0;
     if (lane < cnt) {
         const nvshmemx_put_request_t r = reqs[own[lane]];
         tgt = (uint64_t)r.target, src = (uint64_t)r.source, nb = (uint64_t)r.nbytes, pe = (unsigned)r.pe;
+        get = POSTED && r.elem_bytes == kNbiGet;
     }
     for (unsigned j = 0; j < cnt; ++j) {
         const size_t n = (size_t)lane_u64(nb, j);
         if (n < kWaveSegMax) continue;
-        char *dst = translate((const void *)lane_u64(tgt, j), (int)__builtin_amdgcn_readlane(pe, j));
-        copy<16, kPUnroll, NT_STORE>(dst, (const char *)lane_u64(src, j), n, threadIdx.x, kPBlock);
+        const int jpe = (int)__builtin_amdgcn_readlane(pe, j);
+        const bool jget = POSTED && __builtin_amdgcn_readlane(get, j);
+        char *dst = jget ? (char *)lane_u64(tgt, j) : translate((const void *)lane_u64(tgt, j), jpe);
+        const char *srcp = jget ? translate((const void *)lane_u64(src, j), jpe) : (const char *)lane_u64(src, j);
+        copy<16, kPUnroll, NT_STORE>(dst, srcp, n, threadIdx.x, kPBlock);
     }
     uint64_t smask = __ballot(lane < cnt && nb < kWaveSegMax);
     const unsigned nsmall = (unsigned)__builtin_popcountll(smask);
@@ -292,8 +298,10 @@ __device__ __forceinline__ void copy_own(const nvshmemx_put_request_t *__restric
     const uint64_t js = ((uint64_t)(unsigned)__shfl((int)(src >> 32), j, 64) << 32) | (unsigned)__shfl((int)src, j, 64);
     const size_t n = (size_t)(unsigned)__shfl((int)nb, j, 64); /* below 16 KiB */
     const int jpe = __shfl((int)pe, j, 64);
-    if (rank < nsmall) copy<16, kPUnroll, NT_STORE>(translate((const void *)jt, jpe), (const char *)js, n,
-                                                   threadIdx.x % group, group);
+    const bool jget = POSTED && __shfl((int)get, j, 64);
+    char *dst = jget ? (char *)jt : translate((const void *)jt, jpe);
+    const char *srcp = jget ? translate((const void *)js, jpe) : (const char *)js;
+    if (rank < nsmall) copy<16, kPUnroll, NT_STORE>(dst, srcp, n, threadIdx.x % group, group);
 }
 
 /* leader of PE p: build and publish the PE's queue, or, for a PE the leader
@@ -393,7 +401,7 @@ __device__ void lead(const nvshmemx_put_request_t *__restrict__ reqs, unsigned n
         *S.info(p) = s_me;
         store_release(S.ready(p), (unsigned)(ticket / gridDim.x) + 1u); /* writes back this XCD's L2 first */
     }
-    if (self) copy_own(reqs, s_cnt[p], s_own);
+    if (self) copy_own<false>(reqs, s_cnt[p], s_own);
 }
 
 /* Leader of PE p in a flush of posted puts: its queue is slots
@@ -423,7 +431,7 @@ __device__ void lead_posted(int P, int p, const pscr &S, unsigned long long tick
     const unsigned n = s_n;
     for (unsigned i = threadIdx.x; i < n; i += kPBlock) {
         const nvshmemx_put_request_t r = q[i];
-        const unsigned u = req_units(r, true);
+        const unsigned u = req_units(r, true, r.elem_bytes == kNbiGet);
         if (u) {
             atomicAdd(&s_cnt, 1u);
             atomicAdd(&s_units, (unsigned long long)u);
@@ -450,7 +458,7 @@ __device__ void lead_posted(int P, int p, const pscr &S, unsigned long long tick
     unsigned rc = 0, ru = 0; /* entries and granules of earlier blocks */
     for (unsigned b = 0; b < n; b += kPBlock) {
         const unsigned i = b + threadIdx.x;
-        const unsigned u = i < n ? req_units(q[i], false) : 0u;
+        const unsigned u = i < n ? req_units(q[i], false, q[i].elem_bytes == kNbiGet) : 0u;
         unsigned xo, xu, to, tu;
         block_scan2(u ? 1u : 0u, u, xo, xu, to, tu, s_w);
         if (u) {
@@ -471,7 +479,7 @@ __device__ void lead_posted(int P, int p, const pscr &S, unsigned long long tick
         *S.info(p) = s_me;
         store_release(S.ready(p), (unsigned)(ticket / gridDim.x) + 1u); /* writes back this XCD's L2 first */
     }
-    if (self) copy_own(nbi_queue, s_cnt, s_own);
+    if (self) copy_own<true>(nbi_queue, s_cnt, s_own);
 }
 
 /* a workgroup copies chunk k of a queue: a request segment of kWaveSegMax
@@ -479,6 +487,7 @@ __device__ void lead_posted(int P, int p, const pscr &S, unsigned long long tick
  * turn.  Entries are read 64 at a time, one per lane (the entry, then its
  * request), and then taken in order from the lanes, so a chunk of many small
  * requests waits for two memory round trips per 64 entries, not per entry. */
+template <bool POSTED>
 __device__ __forceinline__ void run_chunk(const nvshmemx_put_request_t *__restrict__ reqs, const pinfo &inf, unsigned k,
                                           const pscr &S) {
     const uint64_t c_lo = (uint64_t)k * inf.chunk_units * kUnit, c_hi = c_lo + (uint64_t)inf.chunk_units * kUnit;
@@ -487,13 +496,14 @@ __device__ __forceinline__ void run_chunk(const nvshmemx_put_request_t *__restri
     unsigned small = 0;
     for (unsigned e = coherent_u32(S.table() + inf.chunk0 + k); e < e_end; e += 64) {
         uint64_t vs = ~0ull, tgt = 0, src = 0, nb = 0;
-        unsigned pe = 0;
+        unsigned pe = 0, get = 0;
         if (e + lane < e_end) {
             const pentry ent = coherent_entry(S.entries() + e + lane);
             vs = (uint64_t)ent.vstart * kUnit;
             if (vs < c_hi) {
                 const nvshmemx_put_request_t r = reqs[ent.req];
                 tgt = (uint64_t)r.target, src = (uint64_t)r.source, nb = (uint64_t)r.nbytes, pe = (unsigned)r.pe;
+                get = POSTED && r.elem_bytes == kNbiGet;
             }
         }
         /* entries start in increasing order, so the lanes in the chunk are a prefix */
@@ -504,8 +514,11 @@ __device__ __forceinline__ void run_chunk(const nvshmemx_put_request_t *__restri
             const uint64_t end = jvs + jnb;
             const uint64_t hi = (c_hi < end ? c_hi : end) - jvs;
             if (hi <= lo) continue; /* the chunk starts in the request's padding */
-            char *dst = translate((const void *)lane_u64(tgt, j), (int)__builtin_amdgcn_readlane(pe, j)) + lo;
-            const char *srcp = (const char *)lane_u64(src, j) + lo;
+            const int jpe = (int)__builtin_amdgcn_readlane(pe, j);
+            const bool jget = POSTED && __builtin_amdgcn_readlane(get, j); /* a posted get: the source is remote */
+            char *dst = (jget ? (char *)lane_u64(tgt, j) : translate((const void *)lane_u64(tgt, j), jpe)) + lo;
+            const char *srcp = (jget ? translate((const void *)lane_u64(src, j), jpe)
+                                     : (const char *)lane_u64(src, j)) + lo;
             const size_t n = (size_t)(hi - lo);
             if (n >= kWaveSegMax) {
                 copy<16, kPUnroll, NT_STORE>(dst, srcp, n, threadIdx.x, kPBlock);
@@ -609,7 +622,7 @@ __device__ __forceinline__ void drain(const nvshmemx_put_request_t *__restrict__
             s_qk[(it + 1) & 1] = nk < total ? owner(nk) : -1;
         }
         const pinfo &inf = s_inf[q];
-        if (k - s_g0[q] < inf.nchunks) run_chunk(reqs, inf, k - s_g0[q], S);
+        if (k - s_g0[q] < inf.nchunks) run_chunk<POSTED>(reqs, inf, k - s_g0[q], S);
         __syncthreads();
         ++it;
     }

@@ -99,7 +99,8 @@ SIGNATURES = {
     "nvshmemh_stream_write": (None, [_P, _SZ, _I, _P]),
     "nvshmemh_copy_stamped": (_I, [_P, _P, _SZ, _SZ, _I, _P, _SZ, _P]),
     "nvshmemh_post_nbi": (None, [_P, _P, _I, _P]),
-    "nvshmemh_post_nbi_typed": (None, [_I, _P, _P, _I, _I, _P]),
+    "nvshmemh_post_nbi_typed": (None, [_I, _I, _P, _P, _I, _I, _P]),
+    "nvshmemh_post_nbi_mixed": (None, [_P, _P, _P, _I, _P]),
     "nvshmemh_pcg32_host": (None, [_P, _SZ, _U64, _U64]),
     "nvshmemh_thread_putget": (None, [_I, _I, _P, _P, _I, _I, _I, _P]),
     "nvshmemh_coop_putget": (None, [_I, _I, _P, _P, _SZ, _SZ, _I, _I, _P]),

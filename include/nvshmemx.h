@@ -222,8 +222,9 @@ int nvshmemx_batch_persistent_grid(void);
  * that overlap on one GPU race.  Returns 0. */
 int nvshmemx_batch_put_persistent_noalloc_on_stream(const nvshmemx_put_request_t *reqs, int nreq, hipStream_t stream);
 int nvshmemx_batch_persistent_max_requests_noalloc(void);
-/* No counterpart in the reference.  Move every put posted on the device with
- * nvshmemx_putmem_nbi (nvshmemx_device.h) since the last flush: one launch
+/* No counterpart in the reference.  Move every put and get posted on the
+ * device with nvshmemx_putmem_nbi / nvshmemx_getmem_nbi (nvshmemx_device.h)
+ * since the last flush: one launch
  * of the persistent kernel on `stream`, in which each local PE's leader
  * workgroup plans its own PE's queue (or copies it itself when it is small)
  * and every workgroup copies chunks of all queues in PE order.  Stream-order

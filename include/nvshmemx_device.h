@@ -60,6 +60,14 @@ __device__ void nvshmemx_putmem_nbi(void *target, const void *source, size_t nby
 __device__ void nvshmemx_int_put_nbi(int *target, const int *source, int len, int pe);
 __device__ void nvshmemx_float_put_nbi(float *target, const float *source, int len, int pe);
 __device__ void nvshmemx_double_put_nbi(double *target, const double *source, int len, int pe);
+/* Non-blocking get: posts the copy of `nbytes` of symmetric `source` on `pe`
+ * into the caller's local `target` to the same queue; complete, like the
+ * posted puts, when the flush is (copied at once if the queue is full).
+ * Targets of gets and puts posted between two flushes must not overlap. */
+__device__ void nvshmemx_getmem_nbi(void *target, const void *source, size_t nbytes, int pe);
+__device__ void nvshmemx_int_get_nbi(int *target, const int *source, int len, int pe);
+__device__ void nvshmemx_float_get_nbi(float *target, const float *source, int len, int pe);
+__device__ void nvshmemx_double_get_nbi(double *target, const double *source, int len, int pe);
 
 /* Workgroup-cooperative copies: every thread of the workgroup calls; the
  * call ends with each thread's stores released at agent scope and a

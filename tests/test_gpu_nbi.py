@@ -10,8 +10,10 @@ Covered: the uneven-put workload, byte-granular and zero-byte requests,
 queues small enough for their leaders to copy, a queue that overflows its
 share (the overflow is copied as it is posted), posts refused for a bad PE
 or a target outside the heap, repeated post/flush rounds, a flush with
-nothing posted, the typed int/float/double posts, a captured post+flush
-replayed, and two processes posting into each other's IPC-mapped heaps."""
+nothing posted, the typed int/float/double posts, gets posted beside puts
+(nvshmemx_getmem_nbi, the target's PE fetching the same bytes), a captured
+post+flush replayed, and two processes posting into each other's
+IPC-mapped heaps."""
 import ctypes as C
 
 import numpy as np
@@ -63,28 +65,38 @@ def _expected(oracle, images, reqs):
 
 
 class _Posted:
-    """the request array and its issuing PEs in device memory"""
+    """the request array, its issuing PEs and (optionally) which requests go
+    as gets, in device memory"""
 
-    def __init__(self, lib, hip, base, reqs):
+    def __init__(self, lib, hip, base, reqs, gets=None):
         self.arr = na.make_requests(lib, reqs, base, 0, SRC)
         self.n = len(self.arr)
         self.hip = hip
         self.d = device_requests(hip, self.arr)
-        iss = np.ascontiguousarray(reqs["src_pe"], np.int32)
-        self.iss = hip.malloc(max(4, iss.nbytes))
-        if iss.nbytes:
-            hip.htod(self.iss, iss.view(np.uint8))
+        self.iss = self._dev(np.ascontiguousarray(reqs["src_pe"], np.int32))
+        self.kind = self._dev(np.ascontiguousarray(gets, np.int32)) if gets is not None else None
+
+    def _dev(self, a):
+        d = self.hip.malloc(max(4, a.nbytes))
+        if a.nbytes:
+            self.hip.htod(d, a.view(np.uint8))
+        return d
 
     def post(self, lib, stream=None):
-        lib.nvshmemh_post_nbi(self.d, self.iss, self.n, stream)
+        if self.kind is None:
+            lib.nvshmemh_post_nbi(self.d, self.iss, self.n, stream)
+        else:
+            lib.nvshmemh_post_nbi_mixed(self.d, self.iss, self.kind, self.n, stream)
 
     def close(self):
         self.hip.free(self.d)
         self.hip.free(self.iss)
+        if self.kind is not None:
+            self.hip.free(self.kind)
 
 
-def _post_and_flush(lib, hip, base, reqs, rounds=1):
-    p = _Posted(lib, hip, base, reqs)
+def _post_and_flush(lib, hip, base, reqs, rounds=1, gets=None):
+    p = _Posted(lib, hip, base, reqs, gets)
     try:
         for _ in range(rounds):
             p.post(lib)
@@ -121,6 +133,29 @@ def test_posted_puts_match_oracle(setup, hip, oracle, case):
     _check(lib, hip, base, want)
 
 
+@pytest.mark.parametrize("case", ["workload", "ragged", "leader_served"])
+def test_posted_gets_and_puts(setup, hip, oracle, case):
+    """about half the requests posted as the get that moves the same bytes
+    (by the target's PE, from the source's PE) and the rest as puts, all in
+    one flush: the destinations equal the oracle's puts of every request"""
+    lib, base, images = setup
+    rng = np.random.default_rng(101)
+    if case == "workload":
+        reqs = na.gen_requests(lib, 107, 1024, NPES, SRC, DST)
+    elif case == "ragged":
+        reqs = _ragged(rng, 700, 0, sizes=[0, 1, 3, 17, 255, 4095, 4097, 16383, 16385, 70001, 131073, 1 << 20])
+    else:
+        sizes = [[int(rng.integers(0, 1600)) for _ in range(c)] for c in [1, 5, 16, 17, 40, 64, 0, 3]]
+        sizes[1][0], sizes[3][5] = 16384, 70001
+        reqs = _per_pe(rng, sizes)
+    gets = rng.integers(0, 2, len(reqs["nbytes"])).astype(np.int32)
+    want = _expected(oracle, images, reqs)
+    _reset(lib, hip, base)
+    _post_and_flush(lib, hip, base, reqs, gets=gets)
+    assert lib.nvshmemx_get_last_error() == 0
+    _check(lib, hip, base, want)
+
+
 def test_overflowing_queue_is_copied_as_posted(setup, hip, oracle):
     """one PE posts more puts than its share of the queue between two
     flushes: the first share waits for the flush, the rest are copied by the
@@ -145,17 +180,21 @@ def test_overflowing_queue_is_copied_as_posted(setup, hip, oracle):
 
 def test_refused_posts_and_empty_flush(setup, hip, oracle):
     """posts naming a bad PE or a target past the heap are refused (error
-    bits set) and the rest land; a flush with nothing posted moves nothing"""
+    bits set) and the rest land, and so is a get whose source leaves the
+    heap; a flush with nothing posted moves nothing"""
     lib, base, images = setup
     _reset(lib, hip, base)
     assert lib.nvshmemx_nbi_flush_on_stream(None) == 0
     hip.sync()
     _check(lib, hip, base, images)
     reqs = na.gen_requests(lib, 89, 64, NPES, SRC, DST)
-    p = _Posted(lib, hip, base, reqs)
+    gets = np.zeros(len(reqs["nbytes"]), np.int32)
+    gets[2] = 1  # posted as a get whose source is pushed past the heap
+    p = _Posted(lib, hip, base, reqs, gets)
     try:
         p.arr[0].pe = NPES + 3
         p.arr[1].target = p.arr[1].target + (200 << 20)
+        p.arr[2].source = p.arr[2].source + (200 << 20)
         raw = np.frombuffer(bytes(p.arr), np.uint8)
         hip.htod(p.d, raw)
         p.post(lib)
@@ -166,23 +205,28 @@ def test_refused_posts_and_empty_flush(setup, hip, oracle):
     err = lib.nvshmemx_get_last_error()
     lib.nvshmemx_clear_errors()
     assert err & na.ERR_BAD_PE and err & na.ERR_BAD_ADDRESS, hex(err)
-    _check(lib, hip, base, _expected(oracle, images, {k: v[2:] for k, v in reqs.items()}))
+    _check(lib, hip, base, _expected(oracle, images, {k: v[3:] for k, v in reqs.items()}))
 
 
+@pytest.mark.parametrize("get", [0, 1])
 @pytest.mark.parametrize("dtype,width", [(0, 4), (1, 4), (2, 8)])
-def test_typed_posts(setup, hip, oracle, dtype, width):
-    """nvshmemx_{int,float,double}_put_nbi: len elements of the type, posted
-    by PE 0 into PE 5's destination region (0, 1 and 777 elements, the last
-    at an odd element offset), land after the flush; len 0 posts nothing"""
+def test_typed_posts(setup, hip, oracle, dtype, width, get):
+    """nvshmemx_{int,float,double}_put_nbi / _get_nbi: len elements of the
+    type (0, 1 and 777, the last at an odd element offset), posted by PE 0:
+    puts from PE 0's source into PE 5's destination region, gets from PE 5's
+    source into PE 0's; they land after the flush, len 0 moves nothing"""
     lib, base, images = setup
     _reset(lib, hip, base)
     want = [im.copy() for im in images]
     src_off, dst_off = 4096 + 3 * width, 100 * width
     for k, length in enumerate((0, 1, 777)):
-        t = base + SRC + dst_off + k * 8192
-        lib.nvshmemh_post_nbi_typed(dtype, t, lib.nvshmem_ptr(base, 0) + src_off, length, 5, None)
-        want[5][SRC + dst_off + k * 8192:SRC + dst_off + k * 8192 + length * width] = \
-            images[0][src_off:src_off + length * width]
+        d = SRC + dst_off + k * 8192
+        if get:
+            lib.nvshmemh_post_nbi_typed(dtype, 1, lib.nvshmem_ptr(base, 0) + d, base + src_off, length, 5, None)
+            want[0][d:d + length * width] = images[5][src_off:src_off + length * width]
+        else:
+            lib.nvshmemh_post_nbi_typed(dtype, 0, base + d, lib.nvshmem_ptr(base, 0) + src_off, length, 5, None)
+            want[5][d:d + length * width] = images[0][src_off:src_off + length * width]
     assert lib.nvshmemx_nbi_flush_on_stream(None) == 0
     hip.sync()
     assert lib.nvshmemx_get_last_error() == 0
