@@ -9,6 +9,12 @@
  *   nvshmem_quiet                (nvshmem_device.cu:431-436) -> MPI_Win_flush_all
  *   nvshmem_int_wait             (nvshmem_device.cu:542-562) -> poll local window memory, MPI_Win_sync between reads
  *   nvshmem_barrier_all          (nvshmem.cu:858-867)        -> MPI_Win_flush_all + MPI_Barrier
+ *   nvshmem_barrier_all_offload  (nvshmem_kernels.cu:56-102) -> count on rank 0's window (MPI_Fetch_and_op),
+ *                                  release by rank 0's puts of the episode number (barrier mode)
+ * Every wait gives up after NVSHMEM_WAIT_TIMEOUT_MS (default 10000, the
+ * library's budget): the rank reports the timeout and the job ends with a
+ * non-zero status, where the reference's barrier would continue
+ * (nvshmem_kernels.cu:65-68).
  *
  * Modes (2 ranks):
  *   pingpong ITERS                 p + wait round trips (PE0 p(flag, i); PE1
@@ -26,6 +32,9 @@
  *                                  compares with oracle_jacobi_run
  *   batch NREQ SEED STEPS           the uneven-put workload, 64 logical PEs
  *                                  over the ranks (8 ranks x 8 PEs)
+ *   barrier ROUNDS SKIP             ROUNDS barrier episodes on the window; rank
+ *                                  SKIP (-1: none) leaves out the last one, so
+ *                                  the others' waits must time out
  * Output: one JSON object per line on rank 0.
  */
 #define _GNU_SOURCE
@@ -55,12 +64,73 @@ static double pct(double *v, size_t n, double p) {
     return v[i];
 }
 
+static double g_wait_budget_s = 10.0; /* NVSHMEM_WAIT_TIMEOUT_MS */
+
+/* a wait that outlived the budget: report it and end the whole job */
+static void wait_timed_out(const char *what, long episode) {
+    int rank = 0;
+    MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+    fprintf(stderr, "[mpi_rma_baseline] [%d] %s %ld timed out after %.3f s\n", rank, what, episode, g_wait_budget_s);
+    fflush(stderr);
+    MPI_Abort(MPI_COMM_WORLD, 3);
+}
+
 static void wait_while_equal(MPI_Win win, volatile int *flag, int cmp) {
-    /* nvshmem_int_wait: block while *flag == cmp */
-    for (;;) {
+    /* nvshmem_int_wait: block while *flag == cmp, within the budget */
+    const double t0 = now_s();
+    for (unsigned long n = 0;; ++n) {
         MPI_Win_sync(win);
         if (oracle_int_wait_done(*flag, cmp)) return;
+        if ((n & 255u) == 255u && now_s() - t0 > g_wait_budget_s) wait_timed_out("wait for a value other than", cmp);
     }
+}
+
+static void wait_until_at_least(MPI_Win win, volatile int *word, int value, const char *what, long episode) {
+    const double t0 = now_s();
+    for (unsigned long n = 0;; ++n) {
+        MPI_Win_sync(win);
+        if (*word >= value) return;
+        if ((n & 255u) == 255u && now_s() - t0 > g_wait_budget_s) wait_timed_out(what, episode);
+    }
+}
+
+/* Barrier episodes through the window, the offload barrier's protocol: every
+ * rank adds one to rank 0's count word, rank 0 waits for all of them and
+ * writes the episode number into every rank's release word.  Rank `skip`
+ * leaves out the last episode (a PE that never reaches the barrier). */
+static int do_barrier(int rank, int nranks, long rounds, int skip) {
+    MPI_Win win;
+    int *base = NULL;
+    MPI_Win_allocate_shared(64, sizeof(int), MPI_INFO_NULL, MPI_COMM_WORLD, &base, &win);
+    memset(base, 0, 64);
+    MPI_Win_lock_all(MPI_MODE_NOCHECK, win);
+    MPI_Win_sync(win);
+    MPI_Barrier(MPI_COMM_WORLD);
+    volatile int *count = base, *release = base + 8;
+    const double t0 = now_s();
+    for (long e = 1; e <= rounds; ++e) {
+        if (rank == skip && e == rounds) break;
+        const int one = 1;
+        int old = 0;
+        MPI_Fetch_and_op(&one, &old, MPI_INT, 0, 0, MPI_SUM, win);
+        MPI_Win_flush(0, win);
+        if (rank == 0) {
+            wait_until_at_least(win, count, (int)(e * nranks), "barrier episode (arrivals)", e);
+            const int v = (int)e;
+            for (int r = 0; r < nranks; ++r) MPI_Put(&v, 1, MPI_INT, r, 8, 1, MPI_INT, win);
+            MPI_Win_flush_all(win);
+        }
+        wait_until_at_least(win, release, (int)e, "barrier episode (release)", e);
+    }
+    const double per = (now_s() - t0) / (double)rounds;
+    if (rank == 0) {
+        printf("{\"mode\": \"barrier\", \"ranks\": %d, \"rounds\": %ld, \"episode_us\": %.2f}\n", nranks, rounds,
+               per * 1e6);
+        fflush(stdout);
+    }
+    MPI_Win_unlock_all(win);
+    MPI_Win_free(&win);
+    return 0;
 }
 
 static int do_pingpong(int rank, long iters) {
@@ -404,6 +474,8 @@ int main(int argc, char **argv) {
     int rank, size;
     MPI_Comm_rank(MPI_COMM_WORLD, &rank);
     MPI_Comm_size(MPI_COMM_WORLD, &size);
+    const char *budget = getenv("NVSHMEM_WAIT_TIMEOUT_MS");
+    if (budget && atof(budget) > 0) g_wait_budget_s = atof(budget) * 1e-3;
     int rc = 2;
     const int two_rank_mode = argc >= 2 && (strcmp(argv[1], "pingpong") == 0 || strcmp(argv[1], "putget") == 0);
     if (two_rank_mode && size != 2) {
@@ -414,13 +486,15 @@ int main(int argc, char **argv) {
         rc = do_jacobi(rank, size, atoi(argv[2]), atoi(argv[3]), atoi(argv[4]));
     } else if (argc >= 5 && strcmp(argv[1], "batch") == 0) {
         rc = do_batch(rank, size, atoi(argv[2]), (uint64_t)atoll(argv[3]), atoi(argv[4]));
+    } else if (argc >= 4 && strcmp(argv[1], "barrier") == 0) {
+        rc = do_barrier(rank, size, atol(argv[2]), atoi(argv[3]));
     } else if (argc >= 8 && strcmp(argv[1], "putget") == 0) {
         rc = do_putget(rank, argv[2], argv[3], (size_t)atoll(argv[4]), (size_t)atoll(argv[5]), atol(argv[6]),
                        atof(argv[7]));
     } else if (rank == 0) {
         fprintf(stderr,
                 "usage: %s pingpong ITERS | putget put|get int|float|double MIN MAX REPS BUDGET_S | jacobi N STEPS CHECK "
-                "| batch NREQ SEED STEPS\n",
+                "| batch NREQ SEED STEPS | barrier ROUNDS SKIP\n",
                 argv[0]);
     }
     MPI_Finalize();
