@@ -34,6 +34,7 @@
 #include "nvshmemi_device.h"
 #include "nvshmemi_host.h"
 #include "nvshmemi_trace.h"
+#include "trace_ring.h"
 
 #ifdef NVSHMEM_TRACING
 #include <atomic>
@@ -153,18 +154,10 @@ TraceHost *th = nullptr;
  * holds t->mu.  Returns the number of records taken. */
 unsigned int drain_published(TraceHost *t) {
     const unsigned int cap = t->desc.capacity;
-    unsigned int n = 0;
-    while (n < cap) {
-        const unsigned int idx = t->consumed + n;
-        if (__atomic_load_n(&t->ring[idx & (cap - 1)].seq, __ATOMIC_ACQUIRE) != idx + 1) break;
-        ++n;
-    }
+    const unsigned int n = ring_published(t->ring, cap, t->consumed);
     if (n == 0) return 0;
     t->batch.resize(n);
-    const unsigned int first = t->consumed & (cap - 1);
-    const unsigned int n1 = (first + n <= cap) ? n : cap - first;
-    memcpy(t->batch.data(), t->ring + first, (size_t)n1 * sizeof(nvshmemi_trace_record_t));
-    if (n1 < n) memcpy(t->batch.data() + n1, t->ring, (size_t)(n - n1) * sizeof(nvshmemi_trace_record_t));
+    ring_copy(t->batch.data(), t->ring, cap, t->consumed, n);
     t->consumed += n;
     t->records += n;
     __atomic_store_n(t->host_tail, t->consumed, __ATOMIC_RELEASE); /* the slots are free again */
