@@ -468,6 +468,198 @@ __global__ void __launch_bounds__(256) k_stencil_v(const double *u_sym, double *
     }
 }
 
+/* Two steps per launch (temporal blocking).  A workgroup owns a TJ x 128
+ * tile of rows j and columns k over a chunk of planes i and walks the planes:
+ * u's cells within two of the tile (rows j0-2 .. j0+TJ+1, columns
+ * k0-2 .. k0+129) stream through a ring of three LDS planes; the first step
+ * is computed on the tile widened by one cell into a second ring of three
+ * planes, and the second step on the tile itself from that ring, stored into
+ * v.  Cells outside this PE's block are read straight from the neighbour's u
+ * through the symmetric heap (in the periodic 2x2x2 grid a coordinate past
+ * either end of an axis belongs to the other PE of that axis), so there is no
+ * halo exchange, and one barrier per launch (two steps) suffices: a launch
+ * only reads u and only writes v, on every PE.  The sums are those of the
+ * one-step kernels in the same order, so a launch gives the bits of two steps
+ * there.  Coordinates more than two cells past the block are clamped to valid
+ * addresses; they feed only cells that are not stored.
+ *
+ * Per step, HBM traffic is half that of the one-step kernels (u read once and
+ * v written once per two steps); each chunk re-reads two planes on either
+ * side and each tile two rows and two columns on either side, mostly from L2
+ * (tiles adjacent in j run on the same XCD). */
+/* Workgroup barrier over LDS only: waits for this wavefront's LDS (and
+ * scalar) accesses, then s_barrier; the "memory" clobber keeps the compiler
+ * from moving any access across it.  Global loads and stores stay in flight
+ * across it (__syncthreads() would wait for them too), so a plane's loads
+ * overlap the barriers of the plane before. */
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+/* a coordinate within two cells of an n-cell block (n >= 2), folded onto the
+ * PE that holds it: returns the local coordinate and sets `flip` when that is
+ * the other PE of the axis; coordinates further out are clamped first */
+__device__ __forceinline__ int fold2(int c, int n, int &flip) {
+    c = c < -1 ? -1 : (c > n + 2 ? n + 2 : c);
+    flip = 0;
+    if (c < 1) {
+        c += n;
+        flip = 1;
+    } else if (c > n) {
+        c -= n;
+        flip = 1;
+    }
+    return c;
+}
+
+/* one cell of either step: the update in its fixed order from the cell's
+ * x (planes), y (rows) and z (columns) neighbours */
+__device__ __forceinline__ double upd(double c, double xm, double xp, double ym, double yp, double zm, double zp) {
+    return 0.25 * c + 0.125 * (((((xm + xp) + ym) + yp) + zm) + zp);
+}
+
+template <int TJ>
+__global__ void __launch_bounds__(256) k_stencil_2s(const double *u_sym, double *v_sym, int n, int pe0, int chunk) {
+    constexpr int TK = 128;
+    constexpr int IR = TJ + 4; /* input rows: tile rows -2 .. TJ+1 */
+    constexpr int MR = TJ + 2; /* first-step rows: tile rows -1 .. TJ */
+    constexpr int LW = TK + 8; /* LDS row: tile column c at index c + 4 */
+    constexpr int RQ = IR / 4; /* input rows per wavefront */
+    static_assert(IR % 4 == 0 && TJ % 4 == 0 && 4 * IR <= 64 && 2 * MR <= 64, "tile rows over 4 wavefronts");
+    __shared__ double lin[3][IR][LW];
+    __shared__ double lmid[3][MR][LW];
+    const unsigned gx = gridDim.x, gy = gridDim.y, nwg = gx * gy * gridDim.z;
+    unsigned wl = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if (nwg % kXcds == 0) wl = (wl % kXcds) * (nwg / kXcds) + wl / kXcds;
+    const unsigned bx = wl % gx, by = (wl / gx) % gy, bz = wl / (gx * gy);
+    const geom g(n, pe0 + (int)bz);
+    const int jtiles = (n + TJ - 1) / TJ;
+    const int k0 = (int)bx * TK + 1;
+    const int j0 = (int)(by % (unsigned)jtiles) * TJ + 1;
+    const int i0 = (int)(by / (unsigned)jtiles) * chunk + 1;
+    const int i1 = i0 + chunk - 1 < n ? i0 + chunk - 1 : n;
+    double *__restrict__ v = (double *)nvshmemi::translate(v_sym, g.pe);
+    const size_t rs = g.row_stride(), ps = (size_t)(n + 2) * rs;
+    const int lane = (int)(threadIdx.x & 63u);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const bool vec = (n & 1) == 0; /* even rows: the lane's column pair is 16-byte aligned */
+    const int kc = k0 + 2 * lane;  /* the lane's first column of the main 128 */
+    const int ci = 2 * lane + 4;   /* its LDS index */
+    int kf0, kf1;                  /* its two columns folded, for rows read cell by cell */
+    const int kk0 = fold2(kc, n, kf0), kk1 = fold2(kc + 1, n, kf1);
+    const bool pair = vec && kc + 1 <= n;
+
+    /* main input rows: wavefront w loads rows w, w+4, .. (tile rows -2 + r),
+     * two columns per lane; a row's j fold is fixed for the whole walk */
+    int rj[RQ], rfj[RQ];
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) rj[q] = fold2(j0 - 2 + wv + 4 * q, n, rfj[q]);
+    /* side input cells: lanes 0..4*IR-1 of wavefront 3 load columns -2, -1,
+     * 128, 129 of every input row.  A side lane's row and column are fixed,
+     * so the PE holding its cell depends only on the plane's fold: both
+     * candidates are looked up once here (a per-lane lookup inside the walk
+     * would wait for every load in flight) */
+    const int sq = (int)threadIdx.x - 192;
+    const bool side_ld = sq >= 0 && sq < 4 * IR;
+    const int s_r = sq >> 2, s_c = (sq & 3) < 2 ? (sq & 3) - 2 : TK + (sq & 3) - 2;
+    const double *sb0 = u_sym, *sb1 = u_sym;
+    if (side_ld) {
+        int fj, fk;
+        const int jj = fold2(j0 - 2 + s_r, n, fj), kk = fold2(k0 + s_c, n, fk);
+        const int pyz = ((g.cy ^ fj) << 1) | (g.cz ^ fk);
+        const size_t off = g.idx(0, jj, kk);
+        sb0 = (const double *)nvshmemi::translate(u_sym, (g.cx << 2) | pyz) + off;
+        sb1 = (const double *)nvshmemi::translate(u_sym, ((g.cx ^ 1) << 2) | pyz) + off;
+    }
+    /* side first-step cells: lanes 0..2*MR-1 of wavefront 2 compute columns
+     * -1 and 128 of every first-step row */
+    const int tq = (int)threadIdx.x - 128;
+    const bool side_mid = tq >= 0 && tq < 2 * MR;
+    const int t_r = tq >> 1, t_c = (tq & 1) ? TK + 4 : 3;
+
+    /* Iteration m commits input plane m+1 (loaded one iteration earlier)
+     * into the input ring, starts loading plane m+2, computes the first step
+     * on plane m and the second step on plane m-1.  Ring slots: input planes
+     * m-1, m, m+1 in sa, sb, sc; first-step planes m-2, m-1, m in ma, mb, mc.
+     * A slot is rewritten only past the barrier that follows its last read. */
+    dbl2 pre[RQ];
+    double pres = 0.0;
+    int sa = 0, sb = 1, sc = 2, ma = 0, mb = 1, mc = 2;
+    for (int m = i0 - 4; m <= i1 + 1; ++m) {
+        if (m >= i0 - 3) {
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                lin[sc][wv + 4 * q][ci] = pre[q].x;
+                lin[sc][wv + 4 * q][ci + 1] = pre[q].y;
+            }
+            if (side_ld) lin[sc][s_r][s_c + 4] = pres;
+        }
+        lds_barrier();
+        if (m + 2 <= i1 + 2) {
+            int fi;
+            const int ii = fold2(m + 2, n, fi);
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int pxy = ((g.cx ^ fi) << 2) | ((g.cy ^ rfj[q]) << 1);
+                const size_t off = (size_t)ii * ps + (size_t)rj[q] * rs + kKOff;
+                const double *p0 = (const double *)nvshmemi::translate(u_sym, pxy | g.cz) + off;
+                if (pair) {
+                    pre[q] = *(const dbl2 *)(p0 + kc);
+                } else {
+                    const double *p1 = (const double *)nvshmemi::translate(u_sym, pxy | (g.cz ^ 1)) + off;
+                    pre[q].x = (kf0 ? p1 : p0)[kk0];
+                    pre[q].y = (kf1 ? p1 : p0)[kk1];
+                }
+            }
+            if (side_ld) pres = (fi ? sb1 : sb0)[(size_t)ii * ps];
+        }
+        if (m >= i0 - 1) {
+            for (int rm = wv; rm < MR; rm += 4) {
+                const int r = rm + 1;
+                const double x0 = lin[sb][r][ci], x1 = lin[sb][r][ci + 1];
+                lmid[mc][rm][ci] = upd(x0, lin[sa][r][ci], lin[sc][r][ci], lin[sb][r - 1][ci], lin[sb][r + 1][ci],
+                                       lin[sb][r][ci - 1], x1);
+                lmid[mc][rm][ci + 1] = upd(x1, lin[sa][r][ci + 1], lin[sc][r][ci + 1], lin[sb][r - 1][ci + 1],
+                                           lin[sb][r + 1][ci + 1], x0, lin[sb][r][ci + 2]);
+            }
+            if (side_mid) {
+                const int r = t_r + 1, c = t_c;
+                lmid[mc][t_r][c] = upd(lin[sb][r][c], lin[sa][r][c], lin[sc][r][c], lin[sb][r - 1][c],
+                                       lin[sb][r + 1][c], lin[sb][r][c - 1], lin[sb][r][c + 1]);
+            }
+        }
+        lds_barrier();
+        if (m >= i0 + 1) {
+#pragma unroll
+            for (int q = 0; q < TJ / 4; ++q) {
+                const int ro = wv * (TJ / 4) + q, r = ro + 1;
+                const double x0 = lmid[mb][r][ci], x1 = lmid[mb][r][ci + 1];
+                dbl2 val;
+                val.x = upd(x0, lmid[ma][r][ci], lmid[mc][r][ci], lmid[mb][r - 1][ci], lmid[mb][r + 1][ci],
+                            lmid[mb][r][ci - 1], x1);
+                val.y = upd(x1, lmid[ma][r][ci + 1], lmid[mc][r][ci + 1], lmid[mb][r - 1][ci + 1],
+                            lmid[mb][r + 1][ci + 1], x0, lmid[mb][r][ci + 2]);
+                const int j = j0 + ro;
+                if (j <= n) {
+                    double *dst = v + g.idx(m - 1, j, kc);
+                    if (pair) {
+                        __builtin_nontemporal_store(val, (dbl2 *)dst);
+                    } else {
+                        if (kc <= n) dst[0] = val.x;
+                        if (kc + 1 <= n) dst[1] = val.y;
+                    }
+                }
+            }
+        }
+        int t = sa;
+        sa = sb;
+        sb = sc;
+        sc = t;
+        t = ma;
+        ma = mb;
+        mb = mc;
+        mc = t;
+    }
+}
+
 __global__ void k_init(double *u_sym, int n, int pe0, uint64_t s0, uint64_t inc) {
     const geom g(n, pe0 + (int)blockIdx.z);
     const size_t cells = (size_t)n * n * n;
@@ -515,6 +707,21 @@ static void launch_stencil(double *u_sym, double *v_sym, double *recv_sym, int n
     else
         hipLaunchKernelGGL((k_stencil_t<FUSED, kTJ, kTK>), grid, dim3(256), 0, s, u_sym, v_sym, recv_sym, n, pe0, chunk,
                            pin, pout);
+    NVSHMEMI_HIP(hipGetLastError());
+}
+
+/* Planes per workgroup of the two-step kernel when the caller passes
+ * chunk <= 0: a chunk recomputes the first step on two planes past its ends
+ * and re-reads two input planes on either side, so it wants longer chunks
+ * than the one-step kernel.  Measured at 8 PEs x 256^3 in one launch (ms per
+ * step): chunk 16 0.265, 32 0.246, 64 0.238, 128 0.235, 256 0.234. */
+constexpr int kDefaultChunk2 = 64;
+
+static void launch_stencil2(double *u_sym, double *v_sym, int n, int pe0, int npes_here, int chunk, hipStream_t s) {
+    if (chunk <= 0) chunk = kDefaultChunk2;
+    const int jt = (n + kTJ - 1) / kTJ, ic = (n + chunk - 1) / chunk;
+    const dim3 grid((unsigned)((n + kTK - 1) / kTK), (unsigned)(jt * ic), (unsigned)npes_here);
+    hipLaunchKernelGGL((k_stencil_2s<kTJ>), grid, dim3(256), 0, s, u_sym, v_sym, n, pe0, chunk);
     NVSHMEMI_HIP(hipGetLastError());
 }
 
@@ -604,6 +811,42 @@ double *nvshmemh_jacobi_run(double *u_sym, double *v_sym, double *recv_sym, int 
         a = b;
         b = tmp;
     }
+    return a;
+}
+
+/* Two steps in one launch (u -> v) for every PE of this process, reading
+ * the cells past this PE's block straight from the neighbours' u; every PE's
+ * u must be complete before it starts (a barrier between processes). */
+void nvshmemh_jacobi_stencil2(double *u_sym, double *v_sym, int n, int pe0, int npes_here, int chunk, hipStream_t s) {
+    if (nvshmem_n_pes() != 8) NVSHMEMI_FATAL("jacobi needs exactly 8 PEs (2x2x2), found %d", nvshmem_n_pes());
+    if (pe0 < 0 || npes_here < 1 || pe0 + npes_here > 8) NVSHMEMI_FATAL("jacobi: PEs %d..%d outside 0..7", pe0, pe0 + npes_here - 1);
+    check_n(n);
+    launch_stencil2(u_sym, v_sym, n, pe0, npes_here, chunk, s);
+}
+
+/* `steps` steps, two per launch, each launch followed by the barrier
+ * (nvshmem_barrier_all_offload between processes, the kernel boundary in
+ * local mode); an odd last step is one fused step (pack+put of the faces,
+ * barrier, fused stencil, barrier).  No halo is exchanged for the two-step
+ * launches.  Returns the buffer holding the result. */
+double *nvshmemh_jacobi_run2(double *u_sym, double *v_sym, double *recv_sym, int n, int steps, int chunk,
+                             hipStream_t s) {
+    const bool local = nvshmemx_mode() == NVSHMEMX_MODE_LOCAL;
+    const int pe0 = local ? 0 : nvshmem_my_pe();
+    const int here = local ? nvshmem_n_pes() : 1;
+    if (nvshmem_n_pes() != 8) NVSHMEMI_FATAL("jacobi needs exactly 8 PEs (2x2x2), found %d", nvshmem_n_pes());
+    check_n(n);
+    double *a = u_sym, *b = v_sym;
+    if (steps <= 0) return a;
+    if (steps >= 2 && !local) nvshmemx_barrier_all_on_stream(s);
+    for (int t = 0; t + 1 < steps; t += 2) {
+        launch_stencil2(a, b, n, pe0, here, chunk, s);
+        if (!local) nvshmemx_barrier_all_on_stream(s);
+        double *tmp = a;
+        a = b;
+        b = tmp;
+    }
+    if (steps & 1) a = nvshmemh_jacobi_run(a, b, recv_sym, n, 1, 0, 0, s);
     return a;
 }
 

@@ -192,6 +192,8 @@ SIGNATURES.update({
     "nvshmemh_jacobi_run": (_P, [_P, _P, _P, _I, _I, _I, _I, _P]),
     "nvshmemh_jacobi_pack_put_api": (None, [_P, _P, _I, _I, _I, _I, _P]),
     "nvshmemh_jacobi_run_separate": (_P, [_P, _P, _P, _I, _I, _I, _I, _P]),
+    "nvshmemh_jacobi_stencil2": (None, [_P, _P, _I, _I, _I, _I, _P]),
+    "nvshmemh_jacobi_run2": (_P, [_P, _P, _P, _I, _I, _I, _P]),
 })
 
 

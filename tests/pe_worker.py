@@ -12,8 +12,9 @@ Scenarios (every PE runs the same collective sequence):
            is its own heap (targets in the other processes' IPC-mapped heaps),
            launches it and replays it from a hipGraph; each PE's targets are
            checked bit-exact against the oracle's batch
-  jacobi   the 2x2x2 Jacobi application, one PE per process (fused steps, or
-           unfused ones whose faces go through the device put API)
+  jacobi   the 2x2x2 Jacobi application, one PE per process (fused steps,
+           unfused ones whose faces go through the device put API, or two
+           steps per launch reading the neighbours' fields)
   config2  2 PEs: int/float/double x put/get x 8 B .. 2^max_log2 B through the
            host-launched grid copy and the device per-thread calls, each
            landing checked against the oracle with guard bytes
@@ -259,6 +260,8 @@ def main():
         lib.nvshmem_barrier_all()
         if how == "fused":
             res = lib.nvshmemh_jacobi_run(u, v, recv, n, steps, 0, 0, None)
+        elif how == "two_step":  # two steps per launch, no halo exchange
+            res = lib.nvshmemh_jacobi_run2(u, v, recv, n, steps, 0, None)
         else:  # unfused steps, faces through the library's device puts
             res = lib.nvshmemh_jacobi_run_separate(u, v, recv, n, steps, 0, 1, None)
         hip.sync()
