@@ -370,7 +370,9 @@ def jacobi_worker(layout, name=None, counter_pass=False):
     PE, 100 steps; each step is one fused stencil launch (halo read from this
     PE's receive slots, boundary values written into the neighbours' slots of
     the next parity) and the barrier (kernel boundary in local mode,
-    nvshmem_barrier_all_offload between processes).  Prints one JSON line
+    nvshmem_barrier_all_offload between processes).  The same 100 steps are
+    then timed two per launch (cells past a block read from the neighbours'
+    fields), and the unfused steps for 20 steps each.  Prints one JSON line
     (PE 0)."""
     import numpy as np
     import nvshmem_amd as na
@@ -400,17 +402,35 @@ def jacobi_worker(layout, name=None, counter_pass=False):
     lib.nvshmem_barrier_all()
     # planes per workgroup (0: the library's default, 6)
     chunk = int(os.environ.get("JACOBI_CHUNK", "0"))
-    if counter_pass:  # five steps and one 1 GiB device copy (the counters' scale)
+    chunk2 = int(os.environ.get("JACOBI_CHUNK2", "0"))
+    if counter_pass:  # five steps, two two-step launches and one 1 GiB device copy (the counters' scale)
         lib.nvshmemh_jacobi_run(u, v, recv, n, 5, chunk, 0, None)
+        lib.nvshmemh_jacobi_run2(u, v, recv, n, 4, chunk2, None)
         ref = hip.malloc(2 << 30)
         hip.check(hip.lib.hipMemcpyAsync(ref + (1 << 30), ref, 1 << 30, hip.D2D, None))
         hip.sync()
         lib.nvstop_pes()
         return
-    ms = timed(hip, lambda: lib.nvshmemh_jacobi_run(u, v, recv, n, JACOBI_STEPS, chunk, 0, None), 1, 0) / JACOBI_STEPS
+    def warm_then_time(run):
+        """one untimed run (the device leaves the clocks of an idle spell:
+        the first few milliseconds of a run from idle go up to 30% slower),
+        the initial field again, then the timed run"""
+        run()
+        lib.nvshmemh_jacobi_init(u, n, pe0, len(pes_here), 42, None)
+        hip.sync()
+        lib.nvshmem_barrier_all()
+        return timed(hip, run, 1, 0) / JACOBI_STEPS
+
+    ms = warm_then_time(lambda: lib.nvshmemh_jacobi_run(u, v, recv, n, JACOBI_STEPS, chunk, 0, None))
     res = u if JACOBI_STEPS % 2 == 0 else v
     lib.nvshmem_barrier_all()
     s1 = total(res)
+    # the same steps two per launch (no halo exchange: the cells past a
+    # block are read straight from the neighbours' fields), from the same
+    # initial field; the result must carry the same bits, so the same sum
+    ms2 = warm_then_time(lambda: lib.nvshmemh_jacobi_run2(u, v, recv, n, JACOBI_STEPS, chunk2, None))
+    lib.nvshmem_barrier_all()
+    s2 = total(u if JACOBI_STEPS % 2 == 0 else v)
     # unfused steps (pack+put, barrier, unpack, stencil), the faces stored
     # directly or sent through the library's contiguous and strided device
     # puts; an even number of steps leaves the result back in u
@@ -421,7 +441,9 @@ def jacobi_worker(layout, name=None, counter_pass=False):
     lib.nvshmem_barrier_all()
     err = lib.nvshmemx_get_last_error()
     out = {"layout": layout, "pes": 8, "cells_per_pe": n ** 3, "steps": JACOBI_STEPS, "chunk": chunk or "library default", "ms_per_step": ms,
-           "sum_before": s0, "sum_after": s1, "unfused_ms_per_step": sep, "errors": err}
+           "sum_before": s0, "sum_after": s1, "unfused_ms_per_step": sep,
+           "two_step": {"ms_per_step": ms2, "chunk": chunk2 or "library default", "sum_matches_fused": s2 == s1},
+           "errors": err}
     lib.nvstop_pes()
     if pe0 == 0:
         print(json.dumps(out), flush=True)
@@ -435,6 +457,9 @@ def with_copy_bound(jacobi, copy_bound):
             if isinstance(r, dict) and "hbm_gbps" in r:
                 r["frac_of_peak"] = round(r["hbm_gbps"] / PEAK_HBM_GBPS, 4)
                 r["frac_of_copy_bound"] = round(r["hbm_gbps"] / copy_bound, 4)
+            t2 = r.get("two_step") if isinstance(r, dict) else None
+            if isinstance(t2, dict) and "launch_hbm_gbps" in t2:
+                t2["launch_frac_of_peak"] = round(t2["launch_hbm_gbps"] / PEAK_HBM_GBPS, 4)
     return jacobi
 
 
@@ -468,6 +493,13 @@ def run_jacobi_leg():
             halo = 8 * 6 * JACOBI_N ** 2 * 8 * 2
             r["hbm_gbps"] = round((cells * 16 + halo) / (r["ms_per_step"] * 1e-3) / 1e9, 1)
             r["ms_per_step"] = round(r["ms_per_step"], 4)
+            t2 = r.get("two_step")
+            if isinstance(t2, dict) and t2.get("ms_per_step"):
+                # one read of u and one write of v per cell per launch of two steps
+                t2["ms_per_launch"] = round(2 * t2["ms_per_step"], 4)
+                t2["launch_hbm_gbps"] = round(cells * 16 / (2 * t2["ms_per_step"] * 1e-3) / 1e9, 1)
+                t2["speedup_vs_fused"] = round(r["ms_per_step"] / t2["ms_per_step"], 3)
+                t2["ms_per_step"] = round(t2["ms_per_step"], 4)
             if k == "local":  # all 8 PEs' cells: the stencil conserves the total up to rounding
                 r["rel_sum_drift"] = abs(r["sum_after"] - r["sum_before"]) / abs(r["sum_before"])
             r.pop("sum_before", None)

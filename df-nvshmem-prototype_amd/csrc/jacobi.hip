@@ -468,25 +468,6 @@ __global__ void __launch_bounds__(256) k_stencil_v(const double *u_sym, double *
     }
 }
 
-/* Two steps per launch (temporal blocking).  A workgroup owns a TJ x 128
- * tile of rows j and columns k over a chunk of planes i and walks the planes:
- * u's cells within two of the tile (rows j0-2 .. j0+TJ+1, columns
- * k0-2 .. k0+129) stream through a ring of three LDS planes; the first step
- * is computed on the tile widened by one cell into a second ring of three
- * planes, and the second step on the tile itself from that ring, stored into
- * v.  Cells outside this PE's block are read straight from the neighbour's u
- * through the symmetric heap (in the periodic 2x2x2 grid a coordinate past
- * either end of an axis belongs to the other PE of that axis), so there is no
- * halo exchange, and one barrier per launch (two steps) suffices: a launch
- * only reads u and only writes v, on every PE.  The sums are those of the
- * one-step kernels in the same order, so a launch gives the bits of two steps
- * there.  Coordinates more than two cells past the block are clamped to valid
- * addresses; they feed only cells that are not stored.
- *
- * Per step, HBM traffic is half that of the one-step kernels (u read once and
- * v written once per two steps); each chunk re-reads two planes on either
- * side and each tile two rows and two columns on either side, mostly from L2
- * (tiles adjacent in j run on the same XCD). */
 /* Workgroup barrier over LDS only: waits for this wavefront's LDS (and
  * scalar) accesses, then s_barrier; the "memory" clobber keeps the compiler
  * from moving any access across it.  Global loads and stores stay in flight
@@ -516,6 +497,38 @@ __device__ __forceinline__ double upd(double c, double xm, double xp, double ym,
     return 0.25 * c + 0.125 * (((((xm + xp) + ym) + yp) + zm) + zp);
 }
 
+/* Two steps per launch (temporal blocking).  A workgroup owns a TJ x 128
+ * tile of rows j and columns k over a chunk of planes i and walks the planes.
+ * u's cells within two of the tile (rows j0-2 .. j0+TJ+1, columns
+ * k0-2 .. k0+129) stream in one plane at a time; the first step is computed
+ * on the tile widened by one cell, the second on the tile itself, stored into
+ * v.  Cells outside this PE's block are read straight from the neighbour's u
+ * through the symmetric heap (in the periodic 2x2x2 grid a coordinate past
+ * either end of an axis belongs to the other PE of that axis), so there is no
+ * halo exchange, and one barrier per launch (two steps) suffices: a launch
+ * only reads u and only writes v, on every PE.  The sums are those of the
+ * one-step kernels in the same order, so a launch gives the bits of two steps
+ * there.  Coordinates more than two cells past the block are clamped to valid
+ * addresses; they feed only cells that are not stored.
+ *
+ * Each lane keeps its own cells in registers across planes: the x (plane)
+ * neighbours of both steps come from registers, and two LDS rings of two
+ * slots hold the input plane and the first-step plane whose in-plane (y and
+ * z) neighbours are read (47.9 KB: three workgroups per CU).  Wavefront w
+ * loads input rows (w + 1 + 4q) mod IR (tile rows - 2), computes the first
+ * step on the rows just above them and the second step on the rows above
+ * those, so every row's history stays with the lanes that loaded it; lanes
+ * of wavefront 3 load the side columns and compute the first step on
+ * columns -1 and 128.
+ *
+ * Per step, HBM traffic is half that of the one-step kernels (u read once and
+ * v written once per two steps); each chunk re-reads two planes on either
+ * side and each tile two rows and two columns on either side, mostly from L2
+ * (tiles adjacent in j run on the same XCD).  At 8 PEs x 256^3 this took
+ * 0.210-0.226 ms per step against 0.233-0.238 for the same walk with
+ * three-plane LDS rings and every neighbour read from LDS (two workgroups
+ * per CU), and 0.284-0.37 with 4-row tiles (three per CU, more re-read
+ * rows). */
 template <int TJ>
 __global__ void __launch_bounds__(256) k_stencil_2s(const double *u_sym, double *v_sym, int n, int pe0, int chunk) {
     constexpr int TK = 128;
@@ -523,9 +536,9 @@ __global__ void __launch_bounds__(256) k_stencil_2s(const double *u_sym, double 
     constexpr int MR = TJ + 2; /* first-step rows: tile rows -1 .. TJ */
     constexpr int LW = TK + 8; /* LDS row: tile column c at index c + 4 */
     constexpr int RQ = IR / 4; /* input rows per wavefront */
-    static_assert(IR % 4 == 0 && TJ % 4 == 0 && 4 * IR <= 64 && 2 * MR <= 64, "tile rows over 4 wavefronts");
-    __shared__ double lin[3][IR][LW];
-    __shared__ double lmid[3][MR][LW];
+    static_assert(IR % 4 == 0 && TJ % 4 == 0 && 4 * IR <= 64, "tile rows over 4 wavefronts");
+    __shared__ double lin[2][IR][LW];
+    __shared__ double lmid[2][MR][LW];
     const unsigned gx = gridDim.x, gy = gridDim.y, nwg = gx * gy * gridDim.z;
     unsigned wl = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
     if (nwg % kXcds == 0) wl = (wl % kXcds) * (nwg / kXcds) + wl / kXcds;
@@ -540,26 +553,25 @@ __global__ void __launch_bounds__(256) k_stencil_2s(const double *u_sym, double 
     const size_t rs = g.row_stride(), ps = (size_t)(n + 2) * rs;
     const int lane = (int)(threadIdx.x & 63u);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const bool vec = (n & 1) == 0; /* even rows: the lane's column pair is 16-byte aligned */
-    const int kc = k0 + 2 * lane;  /* the lane's first column of the main 128 */
-    const int ci = 2 * lane + 4;   /* its LDS index */
-    int kf0, kf1;                  /* its two columns folded, for rows read cell by cell */
+    const bool vec = (n & 1) == 0;
+    const int kc = k0 + 2 * lane;
+    const int ci = 2 * lane + 4;
+    int kf0, kf1;
     const int kk0 = fold2(kc, n, kf0), kk1 = fold2(kc + 1, n, kf1);
     const bool pair = vec && kc + 1 <= n;
 
-    /* main input rows: wavefront w loads rows w, w+4, .. (tile rows -2 + r),
-     * two columns per lane; a row's j fold is fixed for the whole walk */
-    int rj[RQ], rfj[RQ];
+    int ir[RQ], rj[RQ], rfj[RQ];
 #pragma unroll
-    for (int q = 0; q < RQ; ++q) rj[q] = fold2(j0 - 2 + wv + 4 * q, n, rfj[q]);
-    /* side input cells: lanes 0..4*IR-1 of wavefront 3 load columns -2, -1,
-     * 128, 129 of every input row.  A side lane's row and column are fixed,
-     * so the PE holding its cell depends only on the plane's fold: both
-     * candidates are looked up once here (a per-lane lookup inside the walk
-     * would wait for every load in flight) */
-    const int sq = (int)threadIdx.x - 192;
-    const bool side_ld = sq >= 0 && sq < 4 * IR;
-    const int s_r = sq >> 2, s_c = (sq & 3) < 2 ? (sq & 3) - 2 : TK + (sq & 3) - 2;
+    for (int q = 0; q < RQ; ++q) {
+        ir[q] = (wv + 1 + 4 * q) % IR;
+        rj[q] = fold2(j0 - 2 + ir[q], n, rfj[q]);
+    }
+    /* side input cells: lanes 0..4*IR-1 of wavefront 3, columns -2, -1, 128,
+     * 129 (lane & 3) of input row lane >> 2; the lanes on columns -1 and 128
+     * of rows 1..IR-2 also compute the first step there */
+    const bool side_ld = wv == 3 && lane < 4 * IR;
+    const int s_r = lane >> 2, s_c = (lane & 3) < 2 ? (lane & 3) - 2 : TK + (lane & 3) - 2;
+    const bool side_mid = side_ld && ((lane & 3) == 1 || (lane & 3) == 2) && s_r >= 1 && s_r <= IR - 2;
     const double *sb0 = u_sym, *sb1 = u_sym;
     if (side_ld) {
         int fj, fk;
@@ -569,28 +581,30 @@ __global__ void __launch_bounds__(256) k_stencil_2s(const double *u_sym, double 
         sb0 = (const double *)nvshmemi::translate(u_sym, (g.cx << 2) | pyz) + off;
         sb1 = (const double *)nvshmemi::translate(u_sym, ((g.cx ^ 1) << 2) | pyz) + off;
     }
-    /* side first-step cells: lanes 0..2*MR-1 of wavefront 2 compute columns
-     * -1 and 128 of every first-step row */
-    const int tq = (int)threadIdx.x - 128;
-    const bool side_mid = tq >= 0 && tq < 2 * MR;
-    const int t_r = tq >> 1, t_c = (tq & 1) ? TK + 4 : 3;
 
-    /* Iteration m commits input plane m+1 (loaded one iteration earlier)
-     * into the input ring, starts loading plane m+2, computes the first step
-     * on plane m and the second step on plane m-1.  Ring slots: input planes
-     * m-1, m, m+1 in sa, sb, sc; first-step planes m-2, m-1, m in ma, mb, mc.
-     * A slot is rewritten only past the barrier that follows its last read. */
-    dbl2 pre[RQ];
-    double pres = 0.0;
-    int sa = 0, sb = 1, sc = 2, ma = 0, mb = 1, mc = 2;
+    /* histories: input planes m-1, m, m+1 (xm, x0, xp) of the lane's rows,
+     * first-step planes m-2, m-1, m (ym, y0, yp) */
+    dbl2 pre[RQ], xm[RQ], x0[RQ], xp[RQ], ym[RQ], y0[RQ], yp[RQ];
+    double pres = 0.0, sxm = 0.0, sx0 = 0.0, sxp = 0.0;
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) pre[q] = xm[q] = x0[q] = xp[q] = ym[q] = y0[q] = yp[q] = dbl2{0.0, 0.0};
     for (int m = i0 - 4; m <= i1 + 1; ++m) {
-        if (m >= i0 - 3) {
+        if (m >= i0 - 3) { /* input plane m+1 arrived: into the histories and LDS */
+            const int sl = (m + 1) & 1;
 #pragma unroll
             for (int q = 0; q < RQ; ++q) {
-                lin[sc][wv + 4 * q][ci] = pre[q].x;
-                lin[sc][wv + 4 * q][ci + 1] = pre[q].y;
+                xm[q] = x0[q];
+                x0[q] = xp[q];
+                xp[q] = pre[q];
+                lin[sl][ir[q]][ci] = pre[q].x;
+                lin[sl][ir[q]][ci + 1] = pre[q].y;
             }
-            if (side_ld) lin[sc][s_r][s_c + 4] = pres;
+            if (side_ld) {
+                sxm = sx0;
+                sx0 = sxp;
+                sxp = pres;
+                lin[sl][s_r][s_c + 4] = pres;
+            }
         }
         lds_barrier();
         if (m + 2 <= i1 + 2) {
@@ -611,52 +625,53 @@ __global__ void __launch_bounds__(256) k_stencil_2s(const double *u_sym, double 
             }
             if (side_ld) pres = (fi ? sb1 : sb0)[(size_t)ii * ps];
         }
-        if (m >= i0 - 1) {
-            for (int rm = wv; rm < MR; rm += 4) {
-                const int r = rm + 1;
-                const double x0 = lin[sb][r][ci], x1 = lin[sb][r][ci + 1];
-                lmid[mc][rm][ci] = upd(x0, lin[sa][r][ci], lin[sc][r][ci], lin[sb][r - 1][ci], lin[sb][r + 1][ci],
-                                       lin[sb][r][ci - 1], x1);
-                lmid[mc][rm][ci + 1] = upd(x1, lin[sa][r][ci + 1], lin[sc][r][ci + 1], lin[sb][r - 1][ci + 1],
-                                           lin[sb][r + 1][ci + 1], x0, lin[sb][r][ci + 2]);
+        if (m >= i0 - 1) { /* first step on plane m */
+            const int sl = m & 1;
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int r = ir[q];
+                ym[q] = y0[q];
+                y0[q] = yp[q];
+                if (r >= 1 && r <= IR - 2) {
+                    const double a = x0[q].x, b = x0[q].y;
+                    yp[q].x = upd(a, xm[q].x, xp[q].x, lin[sl][r - 1][ci], lin[sl][r + 1][ci], lin[sl][r][ci - 1], b);
+                    yp[q].y = upd(b, xm[q].y, xp[q].y, lin[sl][r - 1][ci + 1], lin[sl][r + 1][ci + 1], a,
+                                  lin[sl][r][ci + 2]);
+                    lmid[sl][r - 1][ci] = yp[q].x;
+                    lmid[sl][r - 1][ci + 1] = yp[q].y;
+                }
             }
             if (side_mid) {
-                const int r = t_r + 1, c = t_c;
-                lmid[mc][t_r][c] = upd(lin[sb][r][c], lin[sa][r][c], lin[sc][r][c], lin[sb][r - 1][c],
-                                       lin[sb][r + 1][c], lin[sb][r][c - 1], lin[sb][r][c + 1]);
+                const int c = s_c + 4;
+                lmid[sl][s_r - 1][c] = upd(sx0, sxm, sxp, lin[sl][s_r - 1][c], lin[sl][s_r + 1][c], lin[sl][s_r][c - 1],
+                                           lin[sl][s_r][c + 1]);
             }
         }
         lds_barrier();
-        if (m >= i0 + 1) {
+        if (m >= i0 + 1) { /* second step on plane m-1 */
+            const int sl = (m - 1) & 1;
 #pragma unroll
-            for (int q = 0; q < TJ / 4; ++q) {
-                const int ro = wv * (TJ / 4) + q, r = ro + 1;
-                const double x0 = lmid[mb][r][ci], x1 = lmid[mb][r][ci + 1];
-                dbl2 val;
-                val.x = upd(x0, lmid[ma][r][ci], lmid[mc][r][ci], lmid[mb][r - 1][ci], lmid[mb][r + 1][ci],
-                            lmid[mb][r][ci - 1], x1);
-                val.y = upd(x1, lmid[ma][r][ci + 1], lmid[mc][r][ci + 1], lmid[mb][r - 1][ci + 1],
-                            lmid[mb][r + 1][ci + 1], x0, lmid[mb][r][ci + 2]);
-                const int j = j0 + ro;
-                if (j <= n) {
-                    double *dst = v + g.idx(m - 1, j, kc);
-                    if (pair) {
-                        __builtin_nontemporal_store(val, (dbl2 *)dst);
-                    } else {
-                        if (kc <= n) dst[0] = val.x;
-                        if (kc + 1 <= n) dst[1] = val.y;
+            for (int q = 0; q < RQ; ++q) {
+                const int r = ir[q] - 1, ro = r - 1; /* first-step row, tile row */
+                if (ro >= 0 && ro < TJ) {
+                    const double a = y0[q].x, b = y0[q].y;
+                    dbl2 val;
+                    val.x = upd(a, ym[q].x, yp[q].x, lmid[sl][r - 1][ci], lmid[sl][r + 1][ci], lmid[sl][r][ci - 1], b);
+                    val.y = upd(b, ym[q].y, yp[q].y, lmid[sl][r - 1][ci + 1], lmid[sl][r + 1][ci + 1], a,
+                                lmid[sl][r][ci + 2]);
+                    const int j = j0 + ro;
+                    if (j <= n) {
+                        double *dst = v + g.idx(m - 1, j, kc);
+                        if (pair) {
+                            __builtin_nontemporal_store(val, (dbl2 *)dst);
+                        } else {
+                            if (kc <= n) dst[0] = val.x;
+                            if (kc + 1 <= n) dst[1] = val.y;
+                        }
                     }
                 }
             }
         }
-        int t = sa;
-        sa = sb;
-        sb = sc;
-        sc = t;
-        t = ma;
-        ma = mb;
-        mb = mc;
-        mc = t;
     }
 }
 
@@ -711,17 +726,24 @@ static void launch_stencil(double *u_sym, double *v_sym, double *recv_sym, int n
 }
 
 /* Planes per workgroup of the two-step kernel when the caller passes
- * chunk <= 0: a chunk recomputes the first step on two planes past its ends
- * and re-reads two input planes on either side, so it wants longer chunks
- * than the one-step kernel.  Measured at 8 PEs x 256^3 in one launch (ms per
- * step): chunk 16 0.265, 32 0.246, 64 0.238, 128 0.235, 256 0.234. */
-constexpr int kDefaultChunk2 = 64;
+ * chunk <= 0: a third of the column.  A chunk recomputes the first step on
+ * two planes past its ends and re-reads two input planes on either side, so
+ * it wants longer chunks than the one-step kernel, and the workgroup count
+ * should fill whole rounds of the 768 resident slots (three per CU).
+ * Measured at 8 PEs x 256^3 in one launch (ms per step): chunk 43 0.218,
+ * 52 0.238, 64 0.224, 86 0.210, 96 0.225, 128 0.283 (1024 workgroups: one
+ * round and a third), 172 0.213, 256 0.219; as 8 processes 64 0.271, 86
+ * 0.268, 256 0.325. */
+static int default_chunk2(int n) { return (n + 2) / 3; }
+
+/* Tile rows of the two-step kernel (the tile is 128 columns wide). */
+constexpr int kTJ2 = 8;
 
 static void launch_stencil2(double *u_sym, double *v_sym, int n, int pe0, int npes_here, int chunk, hipStream_t s) {
-    if (chunk <= 0) chunk = kDefaultChunk2;
-    const int jt = (n + kTJ - 1) / kTJ, ic = (n + chunk - 1) / chunk;
+    if (chunk <= 0) chunk = default_chunk2(n);
+    const int jt = (n + kTJ2 - 1) / kTJ2, ic = (n + chunk - 1) / chunk;
     const dim3 grid((unsigned)((n + kTK - 1) / kTK), (unsigned)(jt * ic), (unsigned)npes_here);
-    hipLaunchKernelGGL((k_stencil_2s<kTJ>), grid, dim3(256), 0, s, u_sym, v_sym, n, pe0, chunk);
+    hipLaunchKernelGGL((k_stencil_2s<kTJ2>), grid, dim3(256), 0, s, u_sym, v_sym, n, pe0, chunk);
     NVSHMEMI_HIP(hipGetLastError());
 }
 
