@@ -97,6 +97,13 @@ struct geom {
     }
 };
 
+/* Workgroup barrier over LDS only: waits for this wavefront's LDS (and
+ * scalar) accesses, then s_barrier; the "memory" clobber keeps the compiler
+ * from moving any access across it.  Global loads and stores stay in flight
+ * across it (__syncthreads() would wait for them too), so a plane's loads
+ * overlap the barriers of the plane before. */
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 /* cell (a, b) of face d: interior plane on side d (send) or halo plane (recv) */
 __device__ __forceinline__ void face_cell(const geom &g, int d, int a, int b, bool halo, int &i, int &j, int &k) {
     const int n = g.n;
@@ -468,12 +475,6 @@ __global__ void __launch_bounds__(256) k_stencil_v(const double *u_sym, double *
     }
 }
 
-/* Workgroup barrier over LDS only: waits for this wavefront's LDS (and
- * scalar) accesses, then s_barrier; the "memory" clobber keeps the compiler
- * from moving any access across it.  Global loads and stores stay in flight
- * across it (__syncthreads() would wait for them too), so a plane's loads
- * overlap the barriers of the plane before. */
-__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 /* a coordinate within two cells of an n-cell block (n >= 2), folded onto the
  * PE that holds it: returns the local coordinate and sets `flip` when that is
