@@ -12,7 +12,11 @@ with HIP events on the launch stream.
 
 Also reported on the same line: get GB/s, p+wait round-trip p50/p99 between
 the two processes, the hipMemcpy device-to-device ceiling, the roofline object
-and the CPU baseline (the same put semantics as host MPI-3 RMA on 2 cores).
+and the CPU baseline (the same put semantics as host MPI-3 RMA on 2 cores);
+the 4096-put batch (fused, tiles, persistent, posted, per-call), the learned
+plan choice, the 8 B .. 1 GiB sweep, and the 2x2x2 Jacobi application at
+256^3 per PE (one fused step per launch, two steps per launch, unfused
+steps; one process and 8 processes).
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--layout process|local]
 """
