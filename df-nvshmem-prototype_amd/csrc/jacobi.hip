@@ -590,6 +590,11 @@ __global__ void __launch_bounds__(256) k_stencil_2s(const double *u_sym, double 
 #pragma unroll
     for (int q = 0; q < RQ; ++q) pre[q] = xm[q] = x0[q] = xp[q] = ym[q] = y0[q] = yp[q] = dbl2{0.0, 0.0};
     for (int m = i0 - 4; m <= i1 + 1; ++m) {
+        /* the one barrier of an iteration: past it, slot (m+1)&1 of the input
+         * ring (last read for plane m-1) and slot m&1 of the first-step ring
+         * (last read for plane m-2) may be rewritten, and what the previous
+         * iteration wrote (input plane m, first-step plane m-1) is visible */
+        lds_barrier();
         if (m >= i0 - 3) { /* input plane m+1 arrived: into the histories and LDS */
             const int sl = (m + 1) & 1;
 #pragma unroll
@@ -607,7 +612,6 @@ __global__ void __launch_bounds__(256) k_stencil_2s(const double *u_sym, double 
                 lin[sl][s_r][s_c + 4] = pres;
             }
         }
-        lds_barrier();
         if (m + 2 <= i1 + 2) {
             int fi;
             const int ii = fold2(m + 2, n, fi);
@@ -648,7 +652,6 @@ __global__ void __launch_bounds__(256) k_stencil_2s(const double *u_sym, double 
                                            lin[sl][s_r][c + 1]);
             }
         }
-        lds_barrier();
         if (m >= i0 + 1) { /* second step on plane m-1 */
             const int sl = (m - 1) & 1;
 #pragma unroll
